@@ -1,0 +1,342 @@
+"""dendro4_amd -- MI355X-native octree-FEM hot path of Dendro4.
+
+Python view of the C ABI in include/dendro_mi355x.h (libdendro_mi355x.so,
+built for gfx950 by dendro4_amd.build).  The names mirror the reference's C++
+API on one rank:
+
+    ot::points2Octree        -> points2octree(pts, max_depth, max_num_pts)
+    ot::balanceOctree        -> balance_octree(octree, inc_corner=True)
+    par::sampleSort          -> sample_sort(treenodes)
+    ot::DA                   -> DA(octree)
+    Jacobian1MatMult         -> DA.matvec(x, y)
+
+There is no CPU fallback: importing works without a GPU (for the build check),
+but every call needs the library and a device, and raises otherwise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libdendro_mi355x.so")
+
+CURVE_HILBERT = 0
+CURVE_MORTON = 1
+# ot::TreeNode flags (include/TreeNode.h:79-81, include/oda/odaUtils.h:116-118)
+FLAG_BOUNDARY = 64
+FLAG_NODE = 128
+
+EXPORTED = [
+    "dgx_init", "dgx_last_error", "dgx_stream", "dgx_synchronize", "dgx_device_malloc", "dgx_device_free",
+    "dgx_memcpy_htod", "dgx_memcpy_dtoh", "dgx_memset_device", "dgx_points_to_octree", "dgx_balance_octree",
+    "dgx_octree_import", "dgx_octree_size", "dgx_octree_max_depth", "dgx_octree_export", "dgx_octree_stats",
+    "dgx_octree_free", "dgx_da_create", "dgx_da_get_info", "dgx_da_export", "dgx_da_matvec", "dgx_da_matvec_f32",
+    "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points",
+]
+
+
+class DgxError(RuntimeError):
+    pass
+
+
+class _DaInfo(C.Structure):
+    _fields_ = [
+        ("local_buffer_size", C.c_uint64),
+        ("element_size", C.c_uint64),
+        ("node_size", C.c_uint64),
+        ("boundary_node_size", C.c_uint64),
+        ("n_nodes", C.c_uint64),
+        ("max_depth", C.c_uint32),
+        ("ghost_scans", C.c_uint32),
+    ]
+
+
+_lib = None
+_ready = False
+
+
+def load_library():
+    """Load libdendro_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DgxError(f"{LIB_PATH} is missing: run `python -m dendro4_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, u32, i32, u64, d = C.c_void_p, C.c_size_t, C.c_uint32, C.c_int, C.c_uint64, C.c_double
+    sig = {
+        "dgx_init": ([i32], i32),
+        "dgx_last_error": ([], C.c_char_p),
+        "dgx_stream": ([], vp),
+        "dgx_synchronize": ([], i32),
+        "dgx_device_malloc": ([C.POINTER(vp), sz], i32),
+        "dgx_device_free": ([vp], i32),
+        "dgx_memcpy_htod": ([vp, vp, sz], i32),
+        "dgx_memcpy_dtoh": ([vp, vp, sz], i32),
+        "dgx_memset_device": ([vp, i32, sz], i32),
+        "dgx_points_to_octree": ([vp, sz, u32, u32, i32, i32, C.POINTER(vp)], i32),
+        "dgx_balance_octree": ([vp, i32, C.POINTER(vp)], i32),
+        "dgx_octree_import": ([vp, sz, u32, i32, i32, i32, C.POINTER(vp)], i32),
+        "dgx_octree_size": ([vp, C.POINTER(sz)], i32),
+        "dgx_octree_max_depth": ([vp, C.POINTER(u32)], i32),
+        "dgx_octree_export": ([vp, vp, i32], i32),
+        "dgx_octree_stats": ([vp, C.POINTER(u64)], i32),
+        "dgx_octree_free": ([vp], i32),
+        "dgx_da_create": ([vp, C.POINTER(vp)], i32),
+        "dgx_da_get_info": ([vp, C.POINTER(_DaInfo)], i32),
+        "dgx_da_export": ([vp, vp, vp, vp, vp], i32),
+        "dgx_da_matvec": ([vp, vp, vp, d, d], i32),
+        "dgx_da_matvec_f32": ([vp, vp, vp, C.c_float, C.c_float], i32),
+        "dgx_da_diagonal": ([vp, vp, d, d], i32),
+        "dgx_da_free": ([vp], i32),
+        "dgx_gen_points": ([i32, u64, sz, i32, d, d, d, d, i32, vp], i32),
+        "dgx_sfc_tables": ([i32, vp, vp, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise DgxError(load_library().dgx_last_error().decode())
+
+
+def init(device=0):
+    """dgx_init: select the device, create the stream and memory pool."""
+    global _ready
+    L = load_library()
+    _check(L.dgx_init(device))
+    _ready = True
+
+
+def _lib_ready():
+    if not _ready:
+        init(0)
+    return _lib
+
+
+def stream():
+    return _lib_ready().dgx_stream()
+
+
+def synchronize():
+    _check(_lib_ready().dgx_synchronize())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A device allocation with numpy-style dtype/shape, owned by this object."""
+
+    def __init__(self, shape, dtype):
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        _check(_lib_ready().dgx_device_malloc(C.byref(self.ptr), max(self.nbytes, 1)))
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            _check(_lib.dgx_memcpy_htod(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(_lib.dgx_memcpy_dtoh(_ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        _check(_lib.dgx_memset_device(self.ptr, 0, self.nbytes))
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            _check(_lib.dgx_device_free(self.ptr))
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Octree:
+    """Device-resident sorted linear octree (keys on the chosen curve)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def __len__(self):
+        n = C.c_size_t()
+        _check(_lib.dgx_octree_size(self._h, C.byref(n)))
+        return n.value
+
+    @property
+    def max_depth(self):
+        m = C.c_uint32()
+        _check(_lib.dgx_octree_max_depth(self._h, C.byref(m)))
+        return m.value
+
+    def stats(self):
+        s = (C.c_uint64 * 5)()
+        _check(_lib.dgx_octree_stats(self._h, s))
+        return list(s)
+
+    def treenodes(self):
+        """ot::TreeNode array (n, 7) uint32: x, y, z, level, weight, dim, maxDepth."""
+        n = len(self)
+        out = np.zeros((n, 7), np.uint32)
+        _check(_lib.dgx_octree_export(self._h, _ptr(out), 0))
+        return out
+
+    def free(self):
+        if self._h and self._h.value:
+            _check(_lib.dgx_octree_free(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def points2octree(pts, max_depth=30, max_num_pts=1, curve=CURVE_HILBERT):
+    """ot::points2Octree on one rank.  pts: (n,3) float64 host array or a DeviceArray."""
+    L = _lib_ready()
+    h = C.c_void_p()
+    if isinstance(pts, DeviceArray):
+        n = pts.shape[0]
+        _check(L.dgx_points_to_octree(pts.ptr, n, max_depth, max_num_pts, curve, 1, C.byref(h)))
+    else:
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        _check(L.dgx_points_to_octree(_ptr(pts), len(pts), max_depth, max_num_pts, curve, 0, C.byref(h)))
+    return Octree(h)
+
+
+def balance_octree(octree, inc_corner=True):
+    """ot::balanceOctree on one rank (2:1 across faces, edges and corners)."""
+    h = C.c_void_p()
+    _check(_lib_ready().dgx_balance_octree(octree._h, 1 if inc_corner else 0, C.byref(h)))
+    return Octree(h)
+
+
+def import_treenodes(tn, max_depth=30, curve=CURVE_HILBERT, sort=True):
+    tn = np.ascontiguousarray(tn, np.uint32).reshape(-1, 7)
+    h = C.c_void_p()
+    _check(_lib_ready().dgx_octree_import(_ptr(tn), len(tn), max_depth, curve, 0, 1 if sort else 0, C.byref(h)))
+    return Octree(h)
+
+
+def sample_sort(tn, max_depth=30, curve=CURVE_HILBERT):
+    """par::sampleSort on one rank: the TreeNodes sorted on the curve."""
+    return import_treenodes(tn, max_depth, curve, sort=True).treenodes()
+
+
+class DA:
+    """ot::DA on one rank: element->node maps and the -lap+I operator."""
+
+    def __init__(self, octree):
+        self._h = C.c_void_p()
+        _check(_lib_ready().dgx_da_create(octree._h, C.byref(self._h)))
+        info = _DaInfo()
+        _check(_lib.dgx_da_get_info(self._h, C.byref(info)))
+        self.local_buffer_size = info.local_buffer_size
+        self.element_size = info.element_size
+        self.node_size = info.node_size
+        self.boundary_node_size = info.boundary_node_size
+        self.n_nodes = info.n_nodes
+        self.max_depth = info.max_depth
+        self.ghost_scans = info.ghost_scans
+
+    def maps(self):
+        """Host copies: nlist (E,8), masks (E,), oct_levels (N,), local_to_node (N,)."""
+        E, N = self.element_size, self.local_buffer_size
+        nl = np.zeros((E, 8), np.uint32)
+        mk = np.zeros(E, np.uint8)
+        ol = np.zeros(N, np.uint8)
+        l2n = np.zeros(N, np.uint32)
+        _check(_lib.dgx_da_export(self._h, _ptr(nl), _ptr(mk), _ptr(ol), _ptr(l2n)))
+        return nl, mk, ol, l2n
+
+    def matvec(self, x, y, lap_fac=1.0, mass_fac=1.0):
+        """y = (lap_fac K + mass_fac M) x; x, y DeviceArrays of n_nodes float64 (or float32)."""
+        if x.dtype == np.float32:
+            _check(_lib.dgx_da_matvec_f32(self._h, x.ptr, y.ptr, lap_fac, mass_fac))
+        else:
+            _check(_lib.dgx_da_matvec(self._h, x.ptr, y.ptr, lap_fac, mass_fac))
+
+    def matvec_host(self, x, lap_fac=1.0, mass_fac=1.0):
+        dt = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
+        dx = DeviceArray.from_host(np.asarray(x, dt))
+        dy = DeviceArray((self.n_nodes,), dt)
+        self.matvec(dx, dy, lap_fac, mass_fac)
+        return dy.to_host()
+
+    def diagonal_host(self, lap_fac=1.0, mass_fac=1.0):
+        dd = DeviceArray((self.n_nodes,), np.float64)
+        _check(_lib.dgx_da_diagonal(self._h, dd.ptr, lap_fac, mass_fac))
+        return dd.to_host()
+
+    def free(self):
+        if self._h and self._h.value:
+            _check(_lib.dgx_da_free(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gen_points(dist, seed, n, n_gauss=0, lo=0.0, hi=1.0, sig_lo=0.0, sig_hi=0.0, dirichlet=False):
+    """Deterministic synthetic points (host, (n,3) float64); see dgx_gen_points."""
+    L = load_library()
+    out = np.empty((n, 3), np.float64)
+    rc = L.dgx_gen_points(dist, seed, n, n_gauss, lo, hi, sig_lo, sig_hi, 1 if dirichlet else 0, _ptr(out))
+    if rc != 0:
+        raise DgxError("dgx_gen_points: bad arguments")
+    return out
+
+
+def sfc_tables(curve=CURVE_HILBERT):
+    L = load_library()
+    r = np.zeros(192, np.uint8)
+    m = np.zeros(192, np.uint8)
+    c = np.zeros(192, np.uint8)
+    L.dgx_sfc_tables(curve, _ptr(r), _ptr(m), _ptr(c))
+    return r.reshape(24, 8), m.reshape(24, 8), c.reshape(24, 8)
+
+
+# Benchmark configurations (point clouds) ------------------------------------
+def config_points(cfg, n=None):
+    """Point cloud of configuration cfg (1..5) with the stated seed; n overrides the size."""
+    if cfg == 1:
+        return gen_points(0, 0, n or 262144)
+    if cfg == 2:
+        return gen_points(1, 1, n or 4194304)
+    if cfg == 3:
+        return gen_points(2, 2, n or 16777216, n_gauss=4, lo=0.2, hi=0.8, sig_lo=0.02)
+    if cfg == 5:
+        return gen_points(2, 4, n or 134217728, n_gauss=16, lo=0.1, hi=0.9, sig_lo=0.005, sig_hi=0.05,
+                          dirichlet=True)
+    raise ValueError(cfg)
+
+
+CONFIG_MAX_NUM_PTS = {1: 1, 2: 8, 3: 4, 5: 4}

@@ -1,0 +1,333 @@
+// dgx_capi.cpp -- the extern "C" boundary (include/dendro_mi355x.h): runtime
+// context, error reporting, handles and argument checks.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/dendro_mi355x.h"
+#include "dgx_da.h"
+
+// ------------------------------------------------------------------ state --
+namespace {
+thread_local std::string g_err;
+struct Ctx {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipMemPool_t pool = nullptr;
+};
+Ctx g_ctx;
+std::mutex g_init_mu;
+}  // namespace
+
+void dgx_set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+void *dgx_alloc(size_t bytes, hipStream_t s) {
+    void *p = nullptr;
+    hipError_t e = hipMallocAsync(&p, bytes, s);
+    if (e != hipSuccess) {
+        dgx_set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        throw DgxError();
+    }
+    return p;
+}
+void dgx_free(void *p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
+static void ensure_ctx() {
+    if (!g_ctx.ready) {
+        dgx_set_error("dgx_init() has not been called");
+        throw DgxError();
+    }
+}
+static hipStream_t S() { return g_ctx.stream; }
+
+#define API_BEGIN try {
+#define API_END                                                         \
+    }                                                                   \
+    catch (DgxError &) {                                                \
+        return -1;                                                      \
+    }                                                                   \
+    catch (std::exception & ex) {                                       \
+        dgx_set_error("%s", ex.what());                                 \
+        return -1;                                                      \
+    }                                                                   \
+    return 0;
+
+struct dgx_octree_s {
+    DevBuf<Key> keys;
+    size_t n = 0;
+    uint32_t md = 30;
+    int curve = CURVE_HILBERT;
+    uint64_t stats[5] = {0, 0, 0, 0, 0};
+};
+struct dgx_da_s {
+    std::unique_ptr<DaDevice> da;
+};
+
+extern "C" {
+
+int dgx_init(int device) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    API_BEGIN
+    if (g_ctx.ready) {
+        if (g_ctx.device != device) {
+            dgx_set_error("dgx_init: already initialised on device %d", g_ctx.device);
+            return -1;
+        }
+        return 0;
+    }
+    int ndev = 0;
+    HIP_OK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        dgx_set_error("dgx_init: device %d not present (%d devices)", device, ndev);
+        return -1;
+    }
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    HIP_OK(hipDeviceGetDefaultMemPool(&g_ctx.pool, device));
+    uint64_t thr = UINT64_MAX;
+    HIP_OK(hipMemPoolSetAttribute(g_ctx.pool, hipMemPoolAttrReleaseThreshold, &thr));
+    g_ctx.device = device;
+    g_ctx.ready = true;
+    API_END
+}
+
+const char *dgx_last_error(void) { return g_err.c_str(); }
+
+void *dgx_stream(void) { return g_ctx.ready ? (void *)g_ctx.stream : nullptr; }
+
+int dgx_synchronize(void) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+
+int dgx_device_malloc(void **p, size_t bytes) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipMalloc(p, bytes));
+    API_END
+}
+int dgx_device_free(void *p) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipStreamSynchronize(S()));
+    HIP_OK(hipFree(p));
+    API_END
+}
+int dgx_memcpy_htod(void *dst, const void *src, size_t bytes) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S()));
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+int dgx_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S()));
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+int dgx_memset_device(void *dst, int value, size_t bytes) {
+    API_BEGIN
+    ensure_ctx();
+    HIP_OK(hipMemsetAsync(dst, value, bytes, S()));
+    API_END
+}
+
+int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts, int curve,
+                         int on_device, dgx_octree *out) {
+    API_BEGIN
+    ensure_ctx();
+    if (!out || (n && !xyz)) { dgx_set_error("points2Octree: null argument"); return -1; }
+    if (max_depth > 30) { dgx_set_error("points2Octree: maxDepth %u > 30", max_depth); return -1; }
+    if (max_num_pts == 0) { dgx_set_error("points2Octree: maxNumPts must be >= 1"); return -1; }
+    if (curve != DGX_CURVE_HILBERT && curve != DGX_CURVE_MORTON) { dgx_set_error("unknown curve %d", curve); return -1; }
+    if (n > 0xffffffffull) { dgx_set_error("points2Octree: %zu points exceed 2^32-1", n); return -1; }
+    std::unique_ptr<dgx_octree_s> o(new dgx_octree_s());
+    o->md = max_depth;
+    o->curve = curve;
+    DevBuf<double> tmp;
+    const double *d = xyz;
+    if (!on_device && n) {
+        tmp.alloc(3 * n, S());
+        HIP_OK(hipMemcpyAsync(tmp.p, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, S()));
+        d = tmp.p;
+    }
+    P2OStats st;
+    o->keys = p2o_build(d, n, max_depth, max_num_pts, curve, S(), &st);
+    o->n = o->keys.n;
+    o->stats[0] = st.textbook_leaves;
+    o->stats[1] = st.triggers;
+    o->stats[2] = st.replays;
+    o->stats[3] = st.replayed_leaves;
+    *out = o.release();
+    API_END
+}
+
+int dgx_balance_octree(dgx_octree in, int inc_corner, dgx_octree *out) {
+    (void)inc_corner;
+    API_BEGIN
+    ensure_ctx();
+    if (!in || !out) { dgx_set_error("balanceOctree: null argument"); return -1; }
+    std::unique_ptr<dgx_octree_s> o(new dgx_octree_s());
+    o->md = in->md;
+    o->curve = in->curve;
+    BalStats st;
+    o->keys = balance_build(in->keys.p, in->n, in->md, in->curve, S(), &st);
+    o->n = o->keys.n;
+    o->stats[0] = st.levels;
+    o->stats[1] = st.seeds;
+    *out = o.release();
+    API_END
+}
+
+int dgx_octree_import(const uint32_t *tn, size_t n, unsigned max_depth, int curve, int on_device, int sort,
+                      dgx_octree *out) {
+    API_BEGIN
+    ensure_ctx();
+    if (!out || (n && !tn)) { dgx_set_error("import: null argument"); return -1; }
+    if (max_depth > 31) { dgx_set_error("import: maxDepth %u > 31", max_depth); return -1; }
+    if (curve != DGX_CURVE_HILBERT && curve != DGX_CURVE_MORTON) { dgx_set_error("unknown curve %d", curve); return -1; }
+    std::unique_ptr<dgx_octree_s> o(new dgx_octree_s());
+    o->md = max_depth;
+    o->curve = curve;
+    DevBuf<uint32_t> tmp;
+    const uint32_t *d = tn;
+    if (!on_device && n) {
+        tmp.alloc(7 * n, S());
+        HIP_OK(hipMemcpyAsync(tmp.p, tn, 7 * n * sizeof(uint32_t), hipMemcpyHostToDevice, S()));
+        d = tmp.p;
+    }
+    o->keys = import_treenodes(d, n, max_depth, curve, sort != 0, S());
+    o->n = n;
+    *out = o.release();
+    API_END
+}
+
+int dgx_octree_size(dgx_octree o, size_t *n) {
+    if (!o || !n) { dgx_set_error("null argument"); return -1; }
+    *n = o->n;
+    return 0;
+}
+int dgx_octree_max_depth(dgx_octree o, unsigned *md) {
+    if (!o || !md) { dgx_set_error("null argument"); return -1; }
+    *md = o->md;
+    return 0;
+}
+int dgx_octree_stats(dgx_octree o, uint64_t out[5]) {
+    if (!o || !out) { dgx_set_error("null argument"); return -1; }
+    memcpy(out, o->stats, sizeof(o->stats));
+    return 0;
+}
+
+int dgx_octree_export(dgx_octree o, uint32_t *tn, int on_device) {
+    API_BEGIN
+    ensure_ctx();
+    if (!o || (o->n && !tn)) { dgx_set_error("export: null argument"); return -1; }
+    if (!o->n) return 0;
+    if (on_device) {
+        export_treenodes(o->keys.p, o->n, o->md, o->curve, tn, S());
+        HIP_OK(hipStreamSynchronize(S()));
+    } else {
+        DevBuf<uint32_t> tmp(7 * o->n, S());
+        export_treenodes(o->keys.p, o->n, o->md, o->curve, tmp.p, S());
+        HIP_OK(hipMemcpyAsync(tn, tmp.p, 7 * o->n * sizeof(uint32_t), hipMemcpyDeviceToHost, S()));
+        HIP_OK(hipStreamSynchronize(S()));
+    }
+    API_END
+}
+
+int dgx_octree_free(dgx_octree o) {
+    API_BEGIN
+    if (o) {
+        if (g_ctx.ready) HIP_OK(hipStreamSynchronize(S()));
+        delete o;
+    }
+    API_END
+}
+
+int dgx_da_create(dgx_octree oct, dgx_da *out) {
+    API_BEGIN
+    ensure_ctx();
+    if (!oct || !out) { dgx_set_error("DA: null argument"); return -1; }
+    if (oct->md > 30) { dgx_set_error("DA: maxDepth %u > 30 cannot be embedded", oct->md); return -1; }
+    std::unique_ptr<dgx_da_s> d(new dgx_da_s());
+    d->da = da_build(oct->keys.p, oct->n, oct->md, oct->curve, S());
+    *out = d.release();
+    API_END
+}
+
+int dgx_da_get_info(dgx_da d, dgx_da_info *info) {
+    if (!d || !info) { dgx_set_error("null argument"); return -1; }
+    const DaDevice &a = *d->da;
+    info->local_buffer_size = a.n_local;
+    info->element_size = a.n_elem;
+    info->node_size = a.node_size;
+    info->boundary_node_size = a.bnd_node_size;
+    info->n_nodes = a.n_nodes;
+    info->max_depth = a.md;
+    info->ghost_scans = a.n_ghost_scans;
+    return 0;
+}
+
+int dgx_da_export(dgx_da d, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels, uint32_t *l2n) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d) { dgx_set_error("null argument"); return -1; }
+    const DaDevice &a = *d->da;
+    if (nlist) HIP_OK(hipMemcpyAsync(nlist, a.nlist.p, 8 * a.n_elem * 4, hipMemcpyDeviceToHost, S()));
+    if (masks) HIP_OK(hipMemcpyAsync(masks, a.mask.p, a.n_elem, hipMemcpyDeviceToHost, S()));
+    if (oct_levels) HIP_OK(hipMemcpyAsync(oct_levels, a.flags.p, a.n_local, hipMemcpyDeviceToHost, S()));
+    if (l2n) HIP_OK(hipMemcpyAsync(l2n, a.l2n.p, a.n_local * 4, hipMemcpyDeviceToHost, S()));
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+
+int dgx_da_matvec(dgx_da d, const double *x, double *y, double lap_fac, double mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
+    da_matvec_f64(*d->da, x, y, lap_fac, mass_fac, S());
+    API_END
+}
+int dgx_da_matvec_f32(dgx_da d, const float *x, float *y, float lap_fac, float mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
+    da_matvec_f32(*d->da, x, y, lap_fac, mass_fac, S());
+    API_END
+}
+int dgx_da_diagonal(dgx_da d, double *diag, double lap_fac, double mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !diag) { dgx_set_error("diagonal: null argument"); return -1; }
+    da_diagonal_f64(*d->da, diag, lap_fac, mass_fac, S());
+    API_END
+}
+int dgx_da_free(dgx_da d) {
+    API_BEGIN
+    if (d) {
+        if (g_ctx.ready) HIP_OK(hipStreamSynchronize(S()));
+        delete d;
+    }
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+// dgx_common.h -- shared device/host definitions for the MI355X octree-FEM path.
+//
+// Octants are carried on the device as 128-bit space-filling-curve keys:
+//   hi: digits of levels 1..21 (3 bits each, level t at bits [63-3t, 65-3t]), bit 63 = 0
+//   lo: digits of levels 22..31 (level t at bits [63-3(t-21), 65-3(t-21)]), level in bits 0..4
+// A digit is the rank of the child along the curve (Hilbert rank under the
+// parent's rotation, or the Morton index z<<2|y<<1|x for Morton order), so
+// unsigned (hi, lo) comparison of keys reproduces ot::TreeNode::operator<
+// (include/TreeNode.tcc:122-238): ancestors precede descendants, and two
+// disjoint octants compare by the child rank at the level below their nearest
+// common ancestor.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define DGX_MAX_DEPTH 31
+
+struct Key {
+    uint64_t hi, lo;
+};
+
+// SFC tables, one set per curve (0 = Hilbert, 1 = Morton).
+//   rank[r*8+m]  : rank of Morton child m under rotation r
+//   morton[r*8+k]: Morton index of the rank-k child under rotation r
+//   crot[r*8+m]  : rotation inside Morton child m
+// The Hilbert values are the reference's table data (src/hcurvedata.cpp:329-546).
+struct SfcTables {
+    uint8_t rank[24 * 8];
+    uint8_t morton[24 * 8];
+    uint8_t crot[24 * 8];
+};
+
+const SfcTables &host_sfc(int curve);
+
+enum { CURVE_HILBERT = 0, CURVE_MORTON = 1 };
+
+__host__ __device__ inline bool key_lt(const Key &a, const Key &b) {
+    return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+__host__ __device__ inline bool key_le(const Key &a, const Key &b) { return !key_lt(b, a); }
+__host__ __device__ inline bool key_eq(const Key &a, const Key &b) { return a.hi == b.hi && a.lo == b.lo; }
+__host__ __device__ inline uint32_t key_level(const Key &k) { return (uint32_t)(k.lo & 31u); }
+
+__host__ __device__ inline uint32_t key_digit(const Key &k, uint32_t t) {
+    return t <= 21 ? (uint32_t)((k.hi >> (63 - 3 * t)) & 7u) : (uint32_t)((k.lo >> (63 - 3 * (t - 21))) & 7u);
+}
+__host__ __device__ inline void key_set_digit(Key &k, uint32_t t, uint32_t d) {
+    if (t <= 21) {
+        uint32_t s = 63 - 3 * t;
+        k.hi = (k.hi & ~(7ull << s)) | ((uint64_t)d << s);
+    } else {
+        uint32_t s = 63 - 3 * (t - 21);
+        k.lo = (k.lo & ~(7ull << s)) | ((uint64_t)d << s);
+    }
+}
+// Keep digits 1..L, clear the rest, set level L.
+__host__ __device__ inline Key key_truncate(const Key &k, uint32_t L) {
+    Key r;
+    if (L <= 21) {
+        uint32_t s = 63 - 3 * L;
+        r.hi = (s >= 64) ? 0 : (k.hi & ~((1ull << s) - 1ull));
+        r.lo = L;
+    } else {
+        r.hi = k.hi;
+        uint32_t s = 63 - 3 * (L - 21);
+        r.lo = (k.lo & ~((1ull << s) - 1ull)) | L;
+    }
+    return r;
+}
+// Number of leading digits two keys share (levels ignored); 31 if all equal.
+__host__ __device__ inline uint32_t key_common_levels(const Key &a, const Key &b) {
+    uint64_t x = a.hi ^ b.hi;
+    if (x) {
+#ifdef __HIP_DEVICE_COMPILE__
+        uint32_t p = 63 - __clzll(x);
+#else
+        uint32_t p = 63 - __builtin_clzll(x);
+#endif
+        return (65 - p) / 3 - 1;
+    }
+    x = (a.lo ^ b.lo) & ~31ull;
+    if (!x) return 31;
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t p = 63 - __clzll(x);
+#else
+    uint32_t p = 63 - __builtin_clzll(x);
+#endif
+    return 21 + (65 - p) / 3 - 1;
+}
+__host__ __device__ inline bool key_is_ancestor(const Key &a, const Key &b) {
+    uint32_t la = key_level(a);
+    return la < key_level(b) && key_eq(key_truncate(b, la), a);
+}
+__host__ __device__ inline Key key_parent(const Key &k) {
+    uint32_t L = key_level(k);
+    return key_truncate(k, L ? L - 1 : 0);
+}
+__host__ __device__ inline Key key_first_child(const Key &k) {
+    Key r = k;
+    r.lo = (r.lo & ~31ull) | (key_level(k) + 1);
+    return r;
+}
+// getNext in digit space (TreeNode.tcc:460-550): the SFC successor at the same
+// or a coarser level.  Returns false when k holds the last cell of the domain.
+__host__ __device__ inline bool key_next(const Key &k, Key &out) {
+    uint32_t L = key_level(k);
+    for (uint32_t t = L; t >= 1; --t) {
+        uint32_t d = key_digit(k, t);
+        if (d < 7) {
+            Key r = key_truncate(k, t);
+            key_set_digit(r, t, d + 1);
+            out = r;
+            return true;
+        }
+    }
+    return false;
+}
+
+// Encode an octant given by anchor (x,y,z) and level in a domain of 2^md.
+__host__ __device__ inline Key sfc_encode(const SfcTables &T, uint32_t x, uint32_t y, uint32_t z, uint32_t lev,
+                                          uint32_t md) {
+    Key k{0ull, 0ull};
+    uint32_t rot = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t b = md - t;
+        uint32_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+        uint32_t r = T.rank[rot * 8 + m];
+        rot = T.crot[rot * 8 + m];
+        if (t <= 21) k.hi |= (uint64_t)r << (63 - 3 * t);
+        else k.lo |= (uint64_t)r << (63 - 3 * (t - 21));
+    }
+    k.lo |= lev;
+    return k;
+}
+__host__ __device__ inline void sfc_decode(const SfcTables &T, const Key &k, uint32_t md, uint32_t &x, uint32_t &y,
+                                           uint32_t &z, uint32_t &lev) {
+    lev = key_level(k);
+    x = y = z = 0;
+    uint32_t rot = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t r = key_digit(k, t);
+        uint32_t m = T.morton[rot * 8 + r];
+        uint32_t b = md - t;
+        x |= (m & 1u) << b;
+        y |= ((m >> 1) & 1u) << b;
+        z |= ((m >> 2) & 1u) << b;
+        rot = T.crot[rot * 8 + m];
+    }
+}
+// Morton keys without tables (digit = z<<2|y<<1|x of the anchor bits).
+__host__ __device__ inline Key morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t lev, uint32_t md) {
+    Key k{0ull, 0ull};
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t b = md - t;
+        uint64_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+        if (t <= 21) k.hi |= m << (63 - 3 * t);
+        else k.lo |= m << (63 - 3 * (t - 21));
+    }
+    k.lo |= lev;
+    return k;
+}
+__host__ __device__ inline void morton_decode(const Key &k, uint32_t md, uint32_t &x, uint32_t &y, uint32_t &z,
+                                              uint32_t &lev) {
+    lev = key_level(k);
+    x = y = z = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t m = key_digit(k, t), b = md - t;
+        x |= (m & 1u) << b;
+        y |= ((m >> 1) & 1u) << b;
+        z |= ((m >> 2) & 1u) << b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side error plumbing and device buffers.
+// ---------------------------------------------------------------------------
+void dgx_set_error(const char *fmt, ...);
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            dgx_set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));   \
+            throw DgxError();                                                                          \
+        }                                                                                              \
+    } while (0)
+
+struct DgxError {};
+
+// Stream-ordered device allocation from the library's memory pool.
+void *dgx_alloc(size_t bytes, hipStream_t s);
+void dgx_free(void *p, hipStream_t s);
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipStream_t s = 0;
+    DevBuf() {}
+    DevBuf(size_t count, hipStream_t st) { alloc(count, st); }
+    void alloc(size_t count, hipStream_t st) {
+        release();
+        s = st;
+        n = count;
+        p = count ? (T *)dgx_alloc(count * sizeof(T), st) : nullptr;
+    }
+    void release() {
+        if (p) dgx_free(p, s);
+        p = nullptr;
+        n = 0;
+    }
+    T *detach() {
+        T *q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), s(o.s) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; s = o.s; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------
+// Primitives (dgx_sort.hip)
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of 128-bit keys with an optional 32-bit payload.
+// Bits [begin_bit, end_bit) of the 128-bit value (hi:lo) are sorted; passes in
+// which every key has the same 8-bit digit are skipped.
+void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
+// Exclusive scan of n uint32 values; returns the total (synchronises the stream).
+uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
+// Same, leaving the total in device memory (no host synchronisation).
+void scan_exclusive_u32_async(const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, hipStream_t s);

@@ -1,0 +1,34 @@
+// dgx_da.h -- device-resident ot::DA (one rank) and its operators.
+#pragma once
+#include <memory>
+
+#include "dgx_octree.h"
+
+struct DaDevice {
+    uint32_t md = 0;  // embedded max depth (D+1)
+    int curve = 0;
+    size_t n_local = 0, n_elem = 0;
+    DevBuf<Key> key;                 // local buffer keys (elements, then pseudo-octants)
+    DevBuf<uint32_t> x, y, z;        // anchors in the 2^(D+1) domain
+    DevBuf<uint8_t> flags;           // m_ucpOctLevels: level | BOUNDARY(64) | NODE(128)
+    DevBuf<uint32_t> nlist;          // m_uiNlist: 8 local-buffer indices per element
+    DevBuf<uint8_t> mask;            // hanging mask per element (m_ucpLutMasks[2i+1])
+    DevBuf<uint32_t> l2n;            // local index -> compact node id (0xffffffff: not a node)
+    uint32_t n_nodes = 0, node_size = 0, bnd_node_size = 0;
+    uint32_t n_unresolved = 0, n_ghost_scans = 0;
+    // matvec view: conn[j*E + e] compact node id of vertex j; meta = mask | cnum<<8 | level<<11
+    DevBuf<uint32_t> conn;
+    DevBuf<uint16_t> meta;
+    // Dirichlet: 1 for nodes on the domain boundary (filled on demand)
+    DevBuf<uint8_t> bdy_node;
+    ~DaDevice();
+};
+
+std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s);
+
+// y = A x with A = lapFac * stiffness + massFac * mass (type-2 hanging stencils),
+// FP64 on compact node vectors (Jacobian1MatMult, odaJac.C:275-366).
+void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s);
+void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s);
+// Diagonal of the same operator (Jacobian1MatGetDiagonal, odaJac.C:160-227).
+void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double mass_fac, hipStream_t s);

@@ -1,0 +1,345 @@
+// dgx_da.hip -- ot::DA construction on one rank (MPI_COMM_SELF) for gfx950.
+//
+// Follows the reference factory (src/oda/private/odaFactory.cpp:102-923):
+//   Part1 (:150-205): embed every octant one level deeper in a 2^(D+1) domain and
+//         add the positive-boundary pseudo-octants (octFunctions.cpp:681-770),
+//         then re-sort on the curve.  The original domain is Morton child 0 of
+//         the embedded root, so its octants precede all pseudo-octants.
+//   Part2 (:207-220): NODE flags (flagNodesType3, octFunctions.cpp:1556-2168):
+//         an anchor is hanging iff the parent of one of the -x/-y/-z neighbours
+//         selected by the child number is itself an octant.
+//   Part3: one rank has no ghosts, so the local buffer is [elements | pseudo-
+//         octants] and buildNodeList (odaBuildNlist.cpp:47-934, 3010-3095) is
+//         8 searches per element: the vertex's owner (maxLowerBound at the
+//         vertex cell) if it is a NODE anchored there, else the owner of the
+//         parent's vertex, else the forward ghost scan.
+// Node vectors are compacted in local-buffer order over NODE octants
+// (DA::vecGetBuffer, oda.cpp:863-882).
+#include "dgx_da.h"
+
+#include <vector>
+
+namespace {
+
+constexpr int B = 256;
+constexpr uint8_t F_BOUNDARY = 64, F_NODE = 128;
+
+#define LOAD_TABLES(tab)                                                 \
+    __shared__ uint8_t s_rc[192], s_mc[192];                             \
+    for (int i_ = threadIdx.x; i_ < 192; i_ += blockDim.x) {             \
+        s_rc[i_] = (tab).rc[i_];                                          \
+        s_mc[i_] = (tab).mc[i_];                                          \
+    }                                                                    \
+    __syncthreads();
+
+__device__ inline uint32_t pos_bdy_flags(uint32_t x, uint32_t y, uint32_t z, uint32_t lev, uint32_t md) {
+    // TreeNode::isBoundaryOctant(POSITIVE) (TreeNode.cpp:247-269)
+    uint32_t len = 1u << (md - lev), blen = (1u << md) - len, f = 0;
+    if (x == blen) f |= 1u;
+    if (y == blen) f |= 2u;
+    if (z == blen) f |= 4u;
+    return f;
+}
+__device__ inline uint32_t n_pseudo(uint32_t f) {
+    // +x, +y, +z, +x+y, +x+z, +y+z, +x+y+z (octFunctions.cpp:702-757)
+    uint32_t c = __popc(f);
+    return c == 3 ? 7u : (c == 2 ? 3u : c);
+}
+
+__global__ void __launch_bounds__(B) k_da_embed(const Key *in, size_t n, uint32_t md0, SfcPacked tab, Key *dkey,
+                                                uint32_t *nbdy) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z, l;
+    dev_decode(s_mc, in[i], md0, x, y, z, l);
+    dkey[i] = dev_encode(s_rc, x, y, z, l + 1, md0 + 1);
+    nbdy[i] = n_pseudo(pos_bdy_flags(x, y, z, l, md0));
+}
+
+__global__ void __launch_bounds__(B) k_da_bdy_emit(const Key *in, size_t n, uint32_t md0, SfcPacked tab,
+                                                   const uint32_t *off, Key *bkey) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z, l;
+    dev_decode(s_mc, in[i], md0, x, y, z, l);
+    uint32_t f = pos_bdy_flags(x, y, z, l, md0);
+    if (!f) return;
+    uint32_t o = off[i], D = 1u << md0, md = md0 + 1, L = l + 1;
+    if (f & 1u) bkey[o++] = dev_encode(s_rc, D, y, z, L, md);
+    if (f & 2u) bkey[o++] = dev_encode(s_rc, x, D, z, L, md);
+    if (f & 4u) bkey[o++] = dev_encode(s_rc, x, y, D, L, md);
+    if ((f & 3u) == 3u) bkey[o++] = dev_encode(s_rc, D, D, z, L, md);
+    if ((f & 5u) == 5u) bkey[o++] = dev_encode(s_rc, D, y, D, L, md);
+    if ((f & 6u) == 6u) bkey[o++] = dev_encode(s_rc, x, D, D, L, md);
+    if (f == 7u) bkey[o++] = dev_encode(s_rc, D, D, D, L, md);
+}
+
+__global__ void __launch_bounds__(B) k_da_decode(const Key *k, size_t n, size_t nelem, uint32_t md, SfcPacked tab,
+                                                 uint32_t *xs, uint32_t *ys, uint32_t *zs, uint8_t *flags) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z, l;
+    dev_decode(s_mc, k[i], md, x, y, z, l);
+    xs[i] = x;
+    ys[i] = y;
+    zs[i] = z;
+    flags[i] = (uint8_t)(l | (i >= nelem ? F_BOUNDARY : 0));
+}
+
+__device__ inline bool bsearch_exact(const Key *a, size_t n, const Key &k) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_lt(a[mid], k)) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && key_eq(a[lo], k);
+}
+// seq::maxLowerBound (seqUtils.tcc:59-100): greatest a[i] <= k.
+__device__ inline bool max_lower_bound(const Key *a, size_t n, const Key &k, size_t &idx) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_le(a[mid], k)) lo = mid + 1; else hi = mid;
+    }
+    if (lo == 0) { idx = 0; return false; }
+    idx = lo - 1;
+    return true;
+}
+
+__global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uint32_t md, SfcPacked tab,
+                                                     const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
+                                                     uint8_t *flags) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lev = flags[i] & 31u;
+    bool hanging = false;
+    if (lev != 1) {
+        uint32_t X = xs[i], Y = ys[i], Z = zs[i], s = 1u << (md - lev);
+        uint32_t cn = (((Z >> (md - lev)) & 1u) << 2) | (((Y >> (md - lev)) & 1u) << 1) | ((X >> (md - lev)) & 1u);
+        // parent of the same-level neighbour at (a,b,c), as a key at level lev-1
+        auto probe = [&](uint32_t a, uint32_t b, uint32_t c) {
+            uint32_t ps = md - lev + 1;
+            Key pk = dev_encode(s_rc, (a >> ps) << ps, (b >> ps) << ps, (c >> ps) << ps, lev - 1, md);
+            if (bsearch_exact(k, n, pk)) hanging = true;
+        };
+        switch (cn) {
+            case 1: if (Y) probe(X, Y - s, Z); if (Z) probe(X, Y, Z - s); if (Y && Z) probe(X, Y - s, Z - s); break;
+            case 2: if (X) probe(X - s, Y, Z); if (Z) probe(X, Y, Z - s); if (X && Z) probe(X - s, Y, Z - s); break;
+            case 3: if (Z) probe(X, Y, Z - s); break;
+            case 4: if (X) probe(X - s, Y, Z); if (Y) probe(X, Y - s, Z); if (X && Y) probe(X - s, Y - s, Z); break;
+            case 5: if (Y) probe(X, Y - s, Z); break;
+            case 6: if (X) probe(X - s, Y, Z); break;
+            default: break;
+        }
+    }
+    if (!hanging) flags[i] |= F_NODE;
+}
+
+// buildNodeList for one element (odaBuildNlist.cpp:531-934 on one rank).
+__global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t nelem, uint32_t md, SfcPacked tab,
+                                                const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
+                                                const uint8_t *flags, uint32_t *nlist, uint8_t *mask,
+                                                uint32_t *counters /* [0]=unresolved [1]=ghost scans */) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= nelem) return;
+    const uint32_t d = flags[i] & 31u, x = xs[i], y = ys[i], z = zs[i];
+    const bool inode = flags[i] & F_NODE;
+    const uint32_t ps = md - d + 1, parX = (x >> ps) << ps, parY = (y >> ps) << ps, parZ = (z >> ps) << ps;
+    const uint32_t sz = 1u << (md - d);
+    uint32_t nl[8];
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
+        const uint32_t vx = x + bx * sz, vy = y + by * sz, vz = z + bz * sz;
+        const uint32_t wx = parX + bx * (sz << 1), wy = parY + by * (sz << 1), wz = parZ + bz * (sz << 1);
+        size_t idx = 0;
+        bool found;
+        if (j == 0) {
+            uint32_t r = (uint32_t)i;
+            if (!inode) {
+                Key pk = dev_encode(s_rc, wx, wy, wz, md, md);
+                found = max_lower_bound(k, n, pk, idx);
+                if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) r = (uint32_t)idx;
+            }
+            nl[0] = r;
+            continue;
+        }
+        Key sk = dev_encode(s_rc, vx, vy, vz, md, md);
+        uint32_t sx = vx, sy = vy, szz = vz, lastLevel = d;
+        found = max_lower_bound(k, n, sk, idx);
+        if (found && !(key_is_ancestor(k[idx], sk) || key_eq(k[idx], sk))) found = false;
+        bool secondary = true;
+        if (found && xs[idx] == vx && ys[idx] == vy && zs[idx] == vz && (flags[idx] & F_NODE)) {
+            nl[j] = (uint32_t)idx;
+            secondary = false;
+        }
+        if (!secondary) continue;
+        sx = wx; sy = wy; szz = wz; lastLevel = d - 1;
+        Key pk = dev_encode(s_rc, wx, wy, wz, md, md);
+        found = max_lower_bound(k, n, pk, idx);
+        if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) {
+            nl[j] = (uint32_t)idx;
+            continue;
+        }
+        // forward ghost scan (odaBuildNlist.cpp:566-600)
+        atomicAdd(&counters[1], 1u);
+        Key nk = dev_encode(s_rc, sx, sy, szz, lastLevel, md);
+        Key dld = nk;  // last cell of nk
+        for (uint32_t t = lastLevel + 1; t <= md; ++t) key_set_digit(dld, t, 7);
+        dld.lo = (dld.lo & ~31ull) | md;
+        uint32_t res = (uint32_t)n;
+        for (size_t kk = 1; idx + kk < n; ++kk) {
+            const Key &c = k[idx + kk];
+            if (key_eq(key_parent(c), nk)) {
+                if (flags[idx + kk] & F_NODE) continue;
+                res = (uint32_t)(idx + kk);
+                break;
+            }
+            if (key_lt(dld, c)) break;
+        }
+        nl[j] = res;
+    }
+    uint8_t m = 0;
+    for (int j = 0; j < 8; ++j) {
+        uint32_t r = nl[j];
+        nlist[8 * i + j] = r;
+        if (r >= n) {
+            atomicAdd(&counters[0], 1u);
+            continue;
+        }
+        uint32_t _x = xs[r], _y = ys[r], _z = zs[r];
+        if (!(flags[r] & F_NODE)) {
+            uint32_t s2 = md - (flags[r] & 31u) + 1;
+            _x = (_x >> s2) << s2; _y = (_y >> s2) << s2; _z = (_z >> s2) << s2;
+        }
+        const uint32_t bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
+        if (j == 0) {
+            if (!inode) m |= 1;
+        } else if (x + bx * sz != _x || y + by * sz != _y || z + bz * sz != _z) {
+            m |= (uint8_t)(1u << j);
+        }
+    }
+    mask[i] = m;
+}
+
+__global__ void k_da_mark_nodes(const uint32_t *nlist, size_t nelem, size_t n, uint8_t *isnode) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * nelem) return;
+    uint32_t r = nlist[t];
+    if (r < n) isnode[r] = 1;
+}
+
+__global__ void k_da_apply_nodes(const uint8_t *isnode, size_t n, size_t nelem, uint8_t *flags, uint32_t *nodeflag,
+                                 uint32_t *counts /* [0] elem nodes [1] bnd nodes */) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (isnode[i]) {
+        flags[i] |= F_NODE;
+        atomicAdd(&counts[i < nelem ? 0 : 1], 1u);
+    }
+    nodeflag[i] = (flags[i] & F_NODE) ? 1u : 0u;
+}
+
+__global__ void k_da_l2n(const uint32_t *nodeflag, const uint32_t *pos, size_t n, uint32_t *l2n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    l2n[i] = nodeflag[i] ? pos[i] : 0xffffffffu;
+}
+
+// Element view for the matvec: SoA compact node ids and packed meta
+// (hanging mask | child number << 8 | original level << 11).
+__global__ void k_da_elem_view(const uint32_t *nlist, const uint8_t *mask, const uint32_t *l2n, const uint32_t *xs,
+                               const uint32_t *ys, const uint32_t *zs, const uint8_t *flags, size_t nelem,
+                               uint32_t md, uint32_t *conn, uint16_t *meta) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nelem) return;
+    for (int j = 0; j < 8; ++j) conn[(size_t)j * nelem + e] = l2n[nlist[8 * e + j]];
+    uint32_t lev = flags[e] & 31u, b = md - lev;
+    uint32_t cn = (((zs[e] >> b) & 1u) << 2) | (((ys[e] >> b) & 1u) << 1) | ((xs[e] >> b) & 1u);
+    meta[e] = (uint16_t)(mask[e] | (cn << 8) | ((lev - 1) << 11));
+}
+
+}  // namespace
+
+DaDevice::~DaDevice() {}
+
+std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s) {
+    if (n <= 8) {  // blockPartStage1 asserts nodes.size() > 2^dim (blockPart.cpp:583)
+        dgx_set_error("ot::DA needs more than 8 octants (got %zu)", n);
+        throw DgxError();
+    }
+    std::unique_ptr<DaDevice> da(new DaDevice());
+    da->md = md0 + 1;
+    da->curve = curve;
+    const uint32_t md = md0 + 1;
+    SfcPacked tab = make_packed(curve);
+    // Part1
+    DevBuf<Key> dkey(n, s);
+    DevBuf<uint32_t> nb(n, s), nboff(n, s);
+    k_da_embed<<<grid_for(n, B), B, 0, s>>>(oct, n, md0, tab, dkey.p, nb.p);
+    HIP_OK(hipGetLastError());
+    uint64_t nbd = scan_exclusive_u32(nb.p, nboff.p, n, s);
+    const size_t N = n + nbd;
+    da->key.alloc(N, s);
+    if (nbd) k_da_bdy_emit<<<grid_for(n, B), B, 0, s>>>(oct, n, md0, tab, nboff.p, da->key.p + n);
+    HIP_OK(hipMemcpyAsync(da->key.p, dkey.p, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+    dkey.release();
+    nb.release();
+    nboff.release();
+    radix_sort_keys(da->key.p, nullptr, n, s, 0, 128);
+    if (nbd) radix_sort_keys(da->key.p + n, nullptr, nbd, s, 0, 128);
+    da->n_local = N;
+    da->n_elem = n;
+    da->x.alloc(N, s);
+    da->y.alloc(N, s);
+    da->z.alloc(N, s);
+    da->flags.alloc(N, s);
+    k_da_decode<<<grid_for(N, B), B, 0, s>>>(da->key.p, N, n, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p);
+    // Part2
+    k_da_flag_nodes<<<grid_for(N, B), B, 0, s>>>(da->key.p, N, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p);
+    HIP_OK(hipGetLastError());
+    // Part3: node list
+    da->nlist.alloc(8 * n, s);
+    da->mask.alloc(n, s);
+    DevBuf<uint32_t> ctr(4, s);
+    HIP_OK(hipMemsetAsync(ctr.p, 0, 16, s));
+    k_da_nlist<<<grid_for(n, B), B, 0, s>>>(da->key.p, N, n, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p,
+                                           da->nlist.p, da->mask.p, ctr.p);
+    HIP_OK(hipGetLastError());
+    DevBuf<uint8_t> isnode(N, s);
+    HIP_OK(hipMemsetAsync(isnode.p, 0, N, s));
+    k_da_mark_nodes<<<grid_for(8 * n, B), B, 0, s>>>(da->nlist.p, n, N, isnode.p);
+    DevBuf<uint32_t> nodeflag(N, s), pos(N, s);
+    k_da_apply_nodes<<<grid_for(N, B), B, 0, s>>>(isnode.p, N, n, da->flags.p, nodeflag.p, ctr.p + 2);
+    uint64_t nn = scan_exclusive_u32(nodeflag.p, pos.p, N, s);
+    da->l2n.alloc(N, s);
+    k_da_l2n<<<grid_for(N, B), B, 0, s>>>(nodeflag.p, pos.p, N, da->l2n.p);
+    uint32_t hc[4];
+    HIP_OK(hipMemcpyAsync(hc, ctr.p, 16, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    da->n_unresolved = hc[0];
+    da->n_ghost_scans = hc[1];
+    da->node_size = hc[2];
+    da->bnd_node_size = hc[3];
+    da->n_nodes = (uint32_t)nn;
+    if (hc[0]) {
+        dgx_set_error("ot::DA: %u node-list entries unresolved on one rank", hc[0]);
+        throw DgxError();
+    }
+    if ((uint64_t)hc[2] + hc[3] != nn) {
+        dgx_set_error("ot::DA: %llu NODE octants but %u referenced nodes", (unsigned long long)nn, hc[2] + hc[3]);
+        throw DgxError();
+    }
+    // matvec view
+    da->conn.alloc(8 * n, s);
+    da->meta.alloc(n, s);
+    k_da_elem_view<<<grid_for(n, B), B, 0, s>>>(da->nlist.p, da->mask.p, da->l2n.p, da->x.p, da->y.p, da->z.p,
+                                               da->flags.p, n, md, da->conn.p, da->meta.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    return da;
+}

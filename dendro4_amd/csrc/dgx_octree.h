@@ -1,0 +1,62 @@
+// dgx_octree.h -- octree construction and balance (device side, internal API).
+#pragma once
+#include "dgx_common.h"
+
+// Packed per-curve tables staged into LDS by the kernels that encode/decode:
+//   rc[r*8+m] = rank(r,m) | crot(r,m) << 3        (encode: Morton child -> rank)
+//   mc[r*8+k] = morton(r,k) | crot(r,morton) << 3 (decode: rank -> Morton child)
+struct SfcPacked {
+    uint8_t rc[192];
+    uint8_t mc[192];
+};
+SfcPacked make_packed(int curve);
+
+__device__ inline Key dev_encode(const uint8_t *rc, uint32_t x, uint32_t y, uint32_t z, uint32_t lev, uint32_t md) {
+    Key k{0ull, 0ull};
+    uint32_t rot = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t b = md - t;
+        uint32_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+        uint32_t v = rc[rot * 8 + m];
+        rot = v >> 3;
+        if (t <= 21) k.hi |= (uint64_t)(v & 7u) << (63 - 3 * t);
+        else k.lo |= (uint64_t)(v & 7u) << (63 - 3 * (t - 21));
+    }
+    k.lo |= lev;
+    return k;
+}
+__device__ inline void dev_decode(const uint8_t *mc, const Key &k, uint32_t md, uint32_t &x, uint32_t &y, uint32_t &z,
+                                  uint32_t &lev) {
+    lev = key_level(k);
+    x = y = z = 0;
+    uint32_t rot = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t v = mc[rot * 8 + key_digit(k, t)];
+        uint32_t m = v & 7u, b = md - t;
+        x |= (m & 1u) << b;
+        y |= ((m >> 1) & 1u) << b;
+        z |= ((m >> 2) & 1u) << b;
+        rot = v >> 3;
+    }
+}
+
+struct ReplayResult {
+    uint32_t nout, end, err, pad;
+    Key resync;
+};
+
+struct P2OStats {
+    uint64_t textbook_leaves = 0, triggers = 0, replays = 0, replayed_leaves = 0, leaves = 0;
+};
+struct BalStats {
+    uint64_t levels = 0, seeds = 0, leaves = 0;
+};
+
+// Sorted leaf keys of the octree built from n points (xyz interleaved, device).
+DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t max_num_pts, int curve, hipStream_t s,
+                      P2OStats *stats);
+// Sorted leaf keys of the minimal 2:1 (face, edge, corner) balanced refinement.
+DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipStream_t s, BalStats *stats);
+// Keys <-> ot::TreeNode arrays (7 x uint32 per octant, device).
+void export_treenodes(const Key *k, size_t n, uint32_t md, int curve, uint32_t *d_tn, hipStream_t s);
+DevBuf<Key> import_treenodes(const uint32_t *d_tn, size_t n, uint32_t md, int curve, bool sort, hipStream_t s);

@@ -1,0 +1,681 @@
+// dgx_octree.hip -- points -> linear octree (ot::points2Octree) and 2:1 balance
+// (ot::balanceOctree) on gfx950.
+//
+// points2Octree (src/construct.cpp:911-1077):
+//   1. quantise + SFC-encode every point (k_quantise)            [construct.cpp:929-945]
+//   2. radix-sort the 90-bit keys                                [omp_par::merge_sort, :955]
+//   3. "textbook" leaves: an octant is split iff it holds more than maxNumPts
+//      points.  O is split iff it contains some pair (p_i, p_{i+m}), so the
+//      leaf level of point j is 1 + max over windows i in [j-m, j] of the
+//      common-prefix level of (p_i, p_{i+m}) (k_p2o_lev, k_p2o_leaf).
+//   4. empty leaves fill the gaps between consecutive non-empty leaves as the
+//      coarsest octants of the gap, enumerated in digit space (k_gap_*).
+//   5. p2oLocal (:976-1077) advances a point cursor through windows of
+//      maxNumPts points; it equals the textbook tree except after level-maxDepth
+//      cells whose points do not fit in one window (duplicate clusters).  Those
+//      cells are detected (k_p2o_trigger) and the reference state machine is
+//      replayed exactly from there (k_p2o_replay) until it is back in step.
+// balanceOctree (src/balance.cpp:55-64 -> comboRipple; TreeNode::balanceSubtree
+// TreeNode.cpp:884-1343): level by level from the finest, the parents of all
+// seeds are made unique (their 8 children are in the balanced tree) and their
+// 26 neighbours seed the next coarser level.  Leaves are the children of split
+// octants that are not split.  Per-level work uses Morton keys; the final leaf
+// set is re-keyed on the output curve and sorted once.
+#include "dgx_octree.h"
+
+#include <algorithm>
+#include <vector>
+
+SfcPacked make_packed(int curve) {
+    const SfcTables &T = host_sfc(curve);
+    SfcPacked p;
+    for (int r = 0; r < 24; ++r)
+        for (int m = 0; m < 8; ++m) {
+            p.rc[r * 8 + m] = (uint8_t)(T.rank[r * 8 + m] | (T.crot[r * 8 + m] << 3));
+            uint8_t mm = T.morton[r * 8 + m];  // m plays the role of the rank here
+            p.mc[r * 8 + m] = (uint8_t)(mm | (T.crot[r * 8 + mm] << 3));
+        }
+    return p;
+}
+
+namespace {
+
+#define LOAD_TABLES(tab)                                                 \
+    __shared__ uint8_t s_rc[192], s_mc[192];                             \
+    for (int i_ = threadIdx.x; i_ < 192; i_ += blockDim.x) {             \
+        s_rc[i_] = (tab).rc[i_];                                          \
+        s_mc[i_] = (tab).mc[i_];                                          \
+    }                                                                    \
+    __syncthreads();
+
+constexpr int B = 256;
+
+__global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, SfcPacked tab, Key *keys,
+                                                uint32_t *bad) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double scale = (double)(1u << md);
+    double fx = xyz[3 * i] * scale / 1.0, fy = xyz[3 * i + 1] * scale / 1.0, fz = xyz[3 * i + 2] * scale / 1.0;
+    uint32_t px = 0, py = 0, pz = 0;
+    if (fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale) {
+        px = (uint32_t)fx;
+        py = (uint32_t)fy;
+        pz = (uint32_t)fz;
+    } else {
+        atomicOr(bad, 1u);
+    }
+    keys[i] = dev_encode(s_rc, px, py, pz, md, md);
+}
+
+__global__ void k_p2o_lev(const Key *pk, size_t n, uint32_t m, uint32_t md, uint8_t *lev) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + m >= n) return;
+    uint32_t c = key_common_levels(pk[i], pk[i + m]);
+    lev[i] = (uint8_t)min(c, md - 1);
+}
+
+__global__ void k_p2o_leaf(const Key *pk, size_t n, uint32_t m, const uint8_t *lev, Key *leafkey) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    size_t lo = j >= m ? j - m : 0, hi = min(j, n - m - 1);
+    uint32_t L = 0;
+    for (size_t i = lo; i <= hi; ++i) L = max(L, (uint32_t)lev[i]);
+    leafkey[j] = key_truncate(pk[j], L + 1);
+}
+
+__global__ void k_head(const Key *k, size_t n, uint32_t *head) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    head[j] = (j == 0 || !key_eq(k[j], k[j - 1])) ? 1u : 0u;
+}
+
+__global__ void k_compact_idx(const uint32_t *flag, const uint32_t *pos, size_t n, uint32_t *out) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (flag[j]) out[pos[j]] = (uint32_t)j;
+}
+
+// Number of octants in the coarsest completion strictly between A and B (A < B disjoint).
+__device__ inline uint32_t gap_between(const Key &A, const Key &Bk) {
+    uint32_t k = key_common_levels(A, Bk), la = key_level(A), lb = key_level(Bk);
+    uint32_t c = key_digit(Bk, k + 1) - key_digit(A, k + 1) - 1;
+    for (uint32_t t = k + 2; t <= la; ++t) c += 7 - key_digit(A, t);
+    for (uint32_t t = k + 2; t <= lb; ++t) c += key_digit(Bk, t);
+    return c;
+}
+__device__ inline uint32_t gap_before(const Key &Bk) {
+    uint32_t c = 0, lb = key_level(Bk);
+    for (uint32_t t = 1; t <= lb; ++t) c += key_digit(Bk, t);
+    return c;
+}
+__device__ inline uint32_t gap_after(const Key &A) {
+    uint32_t c = 0, la = key_level(A);
+    for (uint32_t t = 1; t <= la; ++t) c += 7 - key_digit(A, t);
+    return c;
+}
+
+// q in [0, Q]: count of octants emitted by slot q (the gap before leaf q plus the leaf; slot Q: the tail gap).
+// Consecutive non-empty leaves must be disjoint and increasing; otherwise the
+// slot counts 0 and err is raised (the host then refuses the result).
+__global__ void k_gap_count(const Key *leafkey, const uint32_t *NL, size_t Q, uint32_t *cnt, uint32_t *err) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Q) return;
+    if (q == Q) {
+        cnt[q] = gap_after(leafkey[NL[Q - 1]]);
+    } else {
+        Key b = leafkey[NL[q]];
+        if (q == 0) {
+            cnt[q] = 1 + gap_before(b);
+        } else {
+            Key a = leafkey[NL[q - 1]];
+            uint32_t k = key_common_levels(a, b);
+            if (k >= min(key_level(a), key_level(b)) || key_digit(b, k + 1) <= key_digit(a, k + 1)) {
+                atomicOr(err, 1u);
+                cnt[q] = 0;
+                return;
+            }
+            cnt[q] = 1 + gap_between(a, b);
+        }
+    }
+}
+
+__device__ inline Key with_digit(const Key &base, uint32_t t, uint32_t r) {
+    Key k = key_truncate(base, t);
+    key_set_digit(k, t, r);
+    return k;
+}
+
+__global__ void k_gap_emit(const Key *leafkey, const uint32_t *NL, size_t Q, size_t n, const uint32_t *off,
+                           Key *out, uint32_t *ptb, uint32_t *ptc) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Q) return;
+    uint32_t o = off[q];
+    if (q == Q) {
+        Key A = leafkey[NL[Q - 1]];
+        uint32_t la = key_level(A);
+        for (uint32_t t = la; t >= 1; --t)
+            for (uint32_t r = key_digit(A, t) + 1; r < 8; ++r) {
+                out[o] = with_digit(A, t, r);
+                ptb[o] = (uint32_t)n;
+                ptc[o] = 0;
+                ++o;
+            }
+        return;
+    }
+    uint32_t b0 = NL[q], b1 = (q + 1 < Q) ? NL[q + 1] : (uint32_t)n;
+    Key Bk = leafkey[b0];
+    uint32_t lb = key_level(Bk), t0 = 1;
+    if (q > 0) {
+        Key A = leafkey[NL[q - 1]];
+        uint32_t k = key_common_levels(A, Bk), la = key_level(A);
+        for (uint32_t t = la; t >= k + 2; --t)
+            for (uint32_t r = key_digit(A, t) + 1; r < 8; ++r) {
+                out[o] = with_digit(A, t, r); ptb[o] = b0; ptc[o] = 0; ++o;
+            }
+        for (uint32_t r = key_digit(A, k + 1) + 1; r < key_digit(Bk, k + 1); ++r) {
+            out[o] = with_digit(A, k + 1, r); ptb[o] = b0; ptc[o] = 0; ++o;
+        }
+        t0 = k + 2;
+    }
+    for (uint32_t t = t0; t <= lb; ++t)
+        for (uint32_t r = 0; r < key_digit(Bk, t); ++r) {
+            out[o] = with_digit(Bk, t, r); ptb[o] = b0; ptc[o] = 0; ++o;
+        }
+    out[o] = Bk;
+    ptb[o] = b0;
+    ptc[o] = b1 - b0;
+}
+
+// Arrival cursor of the reference state machine at the level-maxDepth leaf k,
+// assuming the leaves before it were emitted in step (see file comment).
+__device__ inline uint64_t arrival_cursor(const Key *T, const uint32_t *tb, const uint32_t *tc, size_t k,
+                                          uint32_t md) {
+    uint64_t s = tb[k];
+    if (k > 0 && key_level(T[k - 1]) == md) {
+        Key nx;
+        if (key_next(T[k - 1], nx) && key_eq(nx, T[k])) s += tc[k];
+    }
+    return s;
+}
+
+__global__ void k_p2o_trigger(const Key *T, const uint32_t *tb, const uint32_t *tc, size_t nT, size_t n, uint32_t m,
+                              uint32_t md, uint32_t *flag) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nT) return;
+    uint32_t f = 0;
+    if (key_level(T[k]) == md) {
+        uint64_t s = arrival_cursor(T, tb, tc, k, md);
+        if (s + m < n) {
+            uint64_t E = (uint64_t)tb[k] + tc[k];
+            Key nx;
+            if (key_next(T[k], nx) && key_level(nx) == md && k + 1 < nT) E += tc[k + 1];
+            f = (E > s + m) ? 1u : 0u;
+        }
+    }
+    flag[k] = f;
+}
+
+__device__ inline size_t lower_bound_lt(const Key *a, size_t lo, size_t hi, const Key &v) {  // first a[i] >= v
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_lt(a[mid], v)) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ inline size_t lower_bound_le(const Key *a, size_t lo, size_t hi, const Key &v) {  // first a[i] > v
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_le(a[mid], v)) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Exact replay of p2oLocal (construct.cpp:1000-1061) from the state after
+// the trigger leaf k, on one thread, until the machine is back in step with the
+// textbook leaves at an octant coarser than maxDepth.
+__global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, const Key *T, size_t nT,
+                             const uint32_t *tb, const uint32_t *tc, size_t ktrig, Key *out, uint32_t cap,
+                             ReplayResult *res) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ReplayResult r;
+    r.nout = 0;
+    r.end = 0;
+    r.err = 0;
+    r.resync = Key{0, 0};
+    uint64_t s = arrival_cursor(T, tb, tc, ktrig, md);
+    Key curr;
+    if (!key_next(T[ktrig], curr)) {
+        r.end = 1;
+        *res = r;
+        return;
+    }
+    uint64_t curr_pt = s + m, next_pt = curr_pt + m;
+    Key next;
+    bool has_next = key_next(curr, next);
+    bool synced = false;
+    while (next_pt < n) {
+        if (key_level(curr) < md && curr_pt == lower_bound_lt(P, 0, n, curr)) {
+            size_t j = lower_bound_lt(T, 0, nT, curr);
+            if (j < nT && (key_eq(T[j], curr) || key_is_ancestor(curr, T[j]))) {
+                synced = true;
+                break;
+            }
+        }
+        while (key_is_ancestor(curr, P[next_pt - 1]) && key_is_ancestor(curr, P[next_pt]) && key_level(curr) < md) {
+            curr = key_first_child(curr);
+            has_next = key_next(curr, next);
+        }
+        if (key_level(curr) == md - 1) next_pt = n - 1;  // construct.cpp:1022-1032
+        if (!has_next) {  // the reference would compare points with the past-the-end sentinel
+            r.err = 2;
+            *res = r;
+            return;
+        }
+        uint64_t found = lower_bound_le(P, curr_pt, next_pt, next) - curr_pt;
+        next_pt = curr_pt + found;
+        if (r.nout >= cap) { r.err = 1; *res = r; return; }
+        out[r.nout++] = curr;
+        curr = next;
+        has_next = key_next(curr, next);
+        if (next_pt > curr_pt) curr_pt = next_pt;
+        next_pt = curr_pt + m;
+    }
+    if (synced) {
+        r.resync = curr;
+    } else {  // tail loop (:1051-1061): coarsest completion to the end of the domain
+        while (true) {
+            if (r.nout >= cap) { r.err = 1; *res = r; return; }
+            out[r.nout++] = curr;
+            Key nx;
+            if (!key_next(curr, nx)) break;
+            curr = nx;
+        }
+        r.end = 1;
+    }
+    *res = r;
+}
+
+// ------------------------------------------------------------ balance ----
+__global__ void k_minmax_level(const Key *k, size_t n, uint32_t *mm) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t l = key_level(k[i]);
+    atomicMin(&mm[0], l);
+    atomicMax(&mm[1], l);
+}
+
+__global__ void __launch_bounds__(B) k_level_hist(const Key *k, size_t n, uint32_t *hist) {
+    __shared__ uint32_t h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i < n) atomicAdd(&h[key_level(k[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Parents (as Morton keys) of the input octants, bucketed by the input level.
+__global__ void __launch_bounds__(B) k_bucket_parents(const Key *k, size_t n, uint32_t md, SfcPacked tab,
+                                                      uint32_t *cursor /*32, pre-offset*/, Key *out) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z, l;
+    dev_decode(s_mc, k[i], md, x, y, z, l);
+    uint32_t pos = atomicAdd(&cursor[l], 1u);
+    out[pos] = key_parent(morton_encode(x, y, z, l, md));
+}
+
+__global__ void k_unique_flag(const Key *k, size_t n, uint32_t *flag) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    flag[j] = (j == 0 || !key_eq(k[j], k[j - 1])) ? 1u : 0u;
+}
+__global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32_t *pos, size_t n, Key *out) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (flag[j]) out[pos[j]] = in[j];
+}
+
+// Parents of the 26 neighbours of every split octant: per axis the neighbours'
+// parents take at most two values, so at most 8 distinct parents per octant.
+__global__ void k_bal_nbr_parents(const Key *Q, size_t nq, uint32_t lvl, uint32_t md, Key *out, uint32_t *counter) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    uint32_t x, y, z, l;
+    morton_decode(Q[i], md, x, y, z, l);
+    const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
+    int64_t cx[2], cy[2], cz[2];
+    int64_t v[3] = {x, y, z};
+    int64_t *c[3] = {cx, cy, cz};
+    for (int a = 0; a < 3; ++a) {
+        int64_t own = (v[a] / plen) * plen;
+        int64_t other = (v[a] % plen) ? own + plen : own - plen;
+        c[a][0] = own;
+        c[a][1] = other;
+    }
+    Key loc[8];
+    int nl = 0;
+    for (int bz = 0; bz < 2; ++bz)
+        for (int by = 0; by < 2; ++by)
+            for (int bx = 0; bx < 2; ++bx) {
+                int64_t px = cx[bx], py = cy[by], pz = cz[bz];
+                if (px < 0 || py < 0 || pz < 0 || px >= dom || py >= dom || pz >= dom) continue;
+                loc[nl++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
+            }
+    uint32_t base = atomicAdd(counter, (uint32_t)nl);
+    for (int j = 0; j < nl; ++j) out[base + j] = loc[j];
+}
+
+// For every split octant at the finer level, mark its child bit in the parent (Qp sorted unique).
+__global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, size_t np, uint32_t *cmask) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    Key par = key_parent(Qc[i]);
+    size_t j = lower_bound_lt(Qp, 0, np, par);
+    if (j >= np || !key_eq(Qp[j], par)) return;  // cannot happen: split octants are closed under parents
+    uint32_t d = key_digit(Qc[i], key_level(Qc[i]));
+    atomicOr(&cmask[j], 1u << d);
+}
+
+__global__ void k_leaf_count(const uint32_t *cmask, size_t nq, uint32_t *cnt) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    cnt[i] = 8 - __popc(cmask[i] & 255u);
+}
+
+__global__ void __launch_bounds__(B) k_leaf_emit(const Key *Q, size_t nq, const uint32_t *cmask, const uint32_t *off,
+                                                 uint32_t md, SfcPacked tab, Key *out) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= nq) return;
+    uint32_t x, y, z, l;
+    morton_decode(Q[i], md, x, y, z, l);
+    uint32_t len = 1u << (md - l - 1), o = off[i], msk = cmask[i];
+    for (uint32_t c = 0; c < 8; ++c) {
+        if (msk & (1u << c)) continue;
+        out[o++] = dev_encode(s_rc, x + (c & 1) * len, y + ((c >> 1) & 1) * len, z + ((c >> 2) & 1) * len, l + 1, md);
+    }
+}
+
+__global__ void __launch_bounds__(B) k_export_tn(const Key *k, size_t n, uint32_t md, SfcPacked tab, uint32_t *tn) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    uint32_t x, y, z, l;
+    dev_decode(s_mc, k[i], md, x, y, z, l);
+    uint32_t *o = tn + 7 * i;
+    o[0] = x; o[1] = y; o[2] = z; o[3] = l; o[4] = 1; o[5] = 3; o[6] = md;
+}
+
+__global__ void __launch_bounds__(B) k_import_tn(const uint32_t *tn, size_t n, uint32_t md, SfcPacked tab, Key *k,
+                                                 uint32_t *bad) {
+    LOAD_TABLES(tab)
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t *t = tn + 7 * i;
+    uint32_t l = t[3] & 31u;
+    uint32_t x = t[0], y = t[1], z = t[2];
+    if (l > md || (l < md && ((x | y | z) & ((1u << (md - l)) - 1u))) || (x >> md) || (y >> md) || (z >> md))
+        atomicOr(bad, 1u);
+    k[i] = dev_encode(s_rc, x, y, z, l, md);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host ----
+static void copy_keys(Key *dst, const Key *src, size_t n, hipStream_t s) {
+    if (n) HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+}
+
+DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, int curve, hipStream_t s,
+                      P2OStats *stats) {
+    SfcPacked tab = make_packed(curve);
+    if (md == 0 || n <= m) {
+        DevBuf<Key> root(1, s);
+        Key r{0ull, 0ull};
+        HIP_OK(hipMemcpyAsync(root.p, &r, sizeof(Key), hipMemcpyHostToDevice, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (stats) *stats = P2OStats{};
+        return root;
+    }
+    DevBuf<Key> pk(n, s);
+    DevBuf<uint32_t> bad(1, s);
+    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
+    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, tab, pk.p, bad.p);
+    HIP_OK(hipGetLastError());
+    radix_sort_keys(pk.p, nullptr, n, s, 0, 128);
+    uint32_t hbad = 0;
+    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+    // textbook leaves
+    DevBuf<uint8_t> lev(n - m, s);
+    k_p2o_lev<<<grid_for(n - m, B), B, 0, s>>>(pk.p, n, m, md, lev.p);
+    DevBuf<Key> leafkey(n, s);
+    k_p2o_leaf<<<grid_for(n, B), B, 0, s>>>(pk.p, n, m, lev.p, leafkey.p);
+    lev.release();
+    DevBuf<uint32_t> head(n, s), hpos(n, s);
+    k_head<<<grid_for(n, B), B, 0, s>>>(leafkey.p, n, head.p);
+    uint64_t Q = scan_exclusive_u32(head.p, hpos.p, n, s);
+    if (hbad) {
+        dgx_set_error("points2Octree: coordinates must lie in [0,1)");
+        throw DgxError();
+    }
+    DevBuf<uint32_t> NL(Q, s);
+    k_compact_idx<<<grid_for(n, B), B, 0, s>>>(head.p, hpos.p, n, NL.p);
+    head.release();
+    hpos.release();
+    DevBuf<uint32_t> cnt(Q + 1, s), off(Q + 1, s);
+    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
+    k_gap_count<<<grid_for(Q + 1, B), B, 0, s>>>(leafkey.p, NL.p, Q, cnt.p, bad.p);
+    uint64_t nT = scan_exclusive_u32(cnt.p, off.p, Q + 1, s);
+    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (hbad) {
+        dgx_set_error("points2Octree: internal error, non-empty leaves overlap");
+        throw DgxError();
+    }
+    cnt.release();
+    DevBuf<Key> T(nT, s);
+    DevBuf<uint32_t> tb(nT, s), tc(nT, s);
+    k_gap_emit<<<grid_for(Q + 1, B), B, 0, s>>>(leafkey.p, NL.p, Q, n, off.p, T.p, tb.p, tc.p);
+    HIP_OK(hipGetLastError());
+    leafkey.release();
+    NL.release();
+    off.release();
+    // saturated-cell triggers
+    DevBuf<uint32_t> flag(nT, s), fpos(nT, s);
+    k_p2o_trigger<<<grid_for(nT, B), B, 0, s>>>(T.p, tb.p, tc.p, nT, n, m, md, flag.p);
+    uint64_t ntrig = scan_exclusive_u32(flag.p, fpos.p, nT, s);
+    P2OStats st{};
+    st.textbook_leaves = nT;
+    st.triggers = ntrig;
+    if (ntrig == 0) {
+        if (stats) *stats = st;
+        return T;
+    }
+    DevBuf<uint32_t> trig(ntrig, s);
+    k_compact_idx<<<grid_for(nT, B), B, 0, s>>>(flag.p, fpos.p, nT, trig.p);
+    std::vector<uint32_t> htrig(ntrig);
+    HIP_OK(hipMemcpyAsync(htrig.data(), trig.p, ntrig * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    struct Seg { int src; size_t begin, count; };  // src 0: T, 1: replay buffer
+    std::vector<Seg> segs;
+    const uint32_t cap = 1u << 20;
+    DevBuf<Key> rbuf(cap, s);
+    DevBuf<ReplayResult> dres(1, s);
+    size_t t_next = 0, rused = 0;
+    for (uint32_t k : htrig) {
+        if (k < t_next) continue;  // inside a replayed stretch
+        segs.push_back({0, t_next, (size_t)k + 1 - t_next});
+        k_p2o_replay<<<1, 64, 0, s>>>(pk.p, n, m, md, T.p, nT, tb.p, tc.p, k, rbuf.p + rused,
+                                     (uint32_t)(cap - rused), dres.p);
+        HIP_OK(hipGetLastError());
+        ReplayResult hr;
+        HIP_OK(hipMemcpyAsync(&hr, dres.p, sizeof(hr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (hr.err) {
+            dgx_set_error("points2Octree: cursor replay failed (code %u) after leaf %u", hr.err, k);
+            throw DgxError();
+        }
+        segs.push_back({1, rused, hr.nout});
+        rused += hr.nout;
+        st.replayed_leaves += hr.nout;
+        st.replays++;
+        if (hr.end) { t_next = nT; break; }
+        // first textbook leaf at or inside the resync octant
+        std::vector<Key> hT;  // binary search on device would need another kernel; T is sorted, search via host copy of a window
+        size_t lo = k + 1, hi = nT;
+        while (lo < hi) {  // lower_bound on device array through small copies
+            size_t mid = lo + (hi - lo) / 2;
+            Key km;
+            HIP_OK(hipMemcpy(&km, T.p + mid, sizeof(Key), hipMemcpyDeviceToHost));
+            if (key_lt(km, hr.resync)) lo = mid + 1; else hi = mid;
+        }
+        t_next = lo;
+    }
+    if (t_next < nT) segs.push_back({0, t_next, nT - t_next});
+    size_t total = 0;
+    for (auto &g : segs) total += g.count;
+    DevBuf<Key> outk(total, s);
+    size_t o = 0;
+    for (auto &g : segs) {
+        copy_keys(outk.p + o, (g.src ? rbuf.p : T.p) + g.begin, g.count, s);
+        o += g.count;
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    st.leaves = total;
+    if (stats) *stats = st;
+    return outk;
+}
+
+DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipStream_t s, BalStats *stats) {
+    DevBuf<Key> out;
+    BalStats st{};
+    st.levels = 0;
+    auto copy_input = [&]() {
+        out.alloc(n, s);
+        copy_keys(out.p, in, n, s);
+        HIP_OK(hipStreamSynchronize(s));
+        if (stats) *stats = st;
+        return std::move(out);
+    };
+    if (n < 2) return copy_input();  // comboRipple: in.size() < 2 (balance.cpp:806-808)
+    DevBuf<uint32_t> mm(2, s);
+    uint32_t init[2] = {31u, 0u};
+    HIP_OK(hipMemcpyAsync(mm.p, init, 8, hipMemcpyHostToDevice, s));
+    k_minmax_level<<<grid_for(n, B), B, 0, s>>>(in, n, mm.p);
+    uint32_t hmm[2];
+    HIP_OK(hipMemcpyAsync(hmm, mm.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (hmm[1] - hmm[0] < 2) return copy_input();  // TreeNode.cpp:929-947
+    SfcPacked tab = make_packed(curve);
+    // bucket the parents of the inputs by input level
+    DevBuf<uint32_t> lh(32, s), cursor(32, s);
+    HIP_OK(hipMemsetAsync(lh.p, 0, 128, s));
+    k_level_hist<<<grid_for(n, B), B, 0, s>>>(in, n, lh.p);
+    uint32_t hl[32];
+    HIP_OK(hipMemcpyAsync(hl, lh.p, 128, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    uint32_t loff[33];
+    loff[0] = 0;
+    for (int l = 0; l < 32; ++l) loff[l + 1] = loff[l] + hl[l];
+    HIP_OK(hipMemcpyAsync(cursor.p, loff, 128, hipMemcpyHostToDevice, s));
+    DevBuf<Key> inpar(n, s);
+    k_bucket_parents<<<grid_for(n, B), B, 0, s>>>(in, n, md, tab, cursor.p, inpar.p);
+    HIP_OK(hipGetLastError());
+    const uint32_t maxL = hmm[1];
+    DevBuf<Key> prevQ;        // split octants one level finer (sorted unique Morton)
+    size_t nprev = 0;
+    DevBuf<Key> nbpar;        // parents of neighbours generated for the current level
+    size_t nnb = 0;
+    std::vector<DevBuf<Key>> leaves;
+    std::vector<size_t> nleaves;
+    DevBuf<uint32_t> counter(1, s);
+    for (uint32_t l = maxL; l >= 1; --l) {
+        size_t nin = hl[l];
+        size_t nw = nin + nnb;
+        if (nw == 0) {
+            prevQ.release();
+            nprev = 0;
+            continue;
+        }
+        DevBuf<Key> W(nw, s);
+        copy_keys(W.p, inpar.p + loff[l], nin, s);
+        copy_keys(W.p + nin, nbpar.p, nnb, s);
+        nbpar.release();
+        nnb = 0;
+        // parents live at level l-1: only digits 1..l-1 vary
+        radix_sort_keys(W.p, nullptr, nw, s, 0, 128);
+        DevBuf<uint32_t> f(nw, s), fp(nw, s);
+        k_unique_flag<<<grid_for(nw, B), B, 0, s>>>(W.p, nw, f.p);
+        uint64_t nq = scan_exclusive_u32(f.p, fp.p, nw, s);
+        DevBuf<Key> Q(nq, s);
+        k_compact_keys<<<grid_for(nw, B), B, 0, s>>>(W.p, f.p, fp.p, nw, Q.p);
+        W.release();
+        f.release();
+        fp.release();
+        // leaves at level l: children of Q not split themselves
+        DevBuf<uint32_t> cmask(nq, s), lc(nq, s), lo(nq, s);
+        HIP_OK(hipMemsetAsync(cmask.p, 0, nq * 4, s));
+        if (nprev) k_mark_split_children<<<grid_for(nprev, B), B, 0, s>>>(prevQ.p, nprev, Q.p, nq, cmask.p);
+        k_leaf_count<<<grid_for(nq, B), B, 0, s>>>(cmask.p, nq, lc.p);
+        uint64_t nlv = scan_exclusive_u32(lc.p, lo.p, nq, s);
+        DevBuf<Key> lv(nlv, s);
+        k_leaf_emit<<<grid_for(nq, B), B, 0, s>>>(Q.p, nq, cmask.p, lo.p, md, tab, lv.p);
+        HIP_OK(hipGetLastError());
+        leaves.push_back(std::move(lv));
+        nleaves.push_back(nlv);
+        // neighbours of the split octants seed the next coarser level
+        if (l >= 2) {
+            nbpar.alloc(8 * nq, s);
+            HIP_OK(hipMemsetAsync(counter.p, 0, 4, s));
+            k_bal_nbr_parents<<<grid_for(nq, B), B, 0, s>>>(Q.p, nq, l - 1, md, nbpar.p, counter.p);
+            uint32_t hc;
+            HIP_OK(hipMemcpyAsync(&hc, counter.p, 4, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            nnb = hc;
+        }
+        prevQ = std::move(Q);
+        nprev = nq;
+        st.levels++;
+        st.seeds += nw;
+    }
+    size_t total = 0;
+    for (size_t v : nleaves) total += v;
+    out.alloc(total, s);
+    size_t o = 0;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        copy_keys(out.p + o, leaves[i].p, nleaves[i], s);
+        o += nleaves[i];
+    }
+    leaves.clear();
+    radix_sort_keys(out.p, nullptr, total, s, 0, 128);
+    HIP_OK(hipStreamSynchronize(s));
+    st.leaves = total;
+    if (stats) *stats = st;
+    return out;
+}
+
+void export_treenodes(const Key *k, size_t n, uint32_t md, int curve, uint32_t *d_tn, hipStream_t s) {
+    if (!n) return;
+    k_export_tn<<<grid_for(n, B), B, 0, s>>>(k, n, md, make_packed(curve), d_tn);
+    HIP_OK(hipGetLastError());
+}
+
+DevBuf<Key> import_treenodes(const uint32_t *d_tn, size_t n, uint32_t md, int curve, bool sort, hipStream_t s) {
+    DevBuf<Key> k(n, s);
+    DevBuf<uint32_t> bad(1, s);
+    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
+    if (n) k_import_tn<<<grid_for(n, B), B, 0, s>>>(d_tn, n, md, make_packed(curve), k.p, bad.p);
+    HIP_OK(hipGetLastError());
+    uint32_t hb = 0;
+    HIP_OK(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (hb) {
+        dgx_set_error("TreeNode array: anchor not aligned to its level or outside the domain");
+        throw DgxError();
+    }
+    if (sort) radix_sort_keys(k.p, nullptr, n, s, 0, 128);
+    return k;
+}

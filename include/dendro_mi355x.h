@@ -1,0 +1,118 @@
+/*
+ * dendro_mi355x.h -- C ABI of the MI355X octree-FEM path.
+ *
+ * Drop-in boundary for the one-rank (MPI_COMM_SELF) hot path of Dendro4:
+ * points -> ot::points2Octree -> ot::balanceOctree -> ot::DA -> -lap+I matvec
+ * -> DAMG-preconditioned CG.  Every entry point returns 0 on success and a
+ * negative value on error; dgx_last_error() then describes the failure.
+ * Handles own device-resident state (keys, DA maps, vectors) on the calling
+ * process's current device; host arrays are copied only when a call asks for
+ * them.  Not thread-safe: calls on one process are serialised by the caller,
+ * as in the reference (global Hilbert tables, hcurvedata.h:42-43).
+ *
+ * TreeNode arrays are 7 x uint32 per octant with the ot::TreeNode layout
+ * (include/TreeNode.h:42): x, y, z, level|flags, weight, dim, maxDepth.
+ */
+#ifndef DENDRO_MI355X_H
+#define DENDRO_MI355X_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dgx_octree_s *dgx_octree;
+typedef struct dgx_da_s *dgx_da;
+
+/* Curves: DGX_CURVE_HILBERT matches a reference build with HILBERT_ORDERING
+ * (its default, CMakeLists.txt:48); DGX_CURVE_MORTON matches one without. */
+enum { DGX_CURVE_HILBERT = 0, DGX_CURVE_MORTON = 1 };
+
+/* ---- runtime ------------------------------------------------------------ */
+/* Selects the device and creates the library stream and memory pool.
+ * Replaces the one-time set-up of _InitializeHcurve (hcurvedata.h:45) and
+ * DA_Initialize (odaUtils.cpp:3352). */
+int dgx_init(int device);
+const char *dgx_last_error(void);
+/* The hipStream_t all library work is queued on. */
+void *dgx_stream(void);
+int dgx_synchronize(void);
+int dgx_device_malloc(void **p, size_t bytes);
+int dgx_device_free(void *p);
+int dgx_memcpy_htod(void *dst, const void *src, size_t bytes);
+int dgx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int dgx_memset_device(void *dst, int value, size_t bytes);
+
+/* ---- octree ------------------------------------------------------------- */
+/* ot::points2Octree on one rank (include/octUtils.h:633-634; construct.cpp:
+ * 759-774 -> points2OctreeSeq :911-974 -> p2oLocal :976-1077).  xyz holds n
+ * interleaved points in [0,1)^3 (gLens = 1); on_device selects whether xyz is
+ * a device pointer.  The result is sorted, linear and complete. */
+int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts, int curve,
+                         int on_device, dgx_octree *out);
+/* ot::balanceOctree on one rank (octUtils.h:692-694; balance.cpp:31-64):
+ * 2:1 balance across faces, edges and corners.  inc_corner is accepted for
+ * signature parity; in 3-D the reference always balances across corners
+ * (TreeNode.cpp:1075 "dim == 3 || incCorner"). */
+int dgx_balance_octree(dgx_octree in, int inc_corner, dgx_octree *out);
+/* par::sampleSort for TreeNodes on one rank (parUtils.h:424-425 ->
+ * omp_par::merge_sort): import n TreeNodes and sort them on the curve. */
+int dgx_octree_import(const uint32_t *treenodes, size_t n, unsigned max_depth, int curve, int on_device, int sort,
+                      dgx_octree *out);
+int dgx_octree_size(dgx_octree o, size_t *n);
+int dgx_octree_max_depth(dgx_octree o, unsigned *max_depth);
+/* Copy the octants out as TreeNodes (n*7 uint32, weight 1, dim 3). */
+int dgx_octree_export(dgx_octree o, uint32_t *treenodes, int on_device);
+/* Counters of the last build: textbook leaves, saturated-cell triggers,
+ * replays, replayed leaves (points2Octree) or levels, seeds (balance). */
+int dgx_octree_stats(dgx_octree o, uint64_t out[5]);
+int dgx_octree_free(dgx_octree o);
+
+/* ---- DA ----------------------------------------------------------------- */
+typedef struct {
+    uint64_t local_buffer_size;  /* getLocalBufferSize(): elements + pseudo-octants */
+    uint64_t element_size;       /* getElementSize() */
+    uint64_t node_size;          /* m_uiNodeSize */
+    uint64_t boundary_node_size; /* m_uiBoundaryNodeSize */
+    uint64_t n_nodes;            /* getNodeSize() = node_size + boundary_node_size */
+    uint32_t max_depth;          /* getMaxDepth(): octree depth + 1 */
+    uint32_t ghost_scans;        /* node-list entries resolved by the forward ghost scan */
+} dgx_da_info;
+
+/* ot::DA::DA(in, MPI_COMM_SELF, MPI_COMM_SELF, tol, compressLut=false)
+ * (include/oda/oda.h:622-624; oda.cpp:250-297).  The octree is not consumed. */
+int dgx_da_create(dgx_octree octree, dgx_da *out);
+int dgx_da_get_info(dgx_da da, dgx_da_info *info);
+/* Host copies of the DA maps; any pointer may be NULL.
+ *   nlist: 8*element_size local-buffer indices (m_uiNlist, DA::getNodeIndices)
+ *   masks: element_size hanging masks (DA::getHangingNodeIndex)
+ *   oct_levels: local_buffer_size flags (m_ucpOctLevels: level|BOUNDARY|NODE)
+ *   local_to_node: local index -> compact node id, 0xffffffff if not a node
+ *     (the vecGetBuffer compaction, oda.cpp:863-882) */
+int dgx_da_export(dgx_da da, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels, uint32_t *local_to_node);
+/* y = (lap_fac * K + mass_fac * M) x on compact node vectors (device pointers
+ * of length n_nodes).  Jacobian1MatMult (examples/src/backend/odaJac.C:
+ * 297-366) with type-2 hanging stencils; lapFac/massFac as read from the
+ * options -lapMatPropFac / -massMatPropFac (odaJac.C:313-317). */
+int dgx_da_matvec(dgx_da da, const double *x, double *y, double lap_fac, double mass_fac);
+int dgx_da_matvec_f32(dgx_da da, const float *x, float *y, float lap_fac, float mass_fac);
+/* Jacobian1MatGetDiagonal (odaJac.C:160-227) on a compact node vector. */
+int dgx_da_diagonal(dgx_da da, double *diag, double lap_fac, double mass_fac);
+int dgx_da_free(dgx_da da);
+
+/* ---- inputs ------------------------------------------------------------- */
+/* Deterministic synthetic point clouds (host memory, n*3 doubles):
+ *   dist 0: i.i.d. uniform [0,1)^3
+ *   dist 1: each coordinate exp(N(-1, 0.5)) clipped to [0,1)
+ *   dist 2: mixture of n_gauss isotropic Gaussians, centres U[lo,hi]^3,
+ *           sigma sig_lo (or log-uniform in [sig_lo, sig_hi] when sig_hi > 0),
+ *           equal weights (or Dirichlet(1) weights when dirichlet != 0), clipped.
+ * SplitMix64 keyed by (seed, stream, counter); Box-Muller normals. */
+int dgx_gen_points(int dist, uint64_t seed, size_t n, int n_gauss, double lo, double hi, double sig_lo, double sig_hi,
+                   int dirichlet, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

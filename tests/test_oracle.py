@@ -1,0 +1,374 @@
+"""CPU tests that pin the oracle (oracle/dendro_oracle.c) to what the
+reference's data, tests and the mathematics fix.  No GPU involved."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+MD = 30
+
+
+def tn(x, y, z, lev, md=MD):
+    return np.array([x, y, z, lev, 1, 3, md], np.uint32)
+
+
+# ---------------------------------------------------------------- tables ----
+def test_hilbert_tables_are_a_hilbert_curve(oracle):
+    """rotations halves are inverse permutations, consecutive children are
+    face-adjacent, and sub-curves join across child boundaries
+    (src/hcurvedata.cpp:329-546 pinned by the defining property)."""
+    rot, ht = oracle.tables()
+    rot, ht = rot.astype(int), ht.astype(int)
+    for r in range(24):
+        perm, rank = rot[r, :8], rot[r, 8:]
+        assert sorted(perm) == list(range(8))
+        assert all(rank[perm[k]] == k for k in range(8))
+        for k in range(7):  # Morton indices of consecutive ranks differ in one bit
+            d = perm[k] ^ perm[k + 1]
+            assert d in (1, 2, 4)
+
+    def cells(level, r=0, base=(0, 0, 0), size=1 << 3):
+        if level == 0:
+            return [base]
+        out = []
+        h = size // 2
+        for k in range(8):
+            m = rot[r, k]
+            c = (base[0] + (m & 1) * h, base[1] + ((m >> 1) & 1) * h, base[2] + ((m >> 2) & 1) * h)
+            out += cells(level - 1, ht[r, m], c, h)
+        return out
+
+    seq = cells(3)
+    assert len(set(seq)) == 512
+    for a, b in zip(seq, seq[1:]):
+        assert sum(abs(p - q) for p, q in zip(a, b)) == 1
+
+
+def test_comparator_orders_cells_along_the_curve(oracle):
+    """Sorting all level-3 cells with TreeNode::operator< gives a face-connected path."""
+    md = 3
+    cells = np.array([tn(x, y, z, 3, md) for x in range(8) for y in range(8) for z in range(8)], np.uint32)
+    rng = np.random.default_rng(1)
+    s = oracle.sort(cells[rng.permutation(len(cells))])
+    xyz = s[:, :3].astype(int)
+    assert np.all(np.abs(np.diff(xyz, axis=0)).sum(1) == 1)
+    # hierarchical: every level-1 octant's cells are contiguous
+    par = (xyz // 4)
+    changes = (np.abs(np.diff(par, axis=0)).sum(1) > 0).sum()
+    assert changes == 7
+
+
+def test_comparator_ancestor_first_and_total(oracle):
+    rng = np.random.default_rng(2)
+    md = 6
+    octs = []
+    for _ in range(300):
+        lev = int(rng.integers(0, md + 1))
+        s = 1 << (md - lev)
+        x, y, z = (rng.integers(0, 1 << lev, 3) * s).tolist()
+        octs.append(tn(x, y, z, lev, md))
+    octs = np.unique(np.array(octs), axis=0)
+    for a in octs[:60]:
+        for b in octs[:60]:
+            if (a[:4] == b[:4]).all():
+                continue
+            la, lb = int(a[3]), int(b[3])
+            anc = la < lb and all(a[i] <= b[i] < a[i] + (1 << (md - la)) for i in range(3))
+            if anc:
+                assert oracle.tn_less(a, b) and not oracle.tn_less(b, a)
+            assert oracle.tn_less(a, b) != oracle.tn_less(b, a)
+
+
+# ------------------------------------------------------------------- p2o ----
+def textbook_leaves(q, m, md):
+    """Brute force: split an octant iff it holds more than m points and level < md."""
+    out = []
+
+    def rec(x, y, z, lev, idx):
+        if len(idx) <= m or lev == md:
+            out.append((x, y, z, lev))
+            return
+        h = 1 << (md - lev - 1)
+        for c in range(8):
+            cx, cy, cz = x + (c & 1) * h, y + ((c >> 1) & 1) * h, z + ((c >> 2) & 1) * h
+            sel = idx[(q[idx, 0] >= cx) & (q[idx, 0] < cx + h) & (q[idx, 1] >= cy) & (q[idx, 1] < cy + h) &
+                      (q[idx, 2] >= cz) & (q[idx, 2] < cz + h)]
+            rec(cx, cy, cz, lev + 1, sel)
+
+    rec(0, 0, 0, 0, np.arange(len(q)))
+    return out
+
+
+@pytest.mark.parametrize("n,m,seed", [(300, 1, 0), (500, 4, 1), (2000, 8, 2), (50, 1, 3)])
+def test_p2o_equals_textbook_octree_without_clusters(oracle, n, m, seed):
+    """Away from saturated maxDepth cells p2oLocal (construct.cpp:976-1077)
+    produces exactly the textbook octree."""
+    pts = np.random.default_rng(seed).random((n, 3))
+    md = 12
+    got = oracle.points2octree(pts, md, m)
+    q = oracle.quantise(pts, md)[:, :3].astype(np.int64)
+    ref = textbook_leaves(q, m, md)
+    ref = oracle.sort(np.array([tn(*r, md=md) for r in ref], np.uint32))
+    assert np.array_equal(got, ref)
+    assert oracle.is_complete(got)
+
+
+def test_p2o_small_cases(oracle):
+    pts = np.array([[0.1, 0.1, 0.1]])
+    got = oracle.points2octree(pts, 30, 1)
+    assert got.tolist() == [[0, 0, 0, 0, 1, 3, 30]]  # n <= maxNumPts: root (construct.cpp:948-952)
+    got = oracle.points2octree(np.random.default_rng(0).random((2, 3)), 0, 1)
+    assert got.tolist() == [[0, 0, 0, 0, 1, 3, 0]]  # maxDepth 0
+
+
+def test_p2o_saturated_cell_hand_traced(oracle):
+    """p2oLocal with a stale cursor (construct.cpp:1000-1061), traced by hand.
+    maxDepth 2, maxNumPts 1, cells A=(2,2,1) x3 and B=(3,0,0) x2.  Under the
+    root rotation the level-1 curve order is Morton 0,2,3,1,5,7,6,4 and inside
+    octant (2,2,0) (rotation 0 again) A is the last child.  The window at the
+    successor of A's left neighbour already counts the first copy of A (it is
+    <= next), the window at A consumes one more, so one copy of A is stale when
+    the cursor reaches octant (2,0,0,1), which holds B twice.  Its refinement
+    test reads nodes[2]=A (outside) and nodes[3]=B, so it is emitted unrefined
+    and the widened window at level maxDepth-1 (:1022-1032) consumes A and both
+    B.  The textbook tree would split (2,0,0,1) into 8 level-2 cells."""
+    pts = (np.array([[2, 2, 1]] * 3 + [[3, 0, 0]] * 2) + 0.5) / 4
+    got = oracle.points2octree(pts, 2, 1)
+    expect = [[0, 0, 0, 1], [0, 2, 0, 1],
+              [2, 2, 0, 2], [2, 3, 0, 2], [3, 3, 0, 2], [3, 2, 0, 2],
+              [3, 2, 1, 2], [3, 3, 1, 2], [2, 3, 1, 2], [2, 2, 1, 2],
+              [2, 0, 0, 1], [2, 0, 2, 1], [2, 2, 2, 1], [0, 2, 2, 1], [0, 0, 2, 1]]
+    assert got[:, :4].tolist() == expect
+    assert oracle.is_complete(got)
+    q = oracle.quantise(pts, 2)[:, :3].astype(np.int64)
+    ref = oracle.sort(np.array([tn(*r, md=2) for r in textbook_leaves(q, 1, 2)], np.uint32))
+    assert len(ref) == len(got) + 7
+
+
+def test_p2o_stale_cursor_at_domain_end_is_reported(oracle):
+    """When a stale cursor is still active at the last octant, the reference
+    compares points with getNext's past-the-end sentinel (TreeNode.tcc:483-492)
+    whose maxDepth is one larger; operator< then underflows ncaLev.  The oracle
+    (and the library) report this instead of looping."""
+    pts = (np.array([[0, 0, 2]] * 4) + 0.5) / 4  # Morton 4 at level 1 is the last octant of the curve
+    with pytest.raises(ValueError):
+        oracle.points2octree(pts, 2, 1)
+
+
+# --------------------------------------------------------------- balance ----
+def ripple_fixpoint(leaves, md):
+    """Brute force minimal balanced refinement: split any leaf that touches a
+    leaf two or more levels finer (faces, edges, corners) until nothing changes."""
+    S = {tuple(l) for l in leaves}
+    changed = True
+    while changed:
+        changed = False
+        cells = {}
+        for (x, y, z, l) in S:
+            cells[(x, y, z, l)] = True
+        lst = sorted(S, key=lambda t: -t[3])
+        for (x, y, z, l) in lst:
+            s = 1 << (md - l)
+            for dx, dy, dz in itertools.product((-1, 0, 1), repeat=3):
+                if not (dx or dy or dz):
+                    continue
+                px = x - 1 if dx < 0 else (x + s if dx > 0 else x)
+                py = y - 1 if dy < 0 else (y + s if dy > 0 else y)
+                pz = z - 1 if dz < 0 else (z + s if dz > 0 else z)
+                if min(px, py, pz) < 0 or max(px, py, pz) >= (1 << md):
+                    continue
+                for L in range(0, l - 1):  # coarser by >= 2
+                    ss = 1 << (md - L)
+                    c = ((px // ss) * ss, (py // ss) * ss, (pz // ss) * ss, L)
+                    if c in S:
+                        S.remove(c)
+                        h = ss // 2
+                        for k in range(8):
+                            S.add((c[0] + (k & 1) * h, c[1] + ((k >> 1) & 1) * h, c[2] + ((k >> 2) & 1) * h, L + 1))
+                        changed = True
+                        break
+                if changed:
+                    break
+            if changed:
+                break
+    return S
+
+
+@pytest.mark.parametrize("n,m,seed", [(40, 1, 0), (80, 1, 1), (120, 2, 2)])
+def test_balance_equals_minimal_balanced_refinement(oracle, n, m, seed):
+    md = 8
+    pts = np.random.default_rng(seed).random((n, 3)) ** 2
+    u = oracle.points2octree(pts, md, m)
+    b = oracle.balance(u)
+    assert oracle.is_complete(b) and oracle.is_balanced(b)
+    ref = ripple_fixpoint(u[:, :4].tolist(), md)
+    assert {tuple(r) for r in b[:, :4].tolist()} == ref
+
+
+def test_balance_properties_and_limit(oracle):
+    pts = np.random.default_rng(7).random((3000, 3)) ** 3
+    u = oracle.points2octree(pts, 30, 1)
+    assert not oracle.is_balanced(u)
+    b = oracle.balance(u)
+    assert oracle.is_complete(b) and oracle.is_balanced(b)
+    # refines the input: every input octant is an ancestor-or-self of output leaves
+    assert len(b) >= len(u)
+    assert np.array_equal(oracle.balance(b), b)  # idempotent
+    # comboRipple's one-rank path above 1e7 octants returns nothing (balance.cpp:821-853);
+    # the emulation flag is inert below the limit
+    assert np.array_equal(oracle.balance(u, emulate_limit=True), b)
+
+
+# ---------------------------------------------------------------- stencils --
+def q1_operator():
+    k = np.array([[.5, -.5], [-.5, .5]])
+    m = np.array([[2 / 3, 1 / 3], [1 / 3, 2 / 3]])
+
+    def kron3(a, b, c):
+        K = np.zeros((8, 8))
+        for i in range(8):
+            for j in range(8):
+                K[i, j] = a[i & 1, j & 1] * b[(i >> 1) & 1, (j >> 1) & 1] * c[(i >> 2) & 1, (j >> 2) & 1]
+        return K
+
+    return kron3(k, m, m) + kron3(m, k, m) + kron3(m, m, k), kron3(m, m, m)
+
+
+def interp_matrix(c, mask):
+    C = np.eye(8)
+    for a in range(8):
+        if (mask >> a) & 1:
+            d = a ^ c
+            C[a] = 0
+            bits = [b for b in range(3) if (d >> b) & 1]
+            assert len(bits) in (1, 2)
+            if len(bits) == 1:
+                C[a, c] += .5
+                C[a, a] += .5
+            else:
+                for v in (c, a, c ^ (1 << bits[0]), c ^ (1 << bits[1])):
+                    C[a, v] += .25
+    return C
+
+
+def test_type2_stencils_are_interpolated_q1(oracle, stencils):
+    """Stencils/{Lap,Mass}Type2Stencils.inp == C^T K_Q1 C for every child number
+    and element type, with C built from the cNumEtype masks (cnumEtypes.h)."""
+    L, M = stencils
+    K0, M0 = q1_operator()
+    masks = oracle.etype_masks()
+    for c in range(8):
+        for t in range(18):
+            C = interp_matrix(c, int(masks[c, t]))
+            assert np.abs(C.T @ K0 @ C - L[c, t]).max() < 1e-14
+            assert np.abs(C.T @ M0 @ C - M[c, t]).max() < 1e-14
+    assert np.allclose(L.sum(-1), 0, atol=3e-15)
+    assert np.allclose(M.sum((-1, -2)), 8)
+
+
+def test_fixture_hashes():
+    import hashlib
+    g = np.load(os.path.join(GOLDEN, "type2_stencils.npz"))
+    h = hashlib.sha256(g["lap"].tobytes() + g["mass"].tobytes()).hexdigest()
+    assert h == open(os.path.join(GOLDEN, "type2_stencils.sha256")).read().split()[0]
+
+
+# -------------------------------------------------------------------- DA ----
+def da_geometry_check(oracle, octree):
+    md = int(octree[0, 6]) + 1
+    da = oracle.DA(octree)
+    octs = da.octs
+    E = da.n_elem
+    assert da.n_unresolved == 0
+    assert not (octs[:E, 3] & 64).any() and (octs[E:, 3] & 64).all()
+    # leaf lookup by point (brute force over the element list)
+    lv = (octs[:E, 3] & 31).astype(np.int64)
+    sz = (1 << (md - lv)).astype(np.int64)
+    org = octs[:E, :3].astype(np.int64)
+
+    def covering(p):
+        inside = np.all((org <= p) & (p < org + sz[:, None]), axis=1)
+        return np.nonzero(inside)[0]
+
+    regular = set()
+    for e in range(E):
+        x = org[e]
+        s = sz[e]
+        par = (x // (2 * s)) * (2 * s)
+        for j in range(8):
+            off = np.array([j & 1, (j >> 1) & 1, (j >> 2) & 1])
+            v = x + off * s
+            r = int(da.nlist[e, j])
+            assert octs[r, 3] & 128  # every referenced octant is a node
+            hang = (da.mask[e] >> j) & 1
+            # a vertex is regular iff it is a corner of every leaf whose closure contains it
+            touching = []
+            for dx, dy, dz in itertools.product((0, -1), repeat=3):
+                q = v + np.array([dx, dy, dz])
+                if (q < 0).any() or (q >= (1 << (md - 1))).any():
+                    continue
+                touching += list(covering(q))
+            is_regular = all(((v - org[t]) % sz[t] == 0).all() for t in set(touching))
+            if j == 0:
+                continue
+            assert bool(hang) == (not is_regular), (e, j)
+            if hang:
+                assert (octs[r, :3].astype(np.int64) == par + off * 2 * s).all()
+            else:
+                assert (octs[r, :3].astype(np.int64) == v).all()
+                regular.add(tuple(v))
+        if not (da.mask[e] & 1):
+            regular.add(tuple(x))
+    assert da.n_nodes == len(regular)
+    return da
+
+
+def test_da_uniform_grid_is_structured_q1(oracle):
+    md = 30
+    k = 2
+    s = 1 << (md - k)
+    cells = [tn(x * s, y * s, z * s, k) for x in range(4) for y in range(4) for z in range(4)]
+    o = oracle.sort(np.array(cells, np.uint32))
+    da = da_geometry_check(oracle, o)
+    assert da.n_nodes == 5 ** 3 and (da.mask == 0).all()
+    assert da.node_size == 64 and da.bnd_node_size == 125 - 64
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_da_adaptive_geometry(oracle, seed):
+    pts = np.random.default_rng(seed).random((60, 3)) ** 2
+    b = oracle.balance(oracle.points2octree(pts, 30, 1))
+    da = da_geometry_check(oracle, b)
+    assert (da.mask != 0).any()
+    for e in range(da.n_elem):
+        cn = 0
+        lev = int(da.octs[e, 3] & 31)
+        sh = 31 - lev
+        cn = ((int(da.octs[e, 2]) >> sh) & 1) << 2 | ((int(da.octs[e, 1]) >> sh) & 1) << 1 | ((int(da.octs[e, 0]) >> sh) & 1)
+        from oracle.pyoracle import lib
+        assert lib().or_elem_type(cn, int(da.mask[e])) >= 0
+
+
+def test_oracle_matvec_invariants(oracle, stencils):
+    L, M = stencils
+    pts = np.random.default_rng(3).random((2000, 3)) ** 2
+    b = oracle.balance(oracle.points2octree(pts, 30, 1))
+    da = oracle.DA(b)
+    one = np.ones(da.n_nodes)
+    assert abs(da.matvec(one, L, M, 0.0, 1.0).sum() - 1.0) < 1e-13  # mass of the unit cube
+    assert np.abs(da.matvec(one, L, M, 1.0, 0.0)).max() < 1e-12     # K 1 = 0
+    rng = np.random.default_rng(4)
+    x, z = rng.standard_normal(da.n_nodes), rng.standard_normal(da.n_nodes)
+    assert abs(z @ da.matvec(x, L, M) - x @ da.matvec(z, L, M)) < 1e-10
+    # a linear field is reproduced: x^T K u = integral of grad x . grad u = 0 for u=1 (done) and
+    # for u = x-coordinate, 1^T K u = 0 as well
+    octs, l2n = da.octs, da.local_to_node
+    idx = np.nonzero(l2n != 0xFFFFFFFF)[0]
+    pos = np.zeros(da.n_nodes)
+    pos[l2n[idx]] = octs[idx, 0] / 2.0 ** 30
+    assert abs(da.matvec(pos, L, M, 1.0, 0.0).sum()) < 1e-11
+    # energy of the linear field: u^T K u = integral |grad x|^2 = 1
+    assert abs(pos @ da.matvec(pos, L, M, 1.0, 0.0) - 1.0) < 1e-11
