@@ -49,6 +49,11 @@ class _DaInfo(C.Structure):
         ("n_nodes", C.c_uint64),
         ("max_depth", C.c_uint32),
         ("ghost_scans", C.c_uint32),
+        ("matvec_blocks", C.c_uint64),
+        ("block_node_pairs", C.c_uint64),
+        ("shared_nodes", C.c_uint64),
+        ("shared_pairs", C.c_uint64),
+        ("max_block_nodes", C.c_uint64),
     ]
 
 
@@ -263,6 +268,8 @@ class DA:
         self.n_nodes = info.n_nodes
         self.max_depth = info.max_depth
         self.ghost_scans = info.ghost_scans
+        self.block_info = {k: getattr(info, k) for k in ("matvec_blocks", "block_node_pairs", "shared_nodes",
+                                                          "shared_pairs", "max_block_nodes")}
 
     def maps(self):
         """Host copies: nlist (E,8), masks (E,), oct_levels (N,), local_to_node (N,)."""

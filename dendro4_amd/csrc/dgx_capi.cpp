@@ -284,6 +284,11 @@ int dgx_da_get_info(dgx_da d, dgx_da_info *info) {
     info->n_nodes = a.n_nodes;
     info->max_depth = a.md;
     info->ghost_scans = a.n_ghost_scans;
+    info->matvec_blocks = a.n_blocks;
+    info->block_node_pairs = a.n_pairs;
+    info->shared_nodes = a.n_shared;
+    info->shared_pairs = a.partial_slots;
+    info->max_block_nodes = a.max_block_nodes;
     return 0;
 }
 

@@ -173,6 +173,31 @@ __host__ __device__ inline void morton_decode(const Key &k, uint32_t md, uint32_
 }
 
 // ---------------------------------------------------------------------------
+// Wave64 helpers: aggregate per-lane counts so that one lane per wave touches
+// a global counter.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_excl_sum(uint32_t v, uint32_t &total) {
+    const int lane = __lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+// Reserve n slots on *counter for this lane; returns the first slot.  Every
+// lane of the wave must call it (inactive lanes pass 0).
+__device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
+    uint32_t total, ex = wave_excl_sum(n, total);
+    uint32_t base = 0;
+    if (__lane_id() == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    return base + ex;
+}
+
+// ---------------------------------------------------------------------------
 // Host-side error plumbing and device buffers.
 // ---------------------------------------------------------------------------
 void dgx_set_error(const char *fmt, ...);

@@ -19,12 +19,27 @@ struct DaDevice {
     // matvec view: conn[j*E + e] compact node id of vertex j; meta = mask | cnum<<8 | level<<11
     DevBuf<uint32_t> conn;
     DevBuf<uint16_t> meta;
+    // Block view for the matvec: elements in SFC-contiguous blocks of kElemBlock.
+    //   bnode_off[b]..bnode_off[b+1]: the block's unique node ids bnode[] (bit 31: node
+    //   also touched by another block), bdst[]: its partial slot when shared.
+    //   lconn[j*E + e]: local index of vertex j ^ c of element e (child number c) in
+    //   its block's list -- the "child frame", where the parent corner is vertex 0.
+    //   cmeta[e]: hanging mask in the child frame | original level << 8.
+    //   shared nodes: snode[s], partial slots soff[s]..soff[s+1] (ordered by block).
+    uint32_t elem_block = 512;  // elements per block (DGX_MATVEC_EB overrides at build time)
+    uint32_t n_blocks = 0, max_block_nodes = 0, n_shared = 0;
+    uint64_t n_pairs = 0, partial_slots = 0;
+    DevBuf<uint32_t> bnode_off, bnode, bdst, soff, snode;
+    DevBuf<uint16_t> lconn, cmeta;
+    DevBuf<double> partial;  // partial sums of shared nodes (float or double view), one per matvec in flight
     // Dirichlet: 1 for nodes on the domain boundary (filled on demand)
     DevBuf<uint8_t> bdy_node;
     ~DaDevice();
 };
 
 std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s);
+// Builds the matvec block view from conn (called by da_build).
+void da_build_blocks(DaDevice &da, hipStream_t s);
 
 // y = A x with A = lapFac * stiffness + massFac * mass (type-2 hanging stencils),
 // FP64 on compact node vectors (Jacobian1MatMult, odaJac.C:275-366).

@@ -17,6 +17,7 @@
 // (DA::vecGetBuffer, oda.cpp:863-882).
 #include "dgx_da.h"
 
+#include <cstdlib>
 #include <vector>
 
 namespace {
@@ -236,11 +237,14 @@ __global__ void k_da_mark_nodes(const uint32_t *nlist, size_t nelem, size_t n, u
 __global__ void k_da_apply_nodes(const uint8_t *isnode, size_t n, size_t nelem, uint8_t *flags, uint32_t *nodeflag,
                                  uint32_t *counts /* [0] elem nodes [1] bnd nodes */) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (isnode[i]) {
-        flags[i] |= F_NODE;
-        atomicAdd(&counts[i < nelem ? 0 : 1], 1u);
+    bool on = i < n && isnode[i];
+    uint32_t ce = __popcll(__ballot(on && i < nelem)), cb = __popcll(__ballot(on && i >= nelem));
+    if (__lane_id() == 0) {
+        if (ce) atomicAdd(&counts[0], ce);
+        if (cb) atomicAdd(&counts[1], cb);
     }
+    if (i >= n) return;
+    if (on) flags[i] |= F_NODE;
     nodeflag[i] = (flags[i] & F_NODE) ? 1u : 0u;
 }
 
@@ -340,6 +344,159 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     k_da_elem_view<<<grid_for(n, B), B, 0, s>>>(da->nlist.p, da->mask.p, da->l2n.p, da->x.p, da->y.p, da->z.p,
                                                da->flags.p, n, md, da->conn.p, da->meta.p);
     HIP_OK(hipGetLastError());
+    da_build_blocks(*da, s);
     HIP_OK(hipStreamSynchronize(s));
     return da;
+}
+
+// ---------------------------------------------------------------------------
+// Matvec block view.
+// ---------------------------------------------------------------------------
+namespace {
+
+// (block, node) pair keys for every element vertex.
+__global__ void k_blk_pairs(const uint32_t *conn, size_t ne, uint32_t eb, Key *pk) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * ne) return;
+    size_t e = t % ne;
+    pk[t] = Key{(uint64_t)(e / eb), (uint64_t)conn[t]};
+}
+__global__ void k_blk_head(const Key *pk, size_t n, uint32_t *head, uint32_t *bstart) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || !key_eq(pk[i], pk[i - 1])) ? 1u : 0u;
+    if (i == 0 || pk[i].hi != pk[i - 1].hi) bstart[pk[i].hi] = (uint32_t)i;  // first raw index of each block
+}
+__global__ void k_blk_compact(const Key *pk, const uint32_t *head, const uint32_t *pos, size_t n, uint32_t *bnode,
+                              uint32_t *pair_block, uint32_t *ncount) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !head[i]) return;
+    uint32_t p = pos[i];
+    bnode[p] = (uint32_t)pk[i].lo;
+    pair_block[p] = (uint32_t)pk[i].hi;
+    atomicAdd(&ncount[pk[i].lo], 1u);
+}
+__global__ void k_blk_offsets(const uint32_t *bstart, const uint32_t *pos, uint32_t nblk, uint64_t npairs,
+                              uint32_t *boff) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblk) return;
+    boff[b] = (b == nblk) ? (uint32_t)npairs : pos[bstart[b]];
+}
+// Local index of every element vertex inside its block's sorted node list,
+// stored in the child frame (slot j ^ c for vertex j of child c).
+__global__ void k_blk_lconn(const uint32_t *conn, const uint16_t *meta, size_t ne, uint32_t eb, const uint32_t *boff,
+                            const uint32_t *bnode, uint16_t *lconn) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8 * ne) return;
+    size_t e = t % ne, j = t / ne;
+    uint32_t b = (uint32_t)(e / eb), g = conn[t], c = (meta[e] >> 8) & 7u;
+    uint32_t lo = boff[b], hi = boff[b + 1];
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (bnode[mid] < g) lo = mid + 1; else hi = mid;
+    }
+    lconn[(j ^ c) * ne + e] = (uint16_t)(lo - boff[b]);
+}
+__global__ void k_blk_cmeta(const uint16_t *meta, size_t ne, uint16_t *cmeta) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    uint32_t mt = meta[e], mask = mt & 255u, c = (mt >> 8) & 7u, cm = 0;
+    for (uint32_t j = 0; j < 8; ++j)
+        if ((mask >> (j ^ c)) & 1u) cm |= 1u << j;
+    cmeta[e] = (uint16_t)(cm | ((mt >> 11) << 8));
+}
+__global__ void k_blk_shared_flag(const uint32_t *bnode, const uint32_t *ncount, uint64_t np, uint32_t *flag) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    flag[p] = ncount[bnode[p]] > 1 ? 1u : 0u;
+}
+// Node-major keys (node, block) of the shared pairs, payload = pair index.
+__global__ void k_blk_shared_keys(const uint32_t *bnode, const uint32_t *pair_block, const uint32_t *flag,
+                                  const uint32_t *pos, uint64_t np, Key *sk, uint32_t *sv) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np || !flag[p]) return;
+    uint32_t q = pos[p];
+    sk[q] = Key{(uint64_t)bnode[p], (uint64_t)pair_block[p]};
+    sv[q] = (uint32_t)p;
+}
+__global__ void k_blk_shared_slots(const Key *sk, const uint32_t *sv, uint64_t ns, uint32_t *bdst, uint32_t *bnode,
+                                   uint32_t *shead) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ns) return;
+    uint32_t p = sv[q];
+    bdst[p] = (uint32_t)q;
+    bnode[p] |= 0x80000000u;
+    shead[q] = (q == 0 || sk[q].hi != sk[q - 1].hi) ? 1u : 0u;
+}
+__global__ void k_blk_shared_nodes(const Key *sk, const uint32_t *shead, const uint32_t *spos, uint64_t ns,
+                                   uint32_t *snode, uint32_t *soff) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ns || !shead[q]) return;
+    snode[spos[q]] = (uint32_t)sk[q].hi;
+    soff[spos[q]] = (uint32_t)q;
+}
+__global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v = b < nblk ? boff[b + 1] - boff[b] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+    if (__lane_id() == 0) atomicMax(mx, v);
+}
+
+}  // namespace
+
+void da_build_blocks(DaDevice &da, hipStream_t s) {
+    const size_t E = da.n_elem, T = 8 * E;
+    const uint32_t eb = da.elem_block;
+    const uint32_t nblk = (uint32_t)((E + eb - 1) / eb);
+    DevBuf<Key> pk(T, s);
+    k_blk_pairs<<<grid_for(T, B), B, 0, s>>>(da.conn.p, E, eb, pk.p);
+    radix_sort_keys(pk.p, nullptr, T, s, 0, 128);
+    DevBuf<uint32_t> head(T, s), pos(T, s), bstart(nblk, s);
+    k_blk_head<<<grid_for(T, B), B, 0, s>>>(pk.p, T, head.p, bstart.p);
+    uint64_t np = scan_exclusive_u32(head.p, pos.p, T, s);
+    da.bnode.alloc(np, s);
+    DevBuf<uint32_t> pair_block(np, s), ncount(da.n_nodes, s);
+    HIP_OK(hipMemsetAsync(ncount.p, 0, (size_t)da.n_nodes * 4, s));
+    k_blk_compact<<<grid_for(T, B), B, 0, s>>>(pk.p, head.p, pos.p, T, da.bnode.p, pair_block.p, ncount.p);
+    da.bnode_off.alloc(nblk + 1, s);
+    k_blk_offsets<<<grid_for(nblk + 1, B), B, 0, s>>>(bstart.p, pos.p, nblk, np, da.bnode_off.p);
+    pk.release();
+    head.release();
+    pos.release();
+    bstart.release();
+    da.lconn.alloc(T, s);
+    k_blk_lconn<<<grid_for(T, B), B, 0, s>>>(da.conn.p, da.meta.p, E, eb, da.bnode_off.p, da.bnode.p, da.lconn.p);
+    da.cmeta.alloc(E, s);
+    k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, da.cmeta.p);
+    // shared nodes and their node-major partial slots
+    DevBuf<uint32_t> flag(np, s), fpos(np, s);
+    k_blk_shared_flag<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, ncount.p, np, flag.p);
+    uint64_t ns = scan_exclusive_u32(flag.p, fpos.p, np, s);
+    da.bdst.alloc(np, s);
+    HIP_OK(hipMemsetAsync(da.bdst.p, 0xff, np * 4, s));
+    if (ns) {
+        DevBuf<Key> sk(ns, s);
+        DevBuf<uint32_t> sv(ns, s), shead(ns, s), spos(ns, s);
+        k_blk_shared_keys<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, pair_block.p, flag.p, fpos.p, np, sk.p, sv.p);
+        radix_sort_keys(sk.p, sv.p, ns, s, 0, 128);
+        k_blk_shared_slots<<<grid_for(ns, B), B, 0, s>>>(sk.p, sv.p, ns, da.bdst.p, da.bnode.p, shead.p);
+        uint64_t nsn = scan_exclusive_u32(shead.p, spos.p, ns, s);
+        da.snode.alloc(nsn, s);
+        da.soff.alloc(nsn + 1, s);
+        k_blk_shared_nodes<<<grid_for(ns, B), B, 0, s>>>(sk.p, shead.p, spos.p, ns, da.snode.p, da.soff.p);
+        uint32_t tail = (uint32_t)ns;
+        HIP_OK(hipMemcpyAsync(da.soff.p + nsn, &tail, 4, hipMemcpyHostToDevice, s));
+        da.n_shared = (uint32_t)nsn;
+    }
+    DevBuf<uint32_t> mx(1, s);
+    HIP_OK(hipMemsetAsync(mx.p, 0, 4, s));
+    k_blk_maxnodes<<<grid_for(nblk, B), B, 0, s>>>(da.bnode_off.p, nblk, mx.p);
+    uint32_t hmx = 0;
+    HIP_OK(hipMemcpyAsync(&hmx, mx.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    da.n_blocks = nblk;
+    da.n_pairs = np;
+    da.max_block_nodes = hmx;
+    da.partial_slots = ns;
+    da.partial.alloc(ns, s);
 }

@@ -68,83 +68,172 @@ __device__ __forceinline__ void apply_q1(const T *u, T fac1, T fac2, T *v) {
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
+// Unscaled 1-D operators along the axis with vertex stride S:
+//   mhat = [[2,1],[1,2]] (3 x mass of [-1,1]),  khat = [[1,-1],[-1,1]] (2 x stiffness)
+template <typename T, int S>
+__device__ __forceinline__ void op_mhat(const T *u, T *o) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i & S) continue;
+        T a = u[i], b = u[i + S];
+        o[i] = fma(a, T(2), b);
+        o[i + S] = fma(b, T(2), a);
+    }
+}
+template <typename T, int S>
+__device__ __forceinline__ void op_khat(const T *u, T *o) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i & S) continue;
+        T d = u[i] - u[i + S];
+        o[i] = d;
+        o[i + S] = -d;
+    }
+}
+// v = g1 (khat mhat mhat + mhat khat mhat + mhat mhat khat) u + g2 mhat mhat mhat u
+// (x fastest), i.e. the Q1 operator with g1 = lapFac h/36, g2 = massFac h^3/216.
 template <typename T>
-__global__ void __launch_bounds__(B) k_matvec(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ meta,
-                                              size_t ne, const T *__restrict__ x, T *__restrict__ y, T lap_fac,
-                                              T mass_fac) {
-    size_t e = (size_t)blockIdx.x * B + threadIdx.x;
-    if (e >= ne) return;
-    uint32_t mt = meta[e];
-    uint32_t mask = mt & 255u, c = (mt >> 8) & 7u, lev = mt >> 11;
-    T h = T(1.0) / T(1u << lev);
-    T fac1 = lap_fac * h * T(0.5), fac2 = mass_fac * h * h * h * T(0.125);
-    uint32_t idx[8];
-    T xv[8], u[8], v[8], r[8];
+__device__ __forceinline__ void apply_q1_hat(const T *u, T g1, T g2, T *v) {
+    T a[8], b[8], c[8], d[8], e[8], w[8], kb[8];
+    op_mhat<T, 4>(u, a);
+    op_mhat<T, 2>(a, b);
+    op_khat<T, 2>(a, c);
+    op_khat<T, 4>(u, d);
+    op_mhat<T, 2>(d, e);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) idx[j] = conn[(size_t)j * ne + e];
+    for (int i = 0; i < 8; ++i) w[i] = fma(g1, c[i] + e[i], g2 * b[i]);
+    op_mhat<T, 1>(w, v);
+    op_khat<T, 1>(b, kb);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = x[idx[j]];
+    for (int i = 0; i < 8; ++i) v[i] = fma(g1, kb[i], v[i]);
+}
+
+// One element in its child frame (vertex 0 = the parent corner it shares, 7 =
+// the parent centre).  Hanging vertices can only be the edge midpoints 1,2,4
+// (mean of vertices 0 and e) and the face centres 3,5,6 (mean of 0, f and
+// f's two edge vertices).  The Q1 operator is invariant under the axis
+// reflections that map any child to child 0, so no table and no child number
+// is needed: r = C^T (g1 K + g2 M) C x with C fixed by the 6-bit mask.
+template <typename T>
+__device__ __forceinline__ void element_child_frame(uint32_t cm, T lap_fac, T mass_fac, const T *x, T *r) {
+    const uint32_t mask = cm & 255u, lev = cm >> 8;
+    const T h = (T)ldexp(1.0, -(int)lev);
+    const T g1 = lap_fac * h * T(1.0 / 36.0), g2 = mass_fac * h * h * h * T(1.0 / 216.0);
     if (mask == 0) {
-        apply_q1<T>(xv, fac1, fac2, v);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) atomic_add(&y[idx[j]], v[j]);
+        apply_q1_hat<T>(x, g1, g2, r);
         return;
     }
-    // u = C x: hanging vertices interpolate from the parent's corners
+    const bool h1 = mask & 2u, h2 = mask & 4u, h4 = mask & 16u, h3 = mask & 8u, h5 = mask & 32u, h6 = mask & 64u;
+    T u[8], v[8];
+    u[0] = x[0];
+    u[7] = x[7];
+    u[1] = h1 ? T(0.5) * (x[0] + x[1]) : x[1];
+    u[2] = h2 ? T(0.5) * (x[0] + x[2]) : x[2];
+    u[4] = h4 ? T(0.5) * (x[0] + x[4]) : x[4];
+    u[3] = h3 ? T(0.25) * ((x[0] + x[3]) + (x[1] + x[2])) : x[3];
+    u[5] = h5 ? T(0.25) * ((x[0] + x[5]) + (x[1] + x[4])) : x[5];
+    u[6] = h6 ? T(0.25) * ((x[0] + x[6]) + (x[2] + x[4])) : x[6];
+    apply_q1_hat<T>(u, g1, g2, v);
+    const T q3 = h3 ? T(0.25) * v[3] : T(0), q5 = h5 ? T(0.25) * v[5] : T(0), q6 = h6 ? T(0.25) * v[6] : T(0);
+    const T e1 = h1 ? T(0.5) * v[1] : T(0), e2 = h2 ? T(0.5) * v[2] : T(0), e4 = h4 ? T(0.5) * v[4] : T(0);
+    r[0] = v[0] + (e1 + e2 + e4) + (q3 + q5 + q6);
+    r[1] = (h1 ? e1 : v[1]) + q3 + q5;
+    r[2] = (h2 ? e2 : v[2]) + q3 + q6;
+    r[4] = (h4 ? e4 : v[4]) + q5 + q6;
+    r[3] = h3 ? q3 : v[3];
+    r[5] = h5 ? q5 : v[5];
+    r[6] = h6 ? q6 : v[6];
+    r[7] = v[7];
+}
+
+// Pass 1: one workgroup per element block.  Element data are prefetched, the
+// block's nodes are gathered into LDS, element contributions are accumulated
+// there in FP64 (ds_add_f64 for both precisions), and each node sum goes to y
+// (node touched by this block only) or to its partial slot (shared node).
+constexpr int PER = 2;  // elements per thread (element block = PER * B)
+template <typename T>
+__global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ lconn, const uint16_t *__restrict__ cmeta,
+                                                  size_t ne, uint32_t eb, const uint32_t *__restrict__ boff,
+                                                  const uint32_t *__restrict__ bnode, const uint32_t *__restrict__ bdst,
+                                                  uint32_t maxn, const T *__restrict__ x, T *__restrict__ y,
+                                                  T *__restrict__ partial, T lap_fac, T mass_fac) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *ys = reinterpret_cast<double *>(smem);
+    T *xs = reinterpret_cast<T *>(ys + maxn);
+    const uint32_t b = blockIdx.x, n0 = boff[b], nb = boff[b + 1] - n0;
+    const size_t e0 = (size_t)b * eb, e1 = min(ne, e0 + eb);
+    uint16_t li[PER][8], mt[PER];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        if (!((mask >> a) & 1u)) {
-            u[a] = xv[a];
-            continue;
-        }
-        uint32_t dff = a ^ c, lo = dff & (0u - dff);
-        if (__popc(dff) == 1) {
-            u[a] = T(0.5) * (xv[c] + xv[a]);
-        } else {
-            u[a] = T(0.25) * (xv[c] + xv[a] + xv[c ^ lo] + xv[c ^ (dff ^ lo)]);
+    for (int k = 0; k < PER; ++k) {
+        const size_t e = e0 + threadIdx.x + (size_t)k * B;
+        if (e < e1) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) li[k][j] = lconn[(size_t)j * ne + e];
+            mt[k] = cmeta[e];
         }
     }
-    apply_q1<T>(u, fac1, fac2, v);
-    // r = C^T v
+    for (uint32_t l = threadIdx.x; l < nb; l += B) {
+        xs[l] = x[bnode[n0 + l] & 0x7fffffffu];
+        ys[l] = 0.0;
+    }
+    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = T(0);
+    for (int k = 0; k < PER; ++k) {
+        const size_t e = e0 + threadIdx.x + (size_t)k * B;
+        if (e < e1) {
+            T xv[8], r[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-        if (!((mask >> a) & 1u)) {
-            r[a] += v[a];
-            continue;
-        }
-        uint32_t dff = a ^ c, lo = dff & (0u - dff);
-        if (__popc(dff) == 1) {
-            T hv = T(0.5) * v[a];
-            r[c] += hv;
-            r[a] += hv;
-        } else {
-            T qv = T(0.25) * v[a];
-            r[c] += qv;
-            r[a] += qv;
-            r[c ^ lo] += qv;
-            r[c ^ (dff ^ lo)] += qv;
+            for (int j = 0; j < 8; ++j) xv[j] = xs[li[k][j]];
+            element_child_frame<T>(mt[k], lap_fac, mass_fac, xv, r);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) atomicAdd(&ys[li[k][j]], (double)r[j]);
         }
     }
-#pragma unroll
-    for (int j = 0; j < 8; ++j) atomic_add(&y[idx[j]], r[j]);
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < nb; l += B) {
+        const uint32_t g = bnode[n0 + l];
+        if (g & 0x80000000u) partial[bdst[n0 + l]] = (T)ys[l];
+        else y[g] = (T)ys[l];
+    }
+}
+
+// Pass 2: shared nodes sum their partials in block order.
+template <typename T>
+__global__ void __launch_bounds__(B) k_matvec_shared(const uint32_t *__restrict__ soff,
+                                                     const uint32_t *__restrict__ snode, uint32_t ns,
+                                                     const T *__restrict__ partial, T *__restrict__ y) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i >= ns) return;
+    T acc = T(0);
+    for (uint32_t p = soff[i]; p < soff[i + 1]; ++p) acc += partial[p];
+    y[snode[i]] = acc;
+}
+
+template <typename T>
+void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, hipStream_t s) {
+    if (!da.n_elem) return;
+    if (da.elem_block != PER * B) {
+        dgx_set_error("matvec: DA built with element blocks of %u, kernel expects %d", da.elem_block, PER * B);
+        throw DgxError();
+    }
+    const size_t lds = (size_t)da.max_block_nodes * (sizeof(double) + sizeof(T));
+    T *partial = reinterpret_cast<T *>(da.partial.p);
+    k_matvec_blk<T><<<da.n_blocks, B, lds, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.elem_block, da.bnode_off.p,
+                                                da.bnode.p, da.bdst.p, da.max_block_nodes, x, y, partial, lap_fac,
+                                                mass_fac);
+    if (da.n_shared)
+        k_matvec_shared<T><<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared, partial, y);
+    HIP_OK(hipGetLastError());
 }
 
 }  // namespace
 
 void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s) {
-    HIP_OK(hipMemsetAsync(y, 0, (size_t)da.n_nodes * sizeof(double), s));
-    if (!da.n_elem) return;
-    k_matvec<double><<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, x, y, lap_fac, mass_fac);
-    HIP_OK(hipGetLastError());
+    matvec_blocks<double>(da, x, y, lap_fac, mass_fac, s);
 }
 
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s) {
-    HIP_OK(hipMemsetAsync(y, 0, (size_t)da.n_nodes * sizeof(float), s));
-    if (!da.n_elem) return;
-    k_matvec<float><<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, x, y, lap_fac, mass_fac);
-    HIP_OK(hipGetLastError());
+    matvec_blocks<float>(da, x, y, lap_fac, mass_fac, s);
 }
 
 namespace {

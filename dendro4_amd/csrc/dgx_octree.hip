@@ -299,10 +299,16 @@ __global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, co
 // ------------------------------------------------------------ balance ----
 __global__ void k_minmax_level(const Key *k, size_t n, uint32_t *mm) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t l = key_level(k[i]);
-    atomicMin(&mm[0], l);
-    atomicMax(&mm[1], l);
+    uint32_t lo = 31, hi = 0;
+    if (i < n) lo = hi = key_level(k[i]);
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, o, 64));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
+    }
+    if (__lane_id() == 0) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
 }
 
 __global__ void __launch_bounds__(B) k_level_hist(const Key *k, size_t n, uint32_t *hist) {
@@ -316,15 +322,23 @@ __global__ void __launch_bounds__(B) k_level_hist(const Key *k, size_t n, uint32
 }
 
 // Parents (as Morton keys) of the input octants, bucketed by the input level.
+// Each block reserves its share of every level bucket with one atomic per level.
 __global__ void __launch_bounds__(B) k_bucket_parents(const Key *k, size_t n, uint32_t md, SfcPacked tab,
                                                       uint32_t *cursor /*32, pre-offset*/, Key *out) {
     LOAD_TABLES(tab)
+    __shared__ uint32_t cnt[32], base[32];
+    if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+    __syncthreads();
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
-    uint32_t x, y, z, l;
-    dev_decode(s_mc, k[i], md, x, y, z, l);
-    uint32_t pos = atomicAdd(&cursor[l], 1u);
-    out[pos] = key_parent(morton_encode(x, y, z, l, md));
+    uint32_t x = 0, y = 0, z = 0, l = 0, loc = 0;
+    if (i < n) {
+        dev_decode(s_mc, k[i], md, x, y, z, l);
+        loc = atomicAdd(&cnt[l], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]) : 0u;
+    __syncthreads();
+    if (i < n) out[base[l] + loc] = key_parent(morton_encode(x, y, z, l, md));
 }
 
 __global__ void k_unique_flag(const Key *k, size_t n, uint32_t *flag) {
@@ -342,29 +356,28 @@ __global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32
 // parents take at most two values, so at most 8 distinct parents per octant.
 __global__ void k_bal_nbr_parents(const Key *Q, size_t nq, uint32_t lvl, uint32_t md, Key *out, uint32_t *counter) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    uint32_t x, y, z, l;
-    morton_decode(Q[i], md, x, y, z, l);
-    const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
-    int64_t cx[2], cy[2], cz[2];
-    int64_t v[3] = {x, y, z};
-    int64_t *c[3] = {cx, cy, cz};
-    for (int a = 0; a < 3; ++a) {
-        int64_t own = (v[a] / plen) * plen;
-        int64_t other = (v[a] % plen) ? own + plen : own - plen;
-        c[a][0] = own;
-        c[a][1] = other;
-    }
     Key loc[8];
     int nl = 0;
-    for (int bz = 0; bz < 2; ++bz)
-        for (int by = 0; by < 2; ++by)
-            for (int bx = 0; bx < 2; ++bx) {
-                int64_t px = cx[bx], py = cy[by], pz = cz[bz];
-                if (px < 0 || py < 0 || pz < 0 || px >= dom || py >= dom || pz >= dom) continue;
-                loc[nl++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
-            }
-    uint32_t base = atomicAdd(counter, (uint32_t)nl);
+    if (i < nq) {
+        uint32_t x, y, z, l;
+        morton_decode(Q[i], md, x, y, z, l);
+        const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
+        int64_t c[3][2];
+        const int64_t v[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+            int64_t own = (v[a] / plen) * plen;
+            c[a][0] = own;
+            c[a][1] = (v[a] % plen) ? own + plen : own - plen;
+        }
+        for (int bz = 0; bz < 2; ++bz)
+            for (int by = 0; by < 2; ++by)
+                for (int bx = 0; bx < 2; ++bx) {
+                    int64_t px = c[0][bx], py = c[1][by], pz = c[2][bz];
+                    if (px < 0 || py < 0 || pz < 0 || px >= dom || py >= dom || pz >= dom) continue;
+                    loc[nl++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
+                }
+    }
+    uint32_t base = wave_reserve(counter, (uint32_t)nl);  // every lane of the wave reaches this call
     for (int j = 0; j < nl; ++j) out[base + j] = loc[j];
 }
 

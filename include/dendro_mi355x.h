@@ -78,6 +78,13 @@ typedef struct {
     uint64_t n_nodes;            /* getNodeSize() = node_size + boundary_node_size */
     uint32_t max_depth;          /* getMaxDepth(): octree depth + 1 */
     uint32_t ghost_scans;        /* node-list entries resolved by the forward ghost scan */
+    /* matvec block view (no reference counterpart): element blocks, their
+     * (block,node) pairs, nodes touched by more than one block, largest block */
+    uint64_t matvec_blocks;
+    uint64_t block_node_pairs;
+    uint64_t shared_nodes;
+    uint64_t shared_pairs;
+    uint64_t max_block_nodes;
 } dgx_da_info;
 
 /* ot::DA::DA(in, MPI_COMM_SELF, MPI_COMM_SELF, tol, compressLut=false)
