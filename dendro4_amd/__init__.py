@@ -32,7 +32,8 @@ EXPORTED = [
     "dgx_memcpy_htod", "dgx_memcpy_dtoh", "dgx_memset_device", "dgx_points_to_octree", "dgx_balance_octree",
     "dgx_octree_import", "dgx_octree_size", "dgx_octree_max_depth", "dgx_octree_export", "dgx_octree_stats",
     "dgx_octree_free", "dgx_da_create", "dgx_da_get_info", "dgx_da_export", "dgx_da_matvec", "dgx_da_matvec_f32",
-    "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points",
+    "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points", "dgx_da_matvec_dirichlet", "dgx_mg_create", "dgx_mg_info",
+    "dgx_mg_finest_da", "dgx_mg_rhs", "dgx_mg_solve", "dgx_mg_free",
 ]
 
 
@@ -55,6 +56,15 @@ class _DaInfo(C.Structure):
         ("shared_pairs", C.c_uint64),
         ("max_block_nodes", C.c_uint64),
     ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("fp32_vcycle", C.c_int), ("pre", C.c_int), ("post", C.c_int), ("omega", C.c_double),
+                ("rtol", C.c_double), ("maxit", C.c_int), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double), ("seconds", C.c_double)]
 
 
 _lib = None
@@ -97,6 +107,13 @@ def load_library():
         "dgx_da_free": ([vp], i32),
         "dgx_gen_points": ([i32, u64, sz, i32, d, d, d, d, i32, vp], i32),
         "dgx_sfc_tables": ([i32, vp, vp, vp], i32),
+        "dgx_da_matvec_dirichlet": ([vp, vp, vp, d, d], i32),
+        "dgx_mg_create": ([vp, i32, d, d, C.POINTER(vp)], i32),
+        "dgx_mg_info": ([vp, C.POINTER(i32), vp, vp], i32),
+        "dgx_mg_finest_da": ([vp, C.POINTER(vp)], i32),
+        "dgx_mg_rhs": ([vp, vp, vp], i32),
+        "dgx_mg_solve": ([vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)], i32),
+        "dgx_mg_free": ([vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -288,6 +305,16 @@ class DA:
         else:
             _check(_lib.dgx_da_matvec(self._h, x.ptr, y.ptr, lap_fac, mass_fac))
 
+    def matvec_dirichlet(self, x, y, lap_fac=1.0, mass_fac=1.0):
+        """DirichletJacobianMatMult: identity rows / dropped columns on boundary nodes."""
+        _check(_lib.dgx_da_matvec_dirichlet(self._h, x.ptr, y.ptr, lap_fac, mass_fac))
+
+    def matvec_dirichlet_host(self, x, lap_fac=1.0, mass_fac=1.0):
+        dx = DeviceArray.from_host(np.asarray(x, np.float64))
+        dy = DeviceArray((self.n_nodes,), np.float64)
+        self.matvec_dirichlet(dx, dy, lap_fac, mass_fac)
+        return dy.to_host()
+
     def matvec_host(self, x, lap_fac=1.0, mass_fac=1.0):
         dt = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
         dx = DeviceArray.from_host(np.asarray(x, dt))
@@ -301,8 +328,63 @@ class DA:
         return dd.to_host()
 
     def free(self):
+        if getattr(self, "_owner", None) is not None:
+            self._h = C.c_void_p()
+            return
         if self._h and self._h.value:
             _check(_lib.dgx_da_free(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @classmethod
+    def _borrow(cls, handle, owner):
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owner = owner
+        info = _DaInfo()
+        _check(_lib.dgx_da_get_info(self._h, C.byref(info)))
+        for k in ("local_buffer_size", "element_size", "node_size", "boundary_node_size", "n_nodes", "max_depth",
+                  "ghost_scans"):
+            setattr(self, k, getattr(info, k))
+        self.block_info = {}
+        return self
+
+
+class MG:
+    """DAMG on one rank: octree hierarchy, per-level DA, V-cycle-preconditioned CG."""
+
+    def __init__(self, octree, nlevels=5, lap_fac=1.0, mass_fac=1.0):
+        self._h = C.c_void_p()
+        _check(_lib_ready().dgx_mg_create(octree._h, nlevels, lap_fac, mass_fac, C.byref(self._h)))
+        n = C.c_int()
+        octs = (C.c_uint64 * 32)()
+        nodes = (C.c_uint64 * 32)()
+        _check(_lib.dgx_mg_info(self._h, C.byref(n), octs, nodes))
+        self.nlevels = n.value
+        self.level_octants = list(octs)[: n.value]
+        self.level_nodes = list(nodes)[: n.value]
+        h = C.c_void_p()
+        _check(_lib.dgx_mg_finest_da(self._h, C.byref(h)))
+        self.da = DA._borrow(h, self)
+        self.n_nodes = self.level_nodes[0]
+
+    def rhs(self, f, b):
+        _check(_lib.dgx_mg_rhs(self._h, f.ptr, b.ptr))
+
+    def solve(self, b, u, fp32=True, pre=2, post=2, omega=0.8, rtol=1e-8, maxit=200, graph=True):
+        o = SolveOpts(1 if fp32 else 0, pre, post, omega, rtol, maxit, 1 if graph else 0)
+        st = SolveStats()
+        _check(_lib.dgx_mg_solve(self._h, b.ptr, u.ptr, C.byref(o), C.byref(st)))
+        return {"iterations": st.iterations, "rel_residual": st.rel_residual, "seconds": st.seconds}
+
+    def free(self):
+        if self._h and self._h.value:
+            _check(_lib.dgx_mg_free(self._h))
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -10,7 +10,7 @@
 #include <string>
 
 #include "../../include/dendro_mi355x.h"
-#include "dgx_da.h"
+#include "dgx_mg.h"
 
 // ------------------------------------------------------------------ state --
 namespace {
@@ -76,6 +76,12 @@ struct dgx_octree_s {
 };
 struct dgx_da_s {
     std::unique_ptr<DaDevice> da;
+    DaDevice *borrowed = nullptr;  // a DA owned by a multigrid hierarchy
+    const DaDevice &get() const { return borrowed ? *borrowed : *da; }
+};
+struct dgx_mg_s {
+    std::unique_ptr<MgHierarchy> mg;
+    dgx_da_s finest;
 };
 
 extern "C" {
@@ -276,7 +282,7 @@ int dgx_da_create(dgx_octree oct, dgx_da *out) {
 
 int dgx_da_get_info(dgx_da d, dgx_da_info *info) {
     if (!d || !info) { dgx_set_error("null argument"); return -1; }
-    const DaDevice &a = *d->da;
+    const DaDevice &a = d->get();
     info->local_buffer_size = a.n_local;
     info->element_size = a.n_elem;
     info->node_size = a.node_size;
@@ -296,7 +302,7 @@ int dgx_da_export(dgx_da d, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels
     API_BEGIN
     ensure_ctx();
     if (!d) { dgx_set_error("null argument"); return -1; }
-    const DaDevice &a = *d->da;
+    const DaDevice &a = d->get();
     if (nlist) HIP_OK(hipMemcpyAsync(nlist, a.nlist.p, 8 * a.n_elem * 4, hipMemcpyDeviceToHost, S()));
     if (masks) HIP_OK(hipMemcpyAsync(masks, a.mask.p, a.n_elem, hipMemcpyDeviceToHost, S()));
     if (oct_levels) HIP_OK(hipMemcpyAsync(oct_levels, a.flags.p, a.n_local, hipMemcpyDeviceToHost, S()));
@@ -309,25 +315,103 @@ int dgx_da_matvec(dgx_da d, const double *x, double *y, double lap_fac, double m
     API_BEGIN
     ensure_ctx();
     if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
-    da_matvec_f64(*d->da, x, y, lap_fac, mass_fac, S());
+    da_matvec_f64(d->get(), x, y, lap_fac, mass_fac, S());
     API_END
 }
 int dgx_da_matvec_f32(dgx_da d, const float *x, float *y, float lap_fac, float mass_fac) {
     API_BEGIN
     ensure_ctx();
     if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
-    da_matvec_f32(*d->da, x, y, lap_fac, mass_fac, S());
+    da_matvec_f32(d->get(), x, y, lap_fac, mass_fac, S());
     API_END
 }
 int dgx_da_diagonal(dgx_da d, double *diag, double lap_fac, double mass_fac) {
     API_BEGIN
     ensure_ctx();
     if (!d || !diag) { dgx_set_error("diagonal: null argument"); return -1; }
-    da_diagonal_f64(*d->da, diag, lap_fac, mass_fac, S());
+    da_diagonal_f64(d->get(), diag, lap_fac, mass_fac, S());
     API_END
 }
+int dgx_da_matvec_dirichlet(dgx_da d, const double *x, double *y, double lap_fac, double mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
+    da_matvec_dirichlet_f64(d->get(), x, y, lap_fac, mass_fac, S());
+    API_END
+}
+
+int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fac, dgx_mg *out) {
+    API_BEGIN
+    ensure_ctx();
+    if (!finest || !out) { dgx_set_error("DAMG: null argument"); return -1; }
+    if (nlevels < 1 || nlevels > 30) { dgx_set_error("DAMG: nlevels %d out of range", nlevels); return -1; }
+    std::unique_ptr<dgx_mg_s> m(new dgx_mg_s());
+    m->mg = mg_build(finest->keys.p, finest->n, finest->md, finest->curve, nlevels, lap_fac, mass_fac, S());
+    m->finest.borrowed = m->mg->lev[0].da.get();
+    *out = m.release();
+    API_END
+}
+int dgx_mg_info(dgx_mg m, int *nlevels, uint64_t *octants, uint64_t *nodes) {
+    if (!m) { dgx_set_error("null argument"); return -1; }
+    const auto &lev = m->mg->lev;
+    if (nlevels) *nlevels = (int)lev.size();
+    for (size_t i = 0; i < lev.size() && i < 32; ++i) {
+        if (octants) octants[i] = lev[i].n_oct;
+        if (nodes) nodes[i] = lev[i].da->n_nodes;
+    }
+    return 0;
+}
+int dgx_mg_finest_da(dgx_mg m, dgx_da *da) {
+    if (!m || !da) { dgx_set_error("null argument"); return -1; }
+    *da = &m->finest;
+    return 0;
+}
+int dgx_mg_rhs(dgx_mg m, const double *f, double *b) {
+    API_BEGIN
+    ensure_ctx();
+    if (!m || !f || !b) { dgx_set_error("rhs: null argument"); return -1; }
+    mg_rhs_from_nodal(*m->mg, f, b, S());
+    API_END
+}
+int dgx_mg_solve(dgx_mg m, const double *b, double *u, const dgx_solve_opts *opts, dgx_solve_stats *stats) {
+    API_BEGIN
+    ensure_ctx();
+    if (!m || !b || !u) { dgx_set_error("solve: null argument"); return -1; }
+    SolveOpts o;
+    if (opts) {
+        o.fp32_vcycle = opts->fp32_vcycle;
+        o.pre = opts->pre;
+        o.post = opts->post;
+        o.omega = opts->omega;
+        o.rtol = opts->rtol;
+        o.maxit = opts->maxit;
+        o.use_graph = opts->use_graph;
+    }
+    if (o.pre < 1 || o.post < 0 || o.maxit < 1) { dgx_set_error("solve: bad options"); return -1; }
+    SolveStats st;
+    mg_solve(*m->mg, b, u, o, &st, S());
+    if (stats) {
+        stats->iterations = st.iterations;
+        stats->rel_residual = st.rel_residual;
+        stats->seconds = st.seconds;
+    }
+    API_END
+}
+int dgx_mg_free(dgx_mg m) {
+    API_BEGIN
+    if (m) {
+        if (g_ctx.ready) HIP_OK(hipStreamSynchronize(S()));
+        delete m;
+    }
+    API_END
+}
+
 int dgx_da_free(dgx_da d) {
     API_BEGIN
+    if (d && d->borrowed) {
+        dgx_set_error("this DA belongs to a multigrid hierarchy (free it with dgx_mg_free)");
+        return -1;
+    }
     if (d) {
         if (g_ctx.ready) HIP_OK(hipStreamSynchronize(S()));
         delete d;

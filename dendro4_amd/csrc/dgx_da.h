@@ -32,8 +32,11 @@ struct DaDevice {
     DevBuf<uint32_t> bnode_off, bnode, bdst, soff, snode;
     DevBuf<uint16_t> lconn, cmeta;
     DevBuf<double> partial;  // partial sums of shared nodes (float or double view), one per matvec in flight
-    // Dirichlet: 1 for nodes on the domain boundary (filled on demand)
+    // Per node: local-buffer index of its octant, and 1 if the node lies on the
+    // domain boundary (homogeneous Dirichlet rows, omgJac.C:958-1057).
+    DevBuf<uint32_t> n2l;
     DevBuf<uint8_t> bdy_node;
+    uint32_t n_bdy_nodes = 0;
     ~DaDevice();
 };
 
@@ -47,3 +50,11 @@ void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fa
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s);
 // Diagonal of the same operator (Jacobian1MatGetDiagonal, odaJac.C:160-227).
 void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double mass_fac, hipStream_t s);
+// Homogeneous-Dirichlet operator (DirichletJacobianMatMult, omgJac.C:958-1057):
+// boundary rows are the identity and boundary columns are dropped.
+void da_matvec_dirichlet_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac,
+                             hipStream_t s);
+void da_matvec_dirichlet_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac,
+                             hipStream_t s);
+// Dense n_nodes x n_nodes matrix of the Dirichlet operator (row-major, FP64).
+void da_assemble_dense_dirichlet(const DaDevice &da, double lap_fac, double mass_fac, double *A, hipStream_t s);

@@ -248,10 +248,24 @@ __global__ void k_da_apply_nodes(const uint8_t *isnode, size_t n, size_t nelem, 
     nodeflag[i] = (flags[i] & F_NODE) ? 1u : 0u;
 }
 
-__global__ void k_da_l2n(const uint32_t *nodeflag, const uint32_t *pos, size_t n, uint32_t *l2n) {
+__global__ void k_da_l2n(const uint32_t *nodeflag, const uint32_t *pos, size_t n, uint32_t *l2n, uint32_t *n2l) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     l2n[i] = nodeflag[i] ? pos[i] : 0xffffffffu;
+    if (nodeflag[i]) n2l[pos[i]] = (uint32_t)i;
+}
+// Nodes on the boundary of the unit cube (anchor coordinate 0 or 2^(md-1)).
+__global__ void k_da_bdy_nodes(const uint32_t *n2l, uint32_t nn, const uint32_t *xs, const uint32_t *ys,
+                               const uint32_t *zs, uint32_t md, uint8_t *bdy, uint32_t *count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool on = false;
+    if (i < nn) {
+        const uint32_t D = 1u << (md - 1), l = n2l[i], x = xs[l], y = ys[l], z = zs[l];
+        on = x == 0 || y == 0 || z == 0 || x == D || y == D || z == D;
+        bdy[i] = on ? 1 : 0;
+    }
+    uint32_t c = __popcll(__ballot(on));
+    if (__lane_id() == 0 && c) atomicAdd(count, c);
 }
 
 // Element view for the matvec: SoA compact node ids and packed meta
@@ -321,7 +335,14 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     k_da_apply_nodes<<<grid_for(N, B), B, 0, s>>>(isnode.p, N, n, da->flags.p, nodeflag.p, ctr.p + 2);
     uint64_t nn = scan_exclusive_u32(nodeflag.p, pos.p, N, s);
     da->l2n.alloc(N, s);
-    k_da_l2n<<<grid_for(N, B), B, 0, s>>>(nodeflag.p, pos.p, N, da->l2n.p);
+    da->n2l.alloc(nn, s);
+    k_da_l2n<<<grid_for(N, B), B, 0, s>>>(nodeflag.p, pos.p, N, da->l2n.p, da->n2l.p);
+    da->bdy_node.alloc(nn, s);
+    DevBuf<uint32_t> bcount(1, s);
+    HIP_OK(hipMemsetAsync(bcount.p, 0, 4, s));
+    k_da_bdy_nodes<<<grid_for(nn, B), B, 0, s>>>(da->n2l.p, (uint32_t)nn, da->x.p, da->y.p, da->z.p, md,
+                                                da->bdy_node.p, bcount.p);
+    HIP_OK(hipMemcpyAsync(&da->n_bdy_nodes, bcount.p, 4, hipMemcpyDeviceToHost, s));
     uint32_t hc[4];
     HIP_OK(hipMemcpyAsync(hc, ctr.p, 16, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -435,6 +456,13 @@ __global__ void k_blk_shared_nodes(const Key *sk, const uint32_t *shead, const u
     snode[spos[q]] = (uint32_t)sk[q].hi;
     soff[spos[q]] = (uint32_t)q;
 }
+// Bit 30 of a block node / shared node entry marks a Dirichlet boundary node.
+__global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t v = ids[p];
+    if (bdy[v & 0x3fffffffu]) ids[p] = v | 0x40000000u;
+}
 __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t v = b < nblk ? boff[b + 1] - boff[b] : 0u;
@@ -488,6 +516,8 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
         HIP_OK(hipMemcpyAsync(da.soff.p + nsn, &tail, 4, hipMemcpyHostToDevice, s));
         da.n_shared = (uint32_t)nsn;
     }
+    k_blk_mark_bdy<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, np, da.bdy_node.p);
+    if (da.n_shared) k_blk_mark_bdy<<<grid_for(da.n_shared, B), B, 0, s>>>(da.snode.p, da.n_shared, da.bdy_node.p);
     DevBuf<uint32_t> mx(1, s);
     HIP_OK(hipMemsetAsync(mx.p, 0, 4, s));
     k_blk_maxnodes<<<grid_for(nblk, B), B, 0, s>>>(da.bnode_off.p, nblk, mx.p);

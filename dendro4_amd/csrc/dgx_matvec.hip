@@ -150,8 +150,11 @@ __device__ __forceinline__ void element_child_frame(uint32_t cm, T lap_fac, T ma
 // block's nodes are gathered into LDS, element contributions are accumulated
 // there in FP64 (ds_add_f64 for both precisions), and each node sum goes to y
 // (node touched by this block only) or to its partial slot (shared node).
+// Node entries: bits 0-29 node id, bit 30 Dirichlet boundary, bit 31 shared.
+// With DIR, boundary columns are dropped and boundary rows are the identity.
 constexpr int PER = 2;  // elements per thread (element block = PER * B)
-template <typename T>
+constexpr uint32_t ID_MASK = 0x3fffffffu, BDY_BIT = 0x40000000u, SHARED_BIT = 0x80000000u;
+template <typename T, bool DIR>
 __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ lconn, const uint16_t *__restrict__ cmeta,
                                                   size_t ne, uint32_t eb, const uint32_t *__restrict__ boff,
                                                   const uint32_t *__restrict__ bnode, const uint32_t *__restrict__ bdst,
@@ -173,7 +176,8 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
         }
     }
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
-        xs[l] = x[bnode[n0 + l] & 0x7fffffffu];
+        const uint32_t g = bnode[n0 + l];
+        xs[l] = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
         ys[l] = 0.0;
     }
     __syncthreads();
@@ -192,24 +196,30 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
-        if (g & 0x80000000u) partial[bdst[n0 + l]] = (T)ys[l];
-        else y[g] = (T)ys[l];
+        if (g & SHARED_BIT) partial[bdst[n0 + l]] = (T)ys[l];
+        else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : (T)ys[l];
     }
 }
 
 // Pass 2: shared nodes sum their partials in block order.
-template <typename T>
+template <typename T, bool DIR>
 __global__ void __launch_bounds__(B) k_matvec_shared(const uint32_t *__restrict__ soff,
                                                      const uint32_t *__restrict__ snode, uint32_t ns,
-                                                     const T *__restrict__ partial, T *__restrict__ y) {
+                                                     const T *__restrict__ partial, const T *__restrict__ x,
+                                                     T *__restrict__ y) {
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i >= ns) return;
+    const uint32_t g = snode[i];
+    if (DIR && (g & BDY_BIT)) {
+        y[g & ID_MASK] = x[g & ID_MASK];
+        return;
+    }
     T acc = T(0);
     for (uint32_t p = soff[i]; p < soff[i + 1]; ++p) acc += partial[p];
-    y[snode[i]] = acc;
+    y[g & ID_MASK] = acc;
 }
 
-template <typename T>
+template <typename T, bool DIR>
 void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, hipStream_t s) {
     if (!da.n_elem) return;
     if (da.elem_block != PER * B) {
@@ -218,22 +228,30 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
     }
     const size_t lds = (size_t)da.max_block_nodes * (sizeof(double) + sizeof(T));
     T *partial = reinterpret_cast<T *>(da.partial.p);
-    k_matvec_blk<T><<<da.n_blocks, B, lds, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.elem_block, da.bnode_off.p,
-                                                da.bnode.p, da.bdst.p, da.max_block_nodes, x, y, partial, lap_fac,
-                                                mass_fac);
+    k_matvec_blk<T, DIR><<<da.n_blocks, B, lds, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.elem_block,
+                                                     da.bnode_off.p, da.bnode.p, da.bdst.p, da.max_block_nodes, x,
+                                                     y, partial, lap_fac, mass_fac);
     if (da.n_shared)
-        k_matvec_shared<T><<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared, partial, y);
+        k_matvec_shared<T, DIR>
+            <<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared, partial, x, y);
     HIP_OK(hipGetLastError());
 }
 
 }  // namespace
 
 void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s) {
-    matvec_blocks<double>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<double, false>(da, x, y, lap_fac, mass_fac, s);
 }
-
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s) {
-    matvec_blocks<float>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<float, false>(da, x, y, lap_fac, mass_fac, s);
+}
+void da_matvec_dirichlet_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac,
+                             hipStream_t s) {
+    matvec_blocks<double, true>(da, x, y, lap_fac, mass_fac, s);
+}
+void da_matvec_dirichlet_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac,
+                             hipStream_t s) {
+    matvec_blocks<float, true>(da, x, y, lap_fac, mass_fac, s);
 }
 
 namespace {
@@ -279,5 +297,52 @@ void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double ma
     HIP_OK(hipMemsetAsync(diag, 0, (size_t)da.n_nodes * sizeof(double), s));
     if (!da.n_elem) return;
     k_diagonal<<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, diag, lap_fac, mass_fac);
+    HIP_OK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Dense assembly of the Dirichlet operator (coarsest multigrid level).
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void __launch_bounds__(B) k_assemble_dense(const uint32_t *__restrict__ conn,
+                                                      const uint16_t *__restrict__ meta, size_t ne,
+                                                      const uint8_t *__restrict__ bdy, double lap_fac,
+                                                      double mass_fac, double *A, uint32_t n) {
+    size_t e = (size_t)blockIdx.x * B + threadIdx.x;
+    if (e >= ne) return;
+    const uint32_t mt = meta[e], c = (mt >> 8) & 7u, mask = mt & 255u;
+    uint32_t cm = 0;  // hanging mask in the child frame
+    for (uint32_t j = 0; j < 8; ++j)
+        if ((mask >> (j ^ c)) & 1u) cm |= 1u << j;
+    cm |= (mt >> 11) << 8;
+    uint32_t id[8];
+    for (int j = 0; j < 8; ++j) id[j] = conn[(size_t)j * ne + e];
+    for (int k = 0; k < 8; ++k) {  // column k (child frame) of the element matrix
+        double xv[8], r[8];
+        for (int j = 0; j < 8; ++j) xv[j] = (j == k) ? 1.0 : 0.0;
+        element_child_frame<double>(cm, lap_fac, mass_fac, xv, r);
+        const uint32_t col = id[k ^ c];
+        if (bdy[col]) continue;
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t row = id[j ^ c];
+            if (bdy[row]) continue;
+            atomicAdd(&A[(size_t)row * n + col], r[j]);
+        }
+    }
+}
+__global__ void k_dense_bdy_identity(const uint8_t *bdy, uint32_t n, double *A) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n && bdy[i]) A[(size_t)i * n + i] = 1.0;
+}
+
+}  // namespace
+
+void da_assemble_dense_dirichlet(const DaDevice &da, double lap_fac, double mass_fac, double *A, hipStream_t s) {
+    const uint32_t n = da.n_nodes;
+    HIP_OK(hipMemsetAsync(A, 0, (size_t)n * n * sizeof(double), s));
+    k_assemble_dense<<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, da.bdy_node.p, lap_fac,
+                                                         mass_fac, A, n);
+    k_dense_bdy_identity<<<grid_for(n, B), B, 0, s>>>(da.bdy_node.p, n, A);
     HIP_OK(hipGetLastError());
 }

@@ -1,0 +1,68 @@
+// dgx_mg.h -- geometric multigrid hierarchy and the CG/V-cycle solve (one rank).
+#pragma once
+#include <vector>
+
+#include "dgx_da.h"
+
+// ot::coarsenOctree (src/coarsen.cpp:53-104): every complete set of 8 sibling
+// leaves is replaced by its parent, once.
+DevBuf<Key> coarsen_build(const Key *in, size_t n, hipStream_t s);
+
+// Interpolation from a coarse level to the next finer one, per fine node:
+// up to 8 (coarse node, weight) pairs (ELL, col = 0xffffffff when unused),
+// and the transpose as CSR over coarse nodes for the restriction.
+struct Transfer {
+    size_t nf = 0, nc = 0, nnz = 0;
+    DevBuf<uint32_t> pcol;    // [8][nf]
+    DevBuf<double> pval;      // [8][nf]
+    DevBuf<float> pval32;     // [8][nf]
+    DevBuf<uint32_t> roff;    // [nc+1]
+    DevBuf<uint32_t> rcol;    // [nnz] fine node ids, ascending per coarse row
+    DevBuf<double> rval;      // [nnz]
+    DevBuf<float> rval32;     // [nnz]
+};
+
+struct MgLevel {
+    DevBuf<Key> keys;  // balanced octree of this level (md of the finest)
+    size_t n_oct = 0;
+    std::unique_ptr<DaDevice> da;
+    DevBuf<double> dinv;   // 1 / diag of the Dirichlet operator
+    DevBuf<float> dinv32;
+    Transfer to_finer;     // interpolation from this level to the finer one (unused on level 0)
+};
+
+struct MgWork;  // V-cycle and CG work vectors (dgx_mg.hip)
+struct MgHierarchy {
+    uint32_t md = 30;
+    int curve = 0;
+    std::vector<MgLevel> lev;  // lev[0] = finest
+    double lap_fac = 1.0, mass_fac = 1.0;
+    // coarsest-level direct solver: dense inverse of the Dirichlet operator (FP64)
+    uint32_t nc = 0;
+    DevBuf<double> cinv;
+    MgWork *work = nullptr;
+    ~MgHierarchy();
+};
+
+std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
+                                      double lap_fac, double mass_fac, hipStream_t s);
+
+struct SolveOpts {
+    int fp32_vcycle = 1;
+    int pre = 2, post = 2;
+    double omega = 0.8;
+    double rtol = 1e-8;
+    int maxit = 200;
+    int use_graph = 1;  // replay hipGraph captures of the CG step and the V-cycle
+};
+struct SolveStats {
+    int iterations = 0;
+    double rel_residual = 0.0;      // ||b - A x|| / ||b|| recomputed in FP64 at the end
+    double seconds = 0.0;
+};
+// Solve A_D u = b on the finest level (device vectors of n_nodes).
+void mg_solve(MgHierarchy &mg, const double *b, double *u, const SolveOpts &o, SolveStats *st, hipStream_t s);
+// Apply one preconditioner (V-cycle) z = M r, FP64 or FP32 inside.
+void mg_vcycle_apply(MgHierarchy &mg, const double *r, double *z, const SolveOpts &o, hipStream_t s);
+// FE load vector b = D M f (mass-weighted nodal f, zero on boundary nodes).
+void mg_rhs_from_nodal(MgHierarchy &mg, const double *f, double *b, hipStream_t s);

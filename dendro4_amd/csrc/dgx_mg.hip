@@ -1,0 +1,698 @@
+// dgx_mg.hip -- geometric multigrid on the octree hierarchy and the FP64 CG
+// driver (one rank, gfx950).
+//
+// Reference structure: DAMGCreateAndSetDA (src/omg/blockDiag.cpp:1312-2190)
+// coarsens the finest octree level by level (coarsenOctree, coarsen.cpp:53-104)
+// and balances each coarse octree, stopping early below 500 octants
+// (:1384-1425); each level gets its own DA.  DAMGSetKSP/DAMGSolve (:780-1170)
+// run PETSc CG preconditioned by a multiplicative V-cycle with Richardson +
+// Jacobi smoothing and an LU coarse solve (bin/optionsFBM:26-57).  PETSc is not
+// part of this build; the same method is implemented here directly:
+//   * operator: homogeneous Dirichlet -lap+I (DirichletJacobianMatMult,
+//     omgJac.C:958-1057): boundary rows identity, boundary columns dropped;
+//   * transfer: P interpolates the coarse finite-element function at the fine
+//     nodes (trilinear, with the coarse element's hanging-node constraints),
+//     R = P^T; Dirichlet nodes are excluded from both;
+//   * smoother: damped Jacobi x += omega D^-1 (b - A x), nu1 pre / nu2 post;
+//   * coarsest level: dense inverse of the Dirichlet operator in FP64;
+//   * outer: flexible preconditioned CG in FP64 (Polak-Ribiere beta), so the
+//     V-cycle may run in FP32 while residuals, dot products, updates and the
+//     coarse solve stay FP64.
+#include "dgx_mg.h"
+
+#include <chrono>
+#include <cmath>
+
+namespace {
+
+constexpr int B = 256;
+constexpr uint32_t NONE = 0xffffffffu;
+
+// ------------------------------------------------------------- coarsen ----
+__global__ void k_coarsen_flags(const Key *k, size_t n, uint32_t *keep, uint32_t *start) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t L = key_level(k[i]);
+    uint32_t st = 0;
+    if (L >= 1 && key_digit(k[i], L) == 0 && i + 7 < n && key_level(k[i + 7]) == L &&
+        key_eq(key_parent(k[i + 7]), key_parent(k[i])))
+        st = 1;
+    start[i] = st;
+    keep[i] = 1;
+}
+__global__ void k_coarsen_mark(const uint32_t *start, size_t n, uint32_t *keep) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !start[i]) return;
+    for (int j = 1; j < 8; ++j) keep[i + j] = 0;
+}
+__global__ void k_coarsen_emit(const Key *k, const uint32_t *keep, const uint32_t *start, const uint32_t *pos, size_t n,
+                               Key *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    out[pos[i]] = start[i] ? key_parent(k[i]) : k[i];
+}
+
+// ------------------------------------------------------------ transfer ----
+__device__ inline bool mlb(const Key *a, size_t n, const Key &k, size_t &idx) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = lo + (hi - lo) / 2;
+        if (key_le(a[mid], k)) lo = mid + 1; else hi = mid;
+    }
+    if (lo == 0) return false;
+    idx = lo - 1;
+    return true;
+}
+
+// One row of P per fine node: interpolate the coarse FE function at the node.
+__global__ void __launch_bounds__(B) k_prolong_rows(uint32_t nf, const uint32_t *n2l_f, const uint32_t *xf,
+                                                    const uint32_t *yf, const uint32_t *zf, const uint8_t *bdy_f,
+                                                    const Key *ckey, size_t nce, const uint32_t *xc,
+                                                    const uint32_t *yc, const uint32_t *zc, const uint8_t *cflags,
+                                                    const uint32_t *cconn, const uint16_t *cmeta,
+                                                    const uint8_t *bdy_c, uint32_t md, SfcPacked tab,
+                                                    uint32_t *pcol, double *pval, uint32_t *err) {
+    __shared__ uint8_t s_rc[192];
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) s_rc[i] = tab.rc[i];
+    __syncthreads();
+    const uint32_t n = blockIdx.x * B + threadIdx.x;
+    if (n >= nf) return;
+    for (int k = 0; k < 8; ++k) {
+        pcol[(size_t)k * nf + n] = NONE;
+        pval[(size_t)k * nf + n] = 0.0;
+    }
+    if (bdy_f[n]) return;
+    const uint32_t l = n2l_f[n], D = 1u << (md - 1);
+    const uint32_t X = xf[l], Y = yf[l], Z = zf[l];
+    const uint32_t qx = X == D ? D - 1 : X, qy = Y == D ? D - 1 : Y, qz = Z == D ? D - 1 : Z;
+    size_t e;
+    if (!mlb(ckey, nce, dev_encode(s_rc, qx, qy, qz, md, md), e)) {
+        atomicOr(err, 1u);
+        return;
+    }
+    const uint32_t lc = cflags[e] & 31u, sc = 1u << (md - lc);
+    const uint32_t ax = xc[e], ay = yc[e], az = zc[e];
+    if (X < ax || Y < ay || Z < az || X > ax + sc || Y > ay + sc || Z > az + sc) {
+        atomicOr(err, 2u);
+        return;
+    }
+    // local coordinates in halves: 0, 1 (midpoint) or 2
+    const uint64_t dx2 = 2ull * (X - ax), dy2 = 2ull * (Y - ay), dz2 = 2ull * (Z - az);
+    if ((dx2 % sc) || (dy2 % sc) || (dz2 % sc)) {
+        atomicOr(err, 4u);
+        return;
+    }
+    const uint32_t tx = (uint32_t)(dx2 / sc), ty = (uint32_t)(dy2 / sc), tz = (uint32_t)(dz2 / sc);
+    const uint32_t mt = cmeta[e], mask = mt & 255u, c = (mt >> 8) & 7u;
+    double w[8];
+    for (int k = 0; k < 8; ++k) w[k] = 0.0;
+    for (uint32_t a = 0; a < 8; ++a) {  // trilinear weight of coarse vertex a
+        const uint32_t bx = a & 1, by = (a >> 1) & 1, bz = (a >> 2) & 1;
+        const double wx = bx ? tx * 0.5 : 1.0 - tx * 0.5, wy = by ? ty * 0.5 : 1.0 - ty * 0.5,
+                     wz = bz ? tz * 0.5 : 1.0 - tz * 0.5;
+        const double wa = wx * wy * wz;
+        if (wa == 0.0) continue;
+        if (!((mask >> a) & 1u)) {
+            w[a] += wa;
+            continue;
+        }
+        const uint32_t F = a ^ c;
+        const double share = wa / (double)(1u << __popc(F));
+        for (uint32_t f = 0; f < 8; ++f)
+            if ((f & ~F) == 0) w[a ^ f] += share;
+    }
+    int o = 0;
+    for (int k = 0; k < 8; ++k) {
+        if (w[k] == 0.0) continue;
+        const uint32_t col = cconn[(size_t)k * nce + e];
+        if (bdy_c[col]) continue;
+        pcol[(size_t)o * nf + n] = col;
+        pval[(size_t)o * nf + n] = w[k];
+        ++o;
+    }
+}
+__global__ void k_prolong_keys(const uint32_t *pcol, uint32_t nf, Key *keys, uint32_t *vals, uint32_t *cnt) {
+    uint32_t n = blockIdx.x * B + threadIdx.x;
+    uint32_t nloc = 0;
+    Key loc[8];
+    uint32_t lv[8];
+    if (n < nf)
+        for (int k = 0; k < 8; ++k) {
+            uint32_t c = pcol[(size_t)k * nf + n];
+            if (c == NONE) break;
+            loc[nloc] = Key{(uint64_t)c, (uint64_t)n};
+            lv[nloc] = (uint32_t)k * nf + n;
+            ++nloc;
+        }
+    uint32_t base = wave_reserve(cnt, nloc);
+    for (uint32_t j = 0; j < nloc; ++j) {
+        keys[base + j] = loc[j];
+        vals[base + j] = lv[j];
+    }
+}
+__global__ void k_restrict_csr(const Key *keys, const uint32_t *vals, uint32_t nnz, const double *pval, uint32_t *rcol,
+                               double *rval, float *rval32, uint32_t *rowcnt) {
+    uint32_t p = blockIdx.x * B + threadIdx.x;
+    if (p >= nnz) return;
+    rcol[p] = (uint32_t)keys[p].lo;
+    rval[p] = pval[vals[p]];
+    rval32[p] = (float)pval[vals[p]];
+    atomicAdd(&rowcnt[(uint32_t)keys[p].hi], 1u);
+}
+__global__ void k_to_f32(const double *a, size_t n, float *b) {
+    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_dinv(const double *diag, const uint8_t *bdy, uint32_t n, double *dinv, float *dinv32) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    double v = bdy[i] ? 1.0 : 1.0 / diag[i];
+    dinv[i] = v;
+    dinv32[i] = (float)v;
+}
+
+// Gauss-Jordan inversion of an SPD matrix, one pivot per launch (no pivoting).
+__global__ void k_gj_scale(double *A, uint32_t n, uint32_t k, double *rowk) {
+    uint32_t j = blockIdx.x * B + threadIdx.x;
+    if (j >= n) return;
+    const double p = A[(size_t)k * n + k];
+    double v = (j == k) ? 1.0 / p : A[(size_t)k * n + j] / p;
+    rowk[j] = v;
+}
+__global__ void k_gj_update(double *A, uint32_t n, uint32_t k, const double *rowk, double *colk) {
+    const uint32_t j = blockIdx.x * B + threadIdx.x, i = blockIdx.y;
+    if (j >= n) return;
+    const double aik = colk[i];
+    if (i == k) {
+        A[(size_t)k * n + j] = rowk[j];
+    } else {
+        const double v = (j == k) ? 0.0 : A[(size_t)i * n + j];
+        A[(size_t)i * n + j] = v - aik * rowk[j];
+    }
+}
+__global__ void k_gj_col(const double *A, uint32_t n, uint32_t k, double *colk) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) colk[i] = A[(size_t)i * n + k];
+}
+
+// -------------------------------------------------------- vector kernels ----
+template <typename T>
+__global__ void k_jacobi_first(const T *b, const T *dinv, T omega, uint32_t n, T *x) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) x[i] = omega * dinv[i] * b[i];
+}
+template <typename T>
+__global__ void k_jacobi(const T *b, const T *ax, const T *dinv, T omega, uint32_t n, T *x) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) x[i] += omega * dinv[i] * (b[i] - ax[i]);
+}
+template <typename T>
+__global__ void k_residual(const T *b, const T *ax, uint32_t n, T *r) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) r[i] = b[i] - ax[i];
+}
+template <typename T>
+__global__ void k_restrict(const uint32_t *roff, const uint32_t *rcol, const T *rval, uint32_t nc, const T *rf,
+                           T *bc) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i >= nc) return;
+    T acc = T(0);
+    for (uint32_t p = roff[i]; p < roff[i + 1]; ++p) acc += rval[p] * rf[rcol[p]];
+    bc[i] = acc;
+}
+template <typename T>
+__global__ void k_prolong_add(const uint32_t *pcol, const T *pval, uint32_t nf, const T *xc, T *xf) {
+    uint32_t n = blockIdx.x * B + threadIdx.x;
+    if (n >= nf) return;
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t c = pcol[(size_t)k * nf + n];
+        if (c == NONE) break;
+        acc += pval[(size_t)k * nf + n] * xc[c];
+    }
+    xf[n] += acc;
+}
+// dense y = A x (n x n row-major), one wave per row
+template <typename T>
+__global__ void k_dense_gemv(const double *A, uint32_t n, const T *x, T *y) {
+    const uint32_t row = blockIdx.x * (B / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double acc = 0.0;
+    for (uint32_t j = lane; j < n; j += 64) acc += A[(size_t)row * n + j] * (double)x[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) y[row] = (T)acc;
+}
+__global__ void k_f64_to_f32(const double *a, uint32_t n, float *b) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_f32_to_f64(const float *a, uint32_t n, double *b) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+// Two-stage deterministic dot products: per-block partial sums, then one block.
+__global__ void k_dot_partial(const double *a, const double *b, uint32_t n, double *part) {
+    __shared__ double sh[B / 64];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) acc += a[i] * b[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < B / 64; ++w) t += sh[w];
+        part[blockIdx.x] = t;
+    }
+}
+__global__ void k_dot_final(const double *part, uint32_t np, double *out) {
+    __shared__ double sh[B / 64];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < np; i += B) acc += part[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < B / 64; ++w) t += sh[w];
+        *out = t;
+    }
+}
+// CG updates with scalars held in device memory (s[0]=alpha numerator etc.)
+__global__ void k_cg_xr(const double *p, const double *q, const double *sc, uint32_t n, double *x, double *r,
+                        double *rold) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = sc[0] / sc[1];
+    rold[i] = r[i];
+    x[i] += alpha * p[i];
+    r[i] -= alpha * q[i];
+}
+__global__ void k_cg_p(const double *z, const double *sc, uint32_t n, double *p) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
+    const double beta = (sc[2] - sc[3]) / sc[0];  // (z.r - z.r_old) / (z_old . r_old)
+    p[i] = z[i] + beta * p[i];
+}
+__global__ void k_mask_bdy(const uint8_t *bdy, uint32_t n, double *v) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n && bdy[i]) v[i] = 0.0;
+}
+
+constexpr uint32_t DOT_BLOCKS = 512;
+
+}  // namespace
+
+// ---------------------------------------------------------------- build ----
+DevBuf<Key> coarsen_build(const Key *in, size_t n, hipStream_t s) {
+    DevBuf<Key> out;
+    if (n < 8) {
+        out.alloc(n, s);
+        if (n) HIP_OK(hipMemcpyAsync(out.p, in, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+        return out;
+    }
+    DevBuf<uint32_t> keep(n, s), start(n, s), pos(n, s);
+    k_coarsen_flags<<<grid_for(n, B), B, 0, s>>>(in, n, keep.p, start.p);
+    k_coarsen_mark<<<grid_for(n, B), B, 0, s>>>(start.p, n, keep.p);
+    uint64_t m = scan_exclusive_u32(keep.p, pos.p, n, s);
+    out.alloc(m, s);
+    k_coarsen_emit<<<grid_for(n, B), B, 0, s>>>(in, keep.p, start.p, pos.p, n, out.p);
+    HIP_OK(hipGetLastError());
+    return out;
+}
+
+static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int curve, hipStream_t s) {
+    const DaDevice &f = *fine.da, &c = *coarse.da;
+    Transfer &t = coarse.to_finer;
+    const uint32_t nf = f.n_nodes;
+    t.nf = nf;
+    t.nc = c.n_nodes;
+    t.pcol.alloc(8 * (size_t)nf, s);
+    t.pval.alloc(8 * (size_t)nf, s);
+    DevBuf<uint32_t> err(1, s);
+    HIP_OK(hipMemsetAsync(err.p, 0, 4, s));
+    k_prolong_rows<<<grid_for(nf, B), B, 0, s>>>(nf, f.n2l.p, f.x.p, f.y.p, f.z.p, f.bdy_node.p, c.key.p, c.n_elem,
+                                                c.x.p, c.y.p, c.z.p, c.flags.p, c.conn.p, c.meta.p, c.bdy_node.p,
+                                                md0 + 1, make_packed(curve), t.pcol.p, t.pval.p, err.p);
+    HIP_OK(hipGetLastError());
+    t.pval32.alloc(8 * (size_t)nf, s);
+    k_to_f32<<<grid_for(8 * (size_t)nf, B), B, 0, s>>>(t.pval.p, 8 * (size_t)nf, t.pval32.p);
+    // transpose into CSR over coarse nodes
+    DevBuf<Key> keys(8 * (size_t)nf, s);
+    DevBuf<uint32_t> vals(8 * (size_t)nf, s), cnt(1, s);
+    HIP_OK(hipMemsetAsync(cnt.p, 0, 4, s));
+    k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(t.pcol.p, nf, keys.p, vals.p, cnt.p);
+    uint32_t hcnt = 0, herr = 0;
+    HIP_OK(hipMemcpyAsync(&hcnt, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (herr) {
+        dgx_set_error("multigrid: interpolation lookup failed (code %u)", herr);
+        throw DgxError();
+    }
+    t.nnz = hcnt;
+    radix_sort_keys(keys.p, vals.p, hcnt, s, 0, 128);
+    t.rcol.alloc(hcnt, s);
+    t.rval.alloc(hcnt, s);
+    t.rval32.alloc(hcnt, s);
+    DevBuf<uint32_t> rowcnt(t.nc, s);
+    HIP_OK(hipMemsetAsync(rowcnt.p, 0, t.nc * 4, s));
+    if (hcnt)
+        k_restrict_csr<<<grid_for(hcnt, B), B, 0, s>>>(keys.p, vals.p, hcnt, t.pval.p, t.rcol.p, t.rval.p,
+                                                      t.rval32.p, rowcnt.p);
+    t.roff.alloc(t.nc + 1, s);
+    uint64_t tot = scan_exclusive_u32(rowcnt.p, t.roff.p, t.nc, s);
+    uint32_t tail = (uint32_t)tot;
+    HIP_OK(hipMemcpyAsync(t.roff.p + t.nc, &tail, 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s) {
+    const DaDevice &d = *L.da;
+    DevBuf<double> diag(d.n_nodes, s);
+    da_diagonal_f64(d, diag.p, lap, mass, s);
+    L.dinv.alloc(d.n_nodes, s);
+    L.dinv32.alloc(d.n_nodes, s);
+    k_dinv<<<grid_for(d.n_nodes, B), B, 0, s>>>(diag.p, d.bdy_node.p, d.n_nodes, L.dinv.p, L.dinv32.p);
+    HIP_OK(hipGetLastError());
+}
+
+static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
+    const DaDevice &d = *mg.lev.back().da;
+    const uint32_t n = d.n_nodes;
+    const uint32_t kMaxDense = 16384;
+    if (n > kMaxDense) {
+        dgx_set_error("multigrid: coarsest level has %u nodes (> %u for the dense FP64 solve); use more levels", n,
+                      kMaxDense);
+        throw DgxError();
+    }
+    mg.nc = n;
+    mg.cinv.alloc((size_t)n * n, s);
+    da_assemble_dense_dirichlet(d, mg.lap_fac, mg.mass_fac, mg.cinv.p, s);
+    DevBuf<double> rowk(n, s), colk(n, s);
+    for (uint32_t k = 0; k < n; ++k) {
+        k_gj_scale<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k, rowk.p);
+        k_gj_col<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k, colk.p);
+        k_gj_update<<<dim3(grid_for(n, B), n), B, 0, s>>>(mg.cinv.p, n, k, rowk.p, colk.p);
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
+                                      double lap_fac, double mass_fac, hipStream_t s) {
+    std::unique_ptr<MgHierarchy> mg(new MgHierarchy());
+    mg->md = md;
+    mg->curve = curve;
+    mg->lap_fac = lap_fac;
+    mg->mass_fac = mass_fac;
+    if (nlevels < 1) nlevels = 1;
+    mg->lev.reserve(32);
+    mg->lev.emplace_back();
+    MgLevel &L0 = mg->lev.back();
+    L0.keys.alloc(n, s);
+    HIP_OK(hipMemcpyAsync(L0.keys.p, finest, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+    L0.n_oct = n;
+    L0.da = da_build(L0.keys.p, n, md, curve, s);
+    const uint32_t kMaxDense = 12288;
+    for (int k = 1; k < 30; ++k) {
+        // nlevels is honoured, but the hierarchy keeps coarsening while the
+        // coarsest level is too large for the dense FP64 coarse solve
+        if (k >= nlevels && mg->lev.back().da->n_nodes <= kMaxDense) break;
+        MgLevel &prev = mg->lev.back();
+        DevBuf<Key> c = coarsen_build(prev.keys.p, prev.n_oct, s);
+        DevBuf<Key> b = balance_build(c.p, c.n, md, curve, s, nullptr);
+        if (b.n >= prev.n_oct || b.n <= 8) break;  // nothing left to coarsen
+        mg->lev.emplace_back();
+        MgLevel &L = mg->lev.back();
+        L.n_oct = b.n;
+        L.keys = std::move(b);
+        L.da = da_build(L.keys.p, L.n_oct, md, curve, s);
+        if (L.n_oct < 500 && L.da->n_nodes <= kMaxDense) break;  // blockDiag.cpp:1417-1420
+    }
+    for (auto &L : mg->lev) build_level_diag(L, lap_fac, mass_fac, s);
+    for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
+    build_coarse_inverse(*mg, s);
+    return mg;
+}
+
+// ---------------------------------------------------------------- solve ----
+namespace {
+
+template <typename T>
+struct Work {
+    std::vector<DevBuf<T>> x, b, y;
+};
+
+template <typename T>
+const T *dinv_of(const MgLevel &L);
+template <>
+const double *dinv_of<double>(const MgLevel &L) { return L.dinv.p; }
+template <>
+const float *dinv_of<float>(const MgLevel &L) { return L.dinv32.p; }
+template <typename T>
+const T *pval_of(const Transfer &t);
+template <>
+const double *pval_of<double>(const Transfer &t) { return t.pval.p; }
+template <>
+const float *pval_of<float>(const Transfer &t) { return t.pval32.p; }
+template <typename T>
+const T *rval_of(const Transfer &t);
+template <>
+const double *rval_of<double>(const Transfer &t) { return t.rval.p; }
+template <>
+const float *rval_of<float>(const Transfer &t) { return t.rval32.p; }
+
+inline void matvec_dir(const DaDevice &d, const double *x, double *y, double lap, double mass, hipStream_t s) {
+    da_matvec_dirichlet_f64(d, x, y, lap, mass, s);
+}
+inline void matvec_dir(const DaDevice &d, const float *x, float *y, double lap, double mass, hipStream_t s) {
+    da_matvec_dirichlet_f32(d, x, y, (float)lap, (float)mass, s);
+}
+
+template <typename T>
+void vcycle(MgHierarchy &mg, Work<T> &w, size_t k, const SolveOpts &o, hipStream_t s) {
+    MgLevel &L = mg.lev[k];
+    const uint32_t n = L.da->n_nodes;
+    const unsigned g = grid_for(n, B);
+    T *x = w.x[k].p, *b = w.b[k].p, *y = w.y[k].p;
+    if (k + 1 == mg.lev.size()) {  // coarsest: dense FP64 inverse
+        k_dense_gemv<T><<<grid_for(n, B / 64), B, 0, s>>>(mg.cinv.p, n, b, x);
+        return;
+    }
+    const T om = (T)o.omega;
+    const T *dinv = dinv_of<T>(L);
+    k_jacobi_first<T><<<g, B, 0, s>>>(b, dinv, om, n, x);
+    for (int it = 1; it < o.pre; ++it) {
+        matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
+        k_jacobi<T><<<g, B, 0, s>>>(b, y, dinv, om, n, x);
+    }
+    matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
+    k_residual<T><<<g, B, 0, s>>>(b, y, n, y);
+    const Transfer &t = mg.lev[k + 1].to_finer;
+    k_restrict<T><<<grid_for(t.nc, B), B, 0, s>>>(t.roff.p, t.rcol.p, rval_of<T>(t), (uint32_t)t.nc, y, w.b[k + 1].p);
+    vcycle<T>(mg, w, k + 1, o, s);
+    k_prolong_add<T><<<g, B, 0, s>>>(t.pcol.p, pval_of<T>(t), n, w.x[k + 1].p, x);
+    for (int it = 0; it < o.post; ++it) {
+        matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
+        k_jacobi<T><<<g, B, 0, s>>>(b, y, dinv, om, n, x);
+    }
+}
+
+template <typename T>
+void ensure_work(MgHierarchy &mg, Work<T> &w, hipStream_t s) {
+    if (w.x.size() == mg.lev.size()) return;
+    w.x.clear();
+    w.b.clear();
+    w.y.clear();
+    for (auto &L : mg.lev) {
+        w.x.emplace_back(L.da->n_nodes, s);
+        w.b.emplace_back(L.da->n_nodes, s);
+        w.y.emplace_back(L.da->n_nodes, s);
+    }
+}
+
+__global__ void k_copy(const double *a, uint32_t n, double *b) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void k_zero(uint32_t n, double *b) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) b[i] = 0.0;
+}
+__global__ void k_scal_copy(double *sc, int from, int to) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) sc[to] = sc[from];
+}
+
+struct Graph {
+    hipGraphExec_t exec = nullptr;
+    SolveOpts key{};
+    bool valid = false;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        valid = false;
+    }
+    ~Graph() { reset(); }
+};
+
+bool same_opts(const SolveOpts &a, const SolveOpts &b) {
+    return a.fp32_vcycle == b.fp32_vcycle && a.pre == b.pre && a.post == b.post && a.omega == b.omega;
+}
+
+}  // namespace
+
+// CG vectors, device scalars and captured graphs, owned by the hierarchy so
+// that every captured pointer stays valid for the hierarchy's lifetime.
+// sc: [0] z.r  [1] p.q  [2] z_new.r_new  [3] z_new.r_old  [4] r.r  [5] b.b
+struct MgWork {
+    Work<double> w64;
+    Work<float> w32;
+    DevBuf<double> x, bvec, r, z, p, q, rold, part, sc;
+    Graph g_precond, g_step, g_dir;
+};
+MgHierarchy::~MgHierarchy() { delete work; }
+
+namespace {
+
+MgWork &work_of(MgHierarchy &mg, hipStream_t s) {
+    if (!mg.work) {
+        mg.work = new MgWork();
+        const uint32_t n = mg.lev[0].da->n_nodes;
+        MgWork &w = *mg.work;
+        w.x.alloc(n, s);
+        w.bvec.alloc(n, s);
+        w.r.alloc(n, s);
+        w.z.alloc(n, s);
+        w.p.alloc(n, s);
+        w.q.alloc(n, s);
+        w.rold.alloc(n, s);
+        w.part.alloc(DOT_BLOCKS, s);
+        w.sc.alloc(8, s);
+        ensure_work(mg, w.w64, s);
+        ensure_work(mg, w.w32, s);
+    }
+    return *mg.work;
+}
+
+void dot_dev(const double *a, const double *b, uint32_t n, double *part, double *out, hipStream_t s) {
+    k_dot_partial<<<DOT_BLOCKS, B, 0, s>>>(a, b, n, part);
+    k_dot_final<<<1, B, 0, s>>>(part, DOT_BLOCKS, out);
+}
+
+// z = M r on the work vectors (V-cycle in FP32 or FP64).
+void precond_body(MgHierarchy &mg, MgWork &w, const SolveOpts &o, hipStream_t s) {
+    const uint32_t n = mg.lev[0].da->n_nodes;
+    if (o.fp32_vcycle) {
+        k_f64_to_f32<<<grid_for(n, B), B, 0, s>>>(w.r.p, n, w.w32.b[0].p);
+        vcycle<float>(mg, w.w32, 0, o, s);
+        k_f32_to_f64<<<grid_for(n, B), B, 0, s>>>(w.w32.x[0].p, n, w.z.p);
+    } else {
+        k_copy<<<grid_for(n, B), B, 0, s>>>(w.r.p, n, w.w64.b[0].p);
+        vcycle<double>(mg, w.w64, 0, o, s);
+        k_copy<<<grid_for(n, B), B, 0, s>>>(w.w64.x[0].p, n, w.z.p);
+    }
+}
+// q = A p; p.q; x += alpha p; r -= alpha q; r.r
+void step_body(MgHierarchy &mg, MgWork &w, hipStream_t s) {
+    const DaDevice &d = *mg.lev[0].da;
+    const uint32_t n = d.n_nodes;
+    da_matvec_dirichlet_f64(d, w.p.p, w.q.p, mg.lap_fac, mg.mass_fac, s);
+    dot_dev(w.p.p, w.q.p, n, w.part.p, w.sc.p + 1, s);
+    k_cg_xr<<<grid_for(n, B), B, 0, s>>>(w.p.p, w.q.p, w.sc.p, n, w.x.p, w.r.p, w.rold.p);
+    dot_dev(w.r.p, w.r.p, n, w.part.p, w.sc.p + 4, s);
+}
+// z = M r; z.r; z.r_old; p = z + beta p; z.r -> sc[0]
+void direction_body(MgHierarchy &mg, MgWork &w, const SolveOpts &o, hipStream_t s) {
+    const uint32_t n = mg.lev[0].da->n_nodes;
+    precond_body(mg, w, o, s);
+    dot_dev(w.z.p, w.r.p, n, w.part.p, w.sc.p + 2, s);
+    dot_dev(w.z.p, w.rold.p, n, w.part.p, w.sc.p + 3, s);
+    k_cg_p<<<grid_for(n, B), B, 0, s>>>(w.z.p, w.sc.p, n, w.p.p);
+    k_scal_copy<<<1, 64, 0, s>>>(w.sc.p, 2, 0);
+}
+
+template <typename F>
+void run_graph(Graph &g, const SolveOpts &o, bool use_graph, hipStream_t s, F body) {
+    if (!use_graph) {
+        body();
+        return;
+    }
+    if (!g.valid || !same_opts(g.key, o)) {
+        g.reset();
+        hipGraph_t graph;
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        HIP_OK(hipStreamEndCapture(s, &graph));
+        HIP_OK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+        HIP_OK(hipGraphDestroy(graph));
+        g.key = o;
+        g.valid = true;
+    }
+    HIP_OK(hipGraphLaunch(g.exec, s));
+}
+
+double read_scalar(const double *p, hipStream_t s) {
+    double h;
+    HIP_OK(hipMemcpyAsync(&h, p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace
+
+void mg_vcycle_apply(MgHierarchy &mg, const double *r, double *z, const SolveOpts &o, hipStream_t s) {
+    MgWork &w = work_of(mg, s);
+    const uint32_t n = mg.lev[0].da->n_nodes;
+    k_copy<<<grid_for(n, B), B, 0, s>>>(r, n, w.r.p);
+    run_graph(w.g_precond, o, o.use_graph, s, [&] { precond_body(mg, w, o, s); });
+    k_copy<<<grid_for(n, B), B, 0, s>>>(w.z.p, n, z);
+    HIP_OK(hipGetLastError());
+}
+
+void mg_rhs_from_nodal(MgHierarchy &mg, const double *f, double *b, hipStream_t s) {
+    const DaDevice &d = *mg.lev[0].da;
+    da_matvec_f64(d, f, b, 0.0, 1.0, s);  // M f
+    k_mask_bdy<<<grid_for(d.n_nodes, B), B, 0, s>>>(d.bdy_node.p, d.n_nodes, b);
+    HIP_OK(hipGetLastError());
+}
+
+void mg_solve(MgHierarchy &mg, const double *b, double *xout, const SolveOpts &o, SolveStats *st, hipStream_t s) {
+    auto t0 = std::chrono::steady_clock::now();
+    MgWork &w = work_of(mg, s);
+    const DaDevice &d = *mg.lev[0].da;
+    const uint32_t n = d.n_nodes;
+    const unsigned g = grid_for(n, B);
+    k_copy<<<g, B, 0, s>>>(b, n, w.bvec.p);
+    k_zero<<<g, B, 0, s>>>(n, w.x.p);
+    k_copy<<<g, B, 0, s>>>(w.bvec.p, n, w.r.p);
+    dot_dev(w.bvec.p, w.bvec.p, n, w.part.p, w.sc.p + 5, s);
+    const double bnorm = std::sqrt(read_scalar(w.sc.p + 5, s));
+    SolveStats res;
+    if (bnorm == 0.0) {
+        k_zero<<<g, B, 0, s>>>(n, xout);
+        if (st) *st = res;
+        return;
+    }
+    run_graph(w.g_precond, o, o.use_graph, s, [&] { precond_body(mg, w, o, s); });
+    k_copy<<<g, B, 0, s>>>(w.z.p, n, w.p.p);
+    dot_dev(w.r.p, w.z.p, n, w.part.p, w.sc.p + 0, s);
+    int it = 0;
+    double rel = 1.0;
+    for (it = 1; it <= o.maxit; ++it) {
+        run_graph(w.g_step, o, o.use_graph, s, [&] { step_body(mg, w, s); });
+        rel = std::sqrt(read_scalar(w.sc.p + 4, s)) / bnorm;
+        if (rel <= o.rtol) break;
+        run_graph(w.g_dir, o, o.use_graph, s, [&] { direction_body(mg, w, o, s); });
+    }
+    // true residual in FP64
+    da_matvec_dirichlet_f64(d, w.x.p, w.q.p, mg.lap_fac, mg.mass_fac, s);
+    k_residual<double><<<g, B, 0, s>>>(w.bvec.p, w.q.p, n, w.r.p);
+    dot_dev(w.r.p, w.r.p, n, w.part.p, w.sc.p + 6, s);
+    res.rel_residual = std::sqrt(read_scalar(w.sc.p + 6, s)) / bnorm;
+    k_copy<<<g, B, 0, s>>>(w.x.p, n, xout);
+    res.iterations = it > o.maxit ? o.maxit : it;
+    HIP_OK(hipStreamSynchronize(s));
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = res;
+}

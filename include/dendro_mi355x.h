@@ -108,6 +108,49 @@ int dgx_da_matvec_f32(dgx_da da, const float *x, float *y, float lap_fac, float 
 int dgx_da_diagonal(dgx_da da, double *diag, double lap_fac, double mass_fac);
 int dgx_da_free(dgx_da da);
 
+/* Homogeneous-Dirichlet operator: DirichletJacobianMatMult (examples/src/
+ * backend/omgJac.C:958-1057): rows of nodes on the boundary of the unit cube
+ * are the identity, boundary columns are dropped. */
+int dgx_da_matvec_dirichlet(dgx_da da, const double *x, double *y, double lap_fac, double mass_fac);
+
+/* ---- multigrid ---------------------------------------------------------- */
+typedef struct dgx_mg_s *dgx_mg;
+
+/* DAMGCreateAndSetDA + DAMGSetKSP (include/omg/omg.h:470-528; src/omg/
+ * blockDiag.cpp:780-1047, 1312-2190) on one rank: nlevels levels built by
+ * coarsenOctree + balanceOctree from the finest octree (stopping early below
+ * 500 octants, blockDiag.cpp:1417-1420; coarsening continues past nlevels
+ * while the coarsest level has more than 12288 nodes, so that the dense FP64
+ * coarse solve fits), a DA per level, interpolation P and
+ * restriction R = P^T between levels, Jacobi diagonals and the coarsest-level
+ * FP64 direct solver.  The octree is not consumed. */
+int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fac, dgx_mg *out);
+/* Number of levels actually built and, per level (finest first), the number of
+ * octants and of nodes (arrays of at least 32 entries, may be NULL). */
+int dgx_mg_info(dgx_mg mg, int *nlevels, uint64_t *octants, uint64_t *nodes);
+/* The finest level's DA (owned by mg; do not free). */
+int dgx_mg_finest_da(dgx_mg mg, dgx_da *da);
+/* Load vector b = M f of a nodal f with boundary entries set to zero. */
+int dgx_mg_rhs(dgx_mg mg, const double *f, double *b);
+
+typedef struct {
+    int fp32_vcycle; /* 1: V-cycle in FP32 inside FP64 CG; 0: all FP64 */
+    int pre, post;   /* Jacobi sweeps before / after the coarse correction */
+    double omega;    /* Jacobi damping */
+    double rtol;     /* stop at ||b - A u|| <= rtol ||b|| */
+    int maxit;
+    int use_graph;   /* 1: replay captured HIP graphs of the CG step and V-cycle */
+} dgx_solve_opts;
+typedef struct {
+    int iterations;
+    double rel_residual; /* recomputed in FP64 after the last iteration */
+    double seconds;
+} dgx_solve_stats;
+/* DAMGSolve (blockDiag.cpp:1122-1170): CG preconditioned by one V-cycle per
+ * iteration on device vectors b, u of n_nodes entries (finest level). */
+int dgx_mg_solve(dgx_mg mg, const double *b, double *u, const dgx_solve_opts *opts, dgx_solve_stats *stats);
+int dgx_mg_free(dgx_mg mg);
+
 /* ---- inputs ------------------------------------------------------------- */
 /* Deterministic synthetic point clouds (host memory, n*3 doubles):
  *   dist 0: i.i.d. uniform [0,1)^3

@@ -1,0 +1,124 @@
+"""GPU tests of the Dirichlet operator and the multigrid-preconditioned CG
+solve.  The reference operator is assembled on the CPU from the oracle's DA
+and the reference's type-2 stencil tables (Jacobian1MatMult element loop,
+odaJac.C:275-292; Dirichlet rows/columns as in omgJac.C:958-1057) and solved
+with scipy's sparse direct solver."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+
+
+def assemble(rda, L, M, lap=1.0, mass=1.0):
+    conn, cn, lev, mk = rda.element_view()
+    from oracle import pyoracle as O
+    tmap = O.etype_masks()
+    E = rda.n_elem
+    t = np.zeros(E, np.int64)
+    hang = np.nonzero(mk)[0]
+    for e in hang:
+        t[e] = int(np.nonzero(tmap[cn[e]] == mk[e])[0][0])
+    h = 2.0 ** -(lev.astype(np.float64) - 1)
+    Ae = (lap * h / 2)[:, None, None] * L[cn, t] + (mass * h ** 3 / 8)[:, None, None] * M[cn, t]
+    rows = np.repeat(conn[:, :, None], 8, axis=2).ravel()
+    cols = np.repeat(conn[:, None, :], 8, axis=1).ravel()
+    A = sp.coo_matrix((Ae.ravel(), (rows, cols)), shape=(rda.n_nodes, rda.n_nodes)).tocsr()
+    return A
+
+
+def boundary_nodes(rda):
+    l2n = rda.local_to_node
+    idx = np.nonzero(l2n != 0xFFFFFFFF)[0]
+    pos = rda.octs[idx, :3].astype(np.int64)
+    D = 1 << 30
+    bd = np.zeros(rda.n_nodes, bool)
+    bd[l2n[idx]] = ((pos == 0) | (pos == D)).any(1)
+    return bd
+
+
+def dirichlet(A, bd):
+    d = sp.diags((~bd).astype(np.float64))
+    return (d @ A @ d + sp.diags(bd.astype(np.float64))).tocsr()
+
+
+@pytest.fixture(scope="module")
+def problem(dgx, oracle, stencils):
+    L, M = stencils
+    pts = dgx.gen_points(1, 21, 30000)
+    ou = dgx.points2octree(pts, 30, 8)
+    ob = dgx.balance_octree(ou)
+    rda = oracle.DA(ob.treenodes())
+    A = assemble(rda, L, M)
+    bd = boundary_nodes(rda)
+    return ob, rda, A, bd
+
+
+def test_dirichlet_matvec_parity(dgx, problem):
+    ob, rda, A, bd = problem
+    da = dgx.DA(ob)
+    x = np.random.default_rng(0).standard_normal(da.n_nodes)
+    AD = dirichlet(A, bd)
+    y = da.matvec_dirichlet_host(x)
+    yr = AD @ x
+    assert np.abs(y - yr).max() / np.abs(yr).max() < 1e-12
+    yn = da.matvec_host(x)
+    assert np.abs(yn - A @ x).max() / np.abs(A @ x).max() < 1e-12
+
+
+@pytest.mark.parametrize("nlevels", [1, 3, 5])
+def test_mg_solve_matches_direct_solve(dgx, problem, nlevels):
+    ob, rda, A, bd = problem
+    mg = dgx.MG(ob, nlevels)
+    print("levels", mg.nlevels, "octants", mg.level_octants, "nodes", mg.level_nodes, flush=True)
+    assert mg.level_nodes[0] == rda.n_nodes
+    assert all(a > b for a, b in zip(mg.level_octants, mg.level_octants[1:]))
+    n = mg.n_nodes
+    f = np.random.default_rng(1).standard_normal(n)
+    df = dgx.DeviceArray.from_host(f)
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    b = db.to_host()
+    M = assemble(rda, *[np.load("tests/golden/type2_stencils.npz")[k] for k in ("lap", "mass")], lap=0.0, mass=1.0)
+    br = M @ f
+    br[bd] = 0.0
+    assert np.abs(b - br).max() / np.abs(br).max() < 1e-12
+    AD = dirichlet(A, bd)
+    uref = spla.spsolve(AD.tocsc(), br)
+    res = {}
+    for fp32 in (False, True):
+        du = dgx.DeviceArray((n,), np.float64)
+        st = mg.solve(db, du, fp32=fp32, pre=2, post=2, omega=0.8, rtol=1e-8, maxit=300)
+        u = du.to_host()
+        true_res = np.linalg.norm(br - AD @ u) / np.linalg.norm(br)
+        err = np.linalg.norm(u - uref) / np.linalg.norm(uref)
+        print(f"nlevels={mg.nlevels} fp32={fp32} it={st['iterations']} relres={st['rel_residual']:.2e} "
+              f"true={true_res:.2e} err={err:.2e} t={st['seconds']*1e3:.1f}ms", flush=True)
+        assert st["rel_residual"] <= 1e-8 and true_res <= 1.5e-8
+        assert err <= 1e-7
+        res[fp32] = u
+    assert np.linalg.norm(res[True] - res[False]) / np.linalg.norm(res[False]) <= 1e-7
+
+
+def test_graph_replay_matches_eager(dgx, problem):
+    """The captured CG step and V-cycle replay the same kernels in the same
+    order as the eager path.  The element loop accumulates in LDS with FP64
+    atomics, whose order is not fixed, so solves agree to rounding, not bits."""
+    ob, rda, A, bd = problem
+    mg = dgx.MG(ob, 4)
+    n = mg.n_nodes
+    df = dgx.DeviceArray.from_host(np.random.default_rng(3).standard_normal(n))
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    for fp32 in (True, False):
+        du = dgx.DeviceArray((n,), np.float64)
+        st0 = mg.solve(db, du, fp32=fp32, graph=False)
+        u0 = du.to_host()
+        for rep in range(3):
+            du.zero()
+            st = mg.solve(db, du, fp32=fp32, graph=True)
+            assert abs(st["iterations"] - st0["iterations"]) <= 1
+            u = du.to_host()
+            assert np.linalg.norm(u - u0) / np.linalg.norm(u0) <= 1e-9, (fp32, rep)
+            assert st["rel_residual"] <= 1e-8
