@@ -1,11 +1,22 @@
-"""Benchmark: octree build + 2:1 balance (octants/s) and the FP64 -lap+I matvec
-on one MI355X, configuration 2 (4,194,304 log-normal points, maxNumPts=8,
-maxDepth=30, corner balance, 100 matvecs).
+"""Benchmark of the octree-FEM path on one MI355X.
 
-One step = points2Octree + balanceOctree over the resident point cloud.
-Prints one JSON line (see the "metric"/"config"/"roofline"/"cpu_baseline"
-fields).  The oracle (CPU restatement) is timed on a bounded sample of the same
-workload on the host as the CPU baseline.
+Headline (value): octree build + 2:1 corner balance, octants/s, on the
+lognormal-4M workload: 4,194,304 points with coordinates exp(N(-1,0.5))
+clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8.  One step =
+points2Octree + balanceOctree over the point cloud already resident in HBM.
+
+Also reported on the same line:
+  roofline      the FP64 -lap+I matvec on that octree's DA (100 launches,
+                hipEvents on the library stream), algorithmic bytes
+                34 E + 16 Nn per launch against 8 TB/s;
+  solve         Poisson -lap u + u = f, homogeneous Dirichlet, MG(5)-V(2,2)
+                Jacobi(0.8)-preconditioned CG to 1e-8, all-FP64 against the
+                FP32 V-cycle inside FP64 CG: solves/s, p50/p99 latency per
+                solve, error of the mixed solve against the FP64 one; on the
+                lognormal-4M octree and on gauss4-16M (16,777,216 points,
+                4 Gaussians sigma 0.02, seed 2, maxNumPts 4);
+  cpu_baseline  the oracle (CPU restatement, one thread) timed on the first
+                --cpu-sample points of the headline workload.
 """
 import argparse
 import ctypes as C
@@ -20,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_f64_lognormal-4M_traffic.txt")
 
 
 class HipEvents:
@@ -48,14 +60,73 @@ class HipEvents:
         return ms.value
 
 
+def read_traffic():
+    """HBM bytes per matvec launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel of known size), if that summary exists."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def solve_section(dgx, ob, nlevels, solves, warmup, seed, graph=True):
+    """FP64 and FP32-V-cycle solves of the Dirichlet problem on octree ob."""
+    t0 = time.perf_counter()
+    mg = dgx.MG(ob, nlevels)
+    dgx.synchronize()
+    t_setup = time.perf_counter() - t0
+    n = mg.n_nodes
+    f = dgx.DeviceArray.from_host(np.random.default_rng(seed).standard_normal(n))
+    b = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(f, b)
+    out = {"levels": mg.nlevels, "level_octants": mg.level_octants, "level_nodes": mg.level_nodes,
+           "mg_setup_s": t_setup}
+    sol = {}
+    for name, fp32 in (("fp64", False), ("fp32_mg", True)):
+        u = dgx.DeviceArray((n,), np.float64)
+        for _ in range(warmup):
+            mg.solve(b, u, fp32=fp32, graph=graph)
+        lat = []
+        st = None
+        for _ in range(solves):
+            t0 = time.perf_counter()
+            st = mg.solve(b, u, fp32=fp32, graph=graph)
+            dgx.synchronize()
+            lat.append(time.perf_counter() - t0)
+        sol[name] = u.to_host()
+        lat = np.array(lat)
+        out[name] = {"solves_per_s": float(len(lat) / lat.sum()), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "iterations": st["iterations"],
+                     "rel_residual": st["rel_residual"]}
+    out["speedup_fp32_mg_vs_fp64"] = out["fp32_mg"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["rel_err_fp32_mg_vs_fp64"] = float(np.linalg.norm(sol["fp32_mg"] - sol["fp64"]) /
+                                           np.linalg.norm(sol["fp64"]))
+    del mg
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--matvecs", type=int, default=100)
+    ap.add_argument("--solves", type=int, default=20)
+    ap.add_argument("--no-solve", action="store_true", help="skip the Poisson solves")
+    ap.add_argument("--no-large", action="store_true", help="skip the gauss4-16M solve")
+    ap.add_argument("--no-graph", action="store_true", help="solve without hipGraph replay (for tracing runs)")
     ap.add_argument("--cpu-sample", type=int, default=524288, help="points in the oracle's CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="build once, run the matvecs and a 256 MiB device copy (for counter runs), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -67,8 +138,10 @@ def main():
     stream = dgx.stream()
     ev = HipEvents()
 
-    N, MP, MD = 4194304, 8, 30
-    pts = dgx.config_points(2, N)
+    W = "lognormal-4M"
+    MD, MP = 30, dgx.workload_max_num_pts(W)
+    pts = dgx.workload_points(W)
+    N = len(pts)
     dpts = dgx.DeviceArray.from_host(pts)
     dgx.synchronize()
 
@@ -93,6 +166,7 @@ def main():
     U, Bn = len(ou), len(ob)
     octants_per_s = Bn / t_build
     build_bytes = 24 * N + 2 * 28 * U + 28 * Bn
+    log(f"build+balance {t_build * 1e3:.2f} ms/step ({Bn} octants)")
 
     # DA + FP64 matvec
     da = dgx.DA(ob)
@@ -112,6 +186,43 @@ def main():
     mv_avg_ms = float(np.mean(mv_ms))
     mv_bytes = 34 * E + 16 * Nn
     achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
+    log(f"matvec {mv_avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
+    block_info = da.block_info
+    del da, x, y
+    if args.matvec_only:
+        # reference copy kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE counters
+        nb = 256 << 20
+        src = dgx.DeviceArray((nb // 8,), np.float64)
+        dst = dgx.DeviceArray((nb // 8,), np.float64)
+        hip = ev.hip
+        hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        for _ in range(3):
+            assert hip.hipMemcpyAsync(dst.ptr, src.ptr, nb, 3, stream) == 0  # hipMemcpyDeviceToDevice
+        dgx.synchronize()
+        print(json.dumps({"matvec_avg_ms": mv_avg_ms, "algorithmic_bytes": mv_bytes, "GB_per_s": achieved,
+                          "copy_bytes": nb}))
+        return
+
+    solve = None
+    if not args.no_solve:
+        solve = {"lognormal-4M": solve_section(dgx, ob, 5, args.solves, 2, 2, not args.no_graph)}
+        log(f"solve lognormal-4M: {json.dumps(solve['lognormal-4M'])}")
+        if not args.no_large:
+            del ou, ob, dpts
+            W3 = "gauss4-16M"
+            p3 = dgx.DeviceArray.from_host(dgx.workload_points(W3))
+            dgx.synchronize()
+            t0 = time.perf_counter()
+            o3 = dgx.balance_octree(dgx.points2octree(p3, MD, dgx.workload_max_num_pts(W3)))
+            dgx.synchronize()
+            t3 = time.perf_counter() - t0
+            del p3
+            s3 = solve_section(dgx, o3, 5, max(5, args.solves // 2), 2, 2, not args.no_graph)
+            s3["build_balance_s"] = t3
+            s3["balanced_octants"] = len(o3)
+            solve[W3] = s3
+            log(f"solve {W3}: {json.dumps(s3)}")
+            del o3
 
     cpu = None
     if not args.no_cpu:
@@ -125,8 +236,8 @@ def main():
         rb = O.balance(ru)
         tc = time.perf_counter() - t0
         cpu = {"value": len(rb) / tc, "unit": "octants/s", "cores": 1, "kind": "port",
-               "sample": f"oracle points2Octree+balanceOctree on the first {len(sub)} points of the workload "
-                         f"({len(rb)} balanced octants, {tc:.1f} s, 1 thread)"}
+               "sample": f"oracle points2Octree+balanceOctree on the first {len(sub)} points of the lognormal-4M "
+                         f"cloud ({len(rb)} balanced octants, {tc:.1f} s, 1 thread)"}
 
     out = {
         "metric": "octants/s (build+2:1 balance); Poisson solves/s & p99 latency, FP32-MG vs FP64",
@@ -141,18 +252,19 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic",
-        "config": {"workload": "config 2: points2Octree + balanceOctree(corner) of 4,194,304 log-normal points "
-                               "exp(N(-1,0.5)) clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8; "
-                               "100 FP64 -lap+I matvecs on the DA",
+        "config": {"workload": "lognormal-4M: points2Octree + balanceOctree(corner) of 4,194,304 points, "
+                               "coordinates exp(N(-1,0.5)) clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8; "
+                               "100 FP64 -lap+I matvecs on its DA",
                    "points": N, "octants": U, "balanced_octants": Bn, "elements": E, "nodes": Nn},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_matvec<double> (FP64 -lap+I element matvec)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "FP64 -lap+I matvec: k_matvec_pipe<double,false,4> + k_matvec_shared<double,false>",
                      "algorithmic_bytes": mv_bytes, "avg_launch_ms": mv_avg_ms},
-        "matvec": {"da_blocks": da.block_info, "avg_ms": mv_avg_ms, "min_ms": float(np.min(mv_ms)), "GB_per_s": achieved,
-                   "flops": 332 * E, "GFLOP_per_s": 332 * E / (mv_avg_ms * 1e-3) / 1e9},
+        "matvec": {"da_blocks": block_info, "avg_ms": mv_avg_ms, "min_ms": float(np.min(mv_ms)),
+                   "GB_per_s": achieved, "flops": 332 * E, "GFLOP_per_s": 332 * E / (mv_avg_ms * 1e-3) / 1e9},
         "build_balance": {"s_per_step": t_build, "p2o_s": t_p2o, "algorithmic_bytes": build_bytes,
                           "GB_per_s": build_bytes / t_build / 1e9},
+        "solve": solve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

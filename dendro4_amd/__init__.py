@@ -413,19 +413,29 @@ def sfc_tables(curve=CURVE_HILBERT):
     return r.reshape(24, 8), m.reshape(24, 8), c.reshape(24, 8)
 
 
-# Benchmark configurations (point clouds) ------------------------------------
-def config_points(cfg, n=None):
-    """Point cloud of configuration cfg (1..5) with the stated seed; n overrides the size."""
-    if cfg == 1:
-        return gen_points(0, 0, n or 262144)
-    if cfg == 2:
-        return gen_points(1, 1, n or 4194304)
-    if cfg == 3:
-        return gen_points(2, 2, n or 16777216, n_gauss=4, lo=0.2, hi=0.8, sig_lo=0.02)
-    if cfg == 5:
-        return gen_points(2, 4, n or 134217728, n_gauss=16, lo=0.1, hi=0.9, sig_lo=0.005, sig_hi=0.05,
-                          dirichlet=True)
-    raise ValueError(cfg)
+# Benchmark workloads (point clouds) ----------------------------------------
+# name -> (generator arguments, maxNumPts); every cloud is deterministic.
+WORKLOADS = {
+    # 262,144 i.i.d. uniform points in [0,1)^3, seed 0, maxNumPts 1
+    "uniform-256K": (dict(dist=0, seed=0, n=262144), 1),
+    # 4,194,304 points, coordinates exp(N(-1, 0.5)) clipped to [0,1), seed 1, maxNumPts 8
+    "lognormal-4M": (dict(dist=1, seed=1, n=4194304), 8),
+    # 16,777,216 points, 4 Gaussians (centres U[0.2,0.8]^3, sigma 0.02), seed 2, maxNumPts 4
+    "gauss4-16M": (dict(dist=2, seed=2, n=16777216, n_gauss=4, lo=0.2, hi=0.8, sig_lo=0.02), 4),
+    # 134,217,728 points, 16 Gaussians (sigma log-U[0.005,0.05], centres U[0.1,0.9]^3,
+    # Dirichlet(1) weights), seed 4, maxNumPts 4
+    "gauss16-128M": (dict(dist=2, seed=4, n=134217728, n_gauss=16, lo=0.1, hi=0.9, sig_lo=0.005, sig_hi=0.05,
+                          dirichlet=True), 4),
+}
 
 
-CONFIG_MAX_NUM_PTS = {1: 1, 2: 8, 3: 4, 5: 4}
+def workload_points(name, n=None):
+    """Point cloud of a named workload; n overrides the size (same stream, first n points)."""
+    a = dict(WORKLOADS[name][0])
+    if n is not None:
+        a["n"] = n
+    return gen_points(a.pop("dist"), a.pop("seed"), a.pop("n"), **a)
+
+
+def workload_max_num_pts(name):
+    return WORKLOADS[name][1]

@@ -197,6 +197,42 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
     return base + ex;
 }
 
+// Histogram add of one count per valid lane into h[v] (LDS or global).  The
+// few most common values of the wave are counted with one atomic each
+// (ballot match), the rest with per-lane atomics, so clustered data do not
+// serialise 64 lanes on one address.  Every lane of the wave must call it.
+__device__ inline void wave_hist_add(uint32_t *h, uint32_t v, bool valid) {
+    uint64_t act = __ballot(valid);
+    for (int it = 0; it < 4 && act; ++it) {
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const uint32_t vl = __shfl(v, leader, 64);
+        const uint64_t m = __ballot(valid && v == vl) & act;
+        if (__lane_id() == leader) atomicAdd(&h[vl], (uint32_t)__popcll(m));
+        act &= ~m;
+    }
+    if ((act >> __lane_id()) & 1ull) atomicAdd(&h[v], 1u);
+}
+// Slot for this lane in bucket v: base counters cnt[] (LDS) are advanced by
+// one atomic per distinct value in the wave; lanes keep their order within a
+// bucket.  Every lane of the wave must call it.
+__device__ inline uint32_t wave_bucket_slot(uint32_t *cnt, uint32_t v, bool valid) {
+    const int lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint64_t act = __ballot(valid);
+    uint32_t slot = 0;
+    while (act) {
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const uint32_t vl = __shfl(v, leader, 64);
+        const uint64_t m = __ballot(valid && v == vl) & act;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[vl], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if ((m >> lane) & 1ull) slot = base + (uint32_t)__popcll(m & lt);
+        act &= ~m;
+    }
+    return slot;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side error plumbing and device buffers.
 // ---------------------------------------------------------------------------
@@ -261,6 +297,9 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // which every key has the same 8-bit digit are skipped.
 void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
 // Exclusive scan of n uint32 values; returns the total (synchronises the stream).
+// *p = v in stream order (a kernel, so it is safe under graph capture and
+// needs no host staging).
+void set_u32_async(uint32_t *p, uint32_t v, hipStream_t s);
 uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
 // Same, leaving the total in device memory (no host synchronisation).
 void scan_exclusive_u32_async(const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, hipStream_t s);

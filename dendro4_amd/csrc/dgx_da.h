@@ -22,11 +22,13 @@ struct DaDevice {
     // Block view for the matvec: elements in SFC-contiguous blocks of kElemBlock.
     //   bnode_off[b]..bnode_off[b+1]: the block's unique node ids bnode[] (bit 31: node
     //   also touched by another block), bdst[]: its partial slot when shared.
-    //   lconn[j*E + e]: local index of vertex j ^ c of element e (child number c) in
-    //   its block's list -- the "child frame", where the parent corner is vertex 0.
-    //   cmeta[e]: hanging mask in the child frame | original level << 8.
+    //   Inside a block the elements are reordered (no hanging vertex first); slot k
+    //   of block b holds lconn[(b*elem_block + k)*8 + j] = 8 x (local index of vertex
+    //   j ^ c in the block's list), c the child number -- the "child frame", where
+    //   the parent corner is vertex 0 -- and cmeta[b*elem_block + k] = hanging mask
+    //   in the child frame | original level << 8.
     //   shared nodes: snode[s], partial slots soff[s]..soff[s+1] (ordered by block).
-    uint32_t elem_block = 512;  // elements per block (DGX_MATVEC_EB overrides at build time)
+    uint32_t elem_block = 512;  // elements per block (8 x elem_block byte offsets must fit 16 bits)
     uint32_t n_blocks = 0, max_block_nodes = 0, n_shared = 0;
     uint64_t n_pairs = 0, partial_slots = 0;
     DevBuf<uint32_t> bnode_off, bnode, bdst, soff, snode;

@@ -405,8 +405,71 @@ __global__ void k_blk_offsets(const uint32_t *bstart, const uint32_t *pos, uint3
 }
 // Local index of every element vertex inside its block's sorted node list,
 // stored in the child frame (slot j ^ c for vertex j of child c).
+// Slot of rank r among n: ranks r, r+1, ..., r+7 (one sibling group in SFC
+// order) go to 8 different rows of a row-major 8-row layout, so the 64 lanes of
+// one instruction hold elements from 64 different sibling groups, which in the
+// child frame touch 64 different nodes at each vertex (no LDS address conflicts).
+__device__ __forceinline__ uint32_t spread8(uint32_t r, uint32_t n) {
+    const uint32_t row = r & 7u, col = r >> 3;
+    uint32_t base = 0;
+    for (uint32_t i = 0; i < row; ++i) base += (n - i + 7u) >> 3;  // length of row i
+    return base + col;
+}
+// Order of the elements inside each block for the matvec: elements without
+// hanging vertices first, then the hanging ones, so that most waves are
+// uniform; inside each class the SFC order is spread by spread8.
+// epos[e] = the element's slot.
+__global__ void __launch_bounds__(B) k_blk_order(const uint16_t *meta, size_t ne, uint32_t eb, uint32_t *epos) {
+    __shared__ uint32_t wf[B / 64], wh[B / 64];
+    __shared__ uint32_t total_flat, carry_f, carry_h;
+    const size_t e0 = (size_t)blockIdx.x * eb;
+    const uint32_t nb = (uint32_t)min((size_t)eb, ne - e0);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        total_flat = 0;
+        carry_f = 0;
+        carry_h = 0;
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t k = threadIdx.x; k < nb; k += B) c += (meta[e0 + k] & 255u) == 0 ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) atomicAdd(&total_flat, c);
+    __syncthreads();
+    const uint32_t nflat = total_flat;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t k0 = 0; k0 < nb; k0 += B) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool valid = k < nb;
+        const bool hang = valid && (meta[e0 + k] & 255u) != 0;
+        const bool flat = valid && !hang;
+        const uint64_t bf = __ballot(flat), bh = __ballot(hang);
+        if (lane == 0) {
+            wf[wid] = (uint32_t)__popcll(bf);
+            wh[wid] = (uint32_t)__popcll(bh);
+        }
+        __syncthreads();
+        uint32_t pf = carry_f, ph = carry_h;
+        for (int w = 0; w < wid; ++w) {
+            pf += wf[w];
+            ph += wh[w];
+        }
+        if (flat) epos[e0 + k] = spread8(pf + (uint32_t)__popcll(bf & lt), nflat);
+        if (hang) epos[e0 + k] = nflat + spread8(ph + (uint32_t)__popcll(bh & lt), nb - nflat);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int w = 0; w < B / 64; ++w) {
+                carry_f += wf[w];
+                carry_h += wh[w];
+            }
+        __syncthreads();
+    }
+}
+// lconn[(b*eb + k)*8 + j] (k = the element's slot in block b): byte offset
+// (8 x local index) of vertex j ^ c of the element (child number c) in the
+// block's node list; one element's 8 entries are one 16-byte load.
 __global__ void k_blk_lconn(const uint32_t *conn, const uint16_t *meta, size_t ne, uint32_t eb, const uint32_t *boff,
-                            const uint32_t *bnode, uint16_t *lconn) {
+                            const uint32_t *bnode, const uint32_t *epos, uint16_t *lconn) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 8 * ne) return;
     size_t e = t % ne, j = t / ne;
@@ -416,15 +479,15 @@ __global__ void k_blk_lconn(const uint32_t *conn, const uint16_t *meta, size_t n
         uint32_t mid = (lo + hi) >> 1;
         if (bnode[mid] < g) lo = mid + 1; else hi = mid;
     }
-    lconn[(j ^ c) * ne + e] = (uint16_t)(lo - boff[b]);
+    lconn[((size_t)b * eb + epos[e]) * 8 + (j ^ c)] = (uint16_t)((lo - boff[b]) * 8u);
 }
-__global__ void k_blk_cmeta(const uint16_t *meta, size_t ne, uint16_t *cmeta) {
+__global__ void k_blk_cmeta(const uint16_t *meta, size_t ne, uint32_t eb, const uint32_t *epos, uint16_t *cmeta) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     uint32_t mt = meta[e], mask = mt & 255u, c = (mt >> 8) & 7u, cm = 0;
     for (uint32_t j = 0; j < 8; ++j)
         if ((mask >> (j ^ c)) & 1u) cm |= 1u << j;
-    cmeta[e] = (uint16_t)(cm | ((mt >> 11) << 8));
+    cmeta[(e / eb) * eb + epos[e]] = (uint16_t)(cm | ((mt >> 11) << 8));
 }
 __global__ void k_blk_shared_flag(const uint32_t *bnode, const uint32_t *ncount, uint64_t np, uint32_t *flag) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,6 +503,8 @@ __global__ void k_blk_shared_keys(const uint32_t *bnode, const uint32_t *pair_bl
     sk[q] = Key{(uint64_t)bnode[p], (uint64_t)pair_block[p]};
     sv[q] = (uint32_t)p;
 }
+// Partial slots are node-major: the partials of one shared node are
+// contiguous, in block order, so the second pass reads them as one run.
 __global__ void k_blk_shared_slots(const Key *sk, const uint32_t *sv, uint64_t ns, uint32_t *bdst, uint32_t *bnode,
                                    uint32_t *shead) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -475,6 +540,10 @@ __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx
 void da_build_blocks(DaDevice &da, hipStream_t s) {
     const size_t E = da.n_elem, T = 8 * E;
     const uint32_t eb = da.elem_block;
+    if (eb == 0 || (size_t)eb * 8 * 8 > 65535) {
+        dgx_set_error("DA: element block of %u elements is out of range", eb);
+        throw DgxError();
+    }
     const uint32_t nblk = (uint32_t)((E + eb - 1) / eb);
     DevBuf<Key> pk(T, s);
     k_blk_pairs<<<grid_for(T, B), B, 0, s>>>(da.conn.p, E, eb, pk.p);
@@ -492,10 +561,16 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
     head.release();
     pos.release();
     bstart.release();
-    da.lconn.alloc(T, s);
-    k_blk_lconn<<<grid_for(T, B), B, 0, s>>>(da.conn.p, da.meta.p, E, eb, da.bnode_off.p, da.bnode.p, da.lconn.p);
-    da.cmeta.alloc(E, s);
-    k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, da.cmeta.p);
+    {
+        DevBuf<uint32_t> epos(E, s);
+        k_blk_order<<<nblk, B, 0, s>>>(da.meta.p, E, eb, epos.p);
+        da.lconn.alloc((size_t)nblk * 8 * eb, s);
+        k_blk_lconn<<<grid_for(T, B), B, 0, s>>>(da.conn.p, da.meta.p, E, eb, da.bnode_off.p, da.bnode.p, epos.p,
+                                                da.lconn.p);
+        da.cmeta.alloc(E, s);
+        k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
+        HIP_OK(hipGetLastError());
+    }
     // shared nodes and their node-major partial slots
     DevBuf<uint32_t> flag(np, s), fpos(np, s);
     k_blk_shared_flag<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, ncount.p, np, flag.p);
@@ -512,8 +587,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
         da.snode.alloc(nsn, s);
         da.soff.alloc(nsn + 1, s);
         k_blk_shared_nodes<<<grid_for(ns, B), B, 0, s>>>(sk.p, shead.p, spos.p, ns, da.snode.p, da.soff.p);
-        uint32_t tail = (uint32_t)ns;
-        HIP_OK(hipMemcpyAsync(da.soff.p + nsn, &tail, 4, hipMemcpyHostToDevice, s));
+        set_u32_async(da.soff.p + nsn, (uint32_t)ns, s);
         da.n_shared = (uint32_t)nsn;
     }
     k_blk_mark_bdy<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, np, da.bdy_node.p);

@@ -16,6 +16,10 @@
 //   K u = (k(x) m(y) m(z) + m(x) k(y) m(z) + m(x) m(y) k(z)) u,  M u = m(x) m(y) m(z) u.
 #include "dgx_da.h"
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 namespace {
 
 constexpr int B = 256;
@@ -146,59 +150,272 @@ __device__ __forceinline__ void element_child_frame(uint32_t cm, T lap_fac, T ma
     r[7] = v[7];
 }
 
-// Pass 1: one workgroup per element block.  Element data are prefetched, the
-// block's nodes are gathered into LDS, element contributions are accumulated
-// there in FP64 (ds_add_f64 for both precisions), and each node sum goes to y
-// (node touched by this block only) or to its partial slot (shared node).
-// Node entries: bits 0-29 node id, bit 30 Dirichlet boundary, bit 31 shared.
-// With DIR, boundary columns are dropped and boundary rows are the identity.
+// 3-D Haar butterfly on the 8 vertex values (x fastest): after it, entry i
+// holds the sum/difference combination with a difference along axis k iff bit
+// k of i is set.  The 1-D Q1 operators are diagonal in that basis,
+//   mhat = [[2,1],[1,2]] = H diag(3,1) H / 2,  khat = [[1,-1],[-1,1]] = H diag(0,2) H / 2,
+// with H = [[1,1],[1,-1]], so g1 (khat mhat mhat + mhat khat mhat + mhat mhat khat)
+// + g2 mhat mhat mhat = H3 diag(lam) H3 / 8 where lam depends only on the number
+// nd of difference axes: nd=0: 27 g2, 1: 18 g1 + 9 g2, 2: 12 g1 + 3 g2, 3: 6 g1 + g2.
+template <typename T>
+__device__ __forceinline__ void haar3(T *u) {
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i & s) continue;
+            const T a = u[i], b = u[i + s];
+            u[i] = a + b;
+            u[i + s] = a - b;
+        }
+    }
+}
+
 constexpr int PER = 2;  // elements per thread (element block = PER * B)
 constexpr uint32_t ID_MASK = 0x3fffffffu, BDY_BIT = 0x40000000u, SHARED_BIT = 0x80000000u;
-template <typename T, bool DIR>
+
+// Haar eigenvalues / 8 per level (h = 2^-level), lam[level][nd].
+template <typename T>
+__device__ __forceinline__ void init_lambda(T (*lam)[4], double lap_fac, double mass_fac) {
+    if (threadIdx.x < 32) {
+        const double h = ldexp(1.0, -(int)threadIdx.x);
+        const double g1 = lap_fac * h * (1.0 / 36.0), g2 = mass_fac * h * h * h * (1.0 / 216.0);
+        lam[threadIdx.x][0] = (T)(27.0 * g2 * 0.125);
+        lam[threadIdx.x][1] = (T)((18.0 * g1 + 9.0 * g2) * 0.125);
+        lam[threadIdx.x][2] = (T)((12.0 * g1 + 3.0 * g2) * 0.125);
+        lam[threadIdx.x][3] = (T)((6.0 * g1 + g2) * 0.125);
+    }
+}
+
+// One element: gather its 8 vertex values from the LDS x slots (byte offsets
+// off[j], child frame), apply r = C^T (g1 K + g2 M) C u and add r into the LDS
+// FP64 accumulators.  Every lane of the wave calls it (the hanging test is a
+// wave vote); lanes with valid == false do nothing.
+template <typename T>
+__device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t cm, bool valid,
+                                                   const unsigned char *xb, unsigned char *yb, const T (*lam)[4]) {
+    const uint32_t mask = cm & 255u, lev = cm >> 8;
+    const bool wave_hanging = __ballot(valid && mask != 0u) != 0ull;
+    if (!valid) return;
+    T u[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) u[j] = *reinterpret_cast<const T *>(xb + off[j]);
+    bool h1 = false, h2 = false, h4 = false, h3 = false, h5 = false, h6 = false;
+    if (wave_hanging) {
+        // hanging vertices in the child frame: edge midpoints 1,2,4 (mean of
+        // vertex 0 and the edge end), face centres 3,5,6 (mean of 0, the face
+        // vertex and its two edge vertices)
+        h1 = mask & 2u;
+        h2 = mask & 4u;
+        h4 = mask & 16u;
+        h3 = mask & 8u;
+        h5 = mask & 32u;
+        h6 = mask & 64u;
+        const T x0 = u[0], x1 = u[1], x2 = u[2], x4 = u[4];
+        if (h1) u[1] = T(0.5) * (x0 + x1);
+        if (h2) u[2] = T(0.5) * (x0 + x2);
+        if (h4) u[4] = T(0.5) * (x0 + x4);
+        if (h3) u[3] = T(0.25) * ((x0 + u[3]) + (x1 + x2));
+        if (h5) u[5] = T(0.25) * ((x0 + u[5]) + (x1 + x4));
+        if (h6) u[6] = T(0.25) * ((x0 + u[6]) + (x2 + x4));
+    }
+    haar3<T>(u);
+    const T l0 = lam[lev][0], l1 = lam[lev][1], l2 = lam[lev][2], l3 = lam[lev][3];
+    u[0] *= l0;
+    u[1] *= l1;
+    u[2] *= l1;
+    u[4] *= l1;
+    u[3] *= l2;
+    u[5] *= l2;
+    u[6] *= l2;
+    u[7] *= l3;
+    haar3<T>(u);
+    if (wave_hanging) {
+        const T q3 = h3 ? T(0.25) * u[3] : T(0), q5 = h5 ? T(0.25) * u[5] : T(0), q6 = h6 ? T(0.25) * u[6] : T(0);
+        const T e1 = h1 ? T(0.5) * u[1] : T(0), e2 = h2 ? T(0.5) * u[2] : T(0), e4 = h4 ? T(0.5) * u[4] : T(0);
+        u[0] = u[0] + (e1 + e2 + e4) + (q3 + q5 + q6);
+        u[1] = (h1 ? e1 : u[1]) + q3 + q5;
+        u[2] = (h2 ? e2 : u[2]) + q3 + q6;
+        u[4] = (h4 ? e4 : u[4]) + q5 + q6;
+        if (h3) u[3] = q3;
+        if (h5) u[5] = q5;
+        if (h6) u[6] = q6;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) atomicAdd(reinterpret_cast<double *>(yb + off[j]), (double)u[j]);
+}
+
+__device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, uint32_t *off) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(lconn + e * 8);
+    off[0] = v.x & 0xffffu;
+    off[1] = v.x >> 16;
+    off[2] = v.y & 0xffffu;
+    off[3] = v.y >> 16;
+    off[4] = v.z & 0xffffu;
+    off[5] = v.z >> 16;
+    off[6] = v.w & 0xffffu;
+    off[7] = v.w >> 16;
+}
+
+// Pass 1, one workgroup per element block (any block size up to MAXN nodes).
+// Element data are loaded first, the block's nodes are gathered into LDS,
+// element contributions are accumulated there in FP64 (ds_add_f64 for both
+// precisions), and each node sum goes to y (node touched by this block only)
+// or to its partial slot (shared node).  Node entries: bits 0-29 node id,
+// bit 30 Dirichlet boundary, bit 31 shared.  With DIR, boundary columns are
+// dropped and boundary rows are the identity.  LDS: x slots (8 bytes each)
+// then the accumulators, both static, so the per-vertex byte offsets read from
+// lconn address them directly.  Elements without hanging vertices come first
+// in each block, so most waves skip the hanging interpolation.
+template <typename T, bool DIR, int MAXN>
 __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ lconn, const uint16_t *__restrict__ cmeta,
-                                                  size_t ne, uint32_t eb, const uint32_t *__restrict__ boff,
+                                                  size_t ne, const uint32_t *__restrict__ boff,
                                                   const uint32_t *__restrict__ bnode, const uint32_t *__restrict__ bdst,
-                                                  uint32_t maxn, const T *__restrict__ x, T *__restrict__ y,
-                                                  T *__restrict__ partial, T lap_fac, T mass_fac) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double *ys = reinterpret_cast<double *>(smem);
-    T *xs = reinterpret_cast<T *>(ys + maxn);
+                                                  const T *__restrict__ x, T *__restrict__ y, T *__restrict__ partial,
+                                                  double lap_fac, double mass_fac) {
+    constexpr int EB = PER * B;
+    __shared__ __align__(16) double xs[MAXN];  // x of the block's nodes (T in the low bytes)
+    __shared__ __align__(16) double ys[MAXN];  // FP64 accumulators
+    __shared__ T lam[32][4];
+    unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
+    unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
     const uint32_t b = blockIdx.x, n0 = boff[b], nb = boff[b + 1] - n0;
-    const size_t e0 = (size_t)b * eb, e1 = min(ne, e0 + eb);
-    uint16_t li[PER][8], mt[PER];
+    const size_t e0 = (size_t)b * EB;
+    uint32_t off[PER][8], mt[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const size_t e = e0 + threadIdx.x + (size_t)k * B;
-        if (e < e1) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) li[k][j] = lconn[(size_t)j * ne + e];
+        const size_t e = e0 + threadIdx.x + k * B;
+        mt[k] = 0;
+        if (e < ne) {
+            unpack_offsets(lconn, e, off[k]);
             mt[k] = cmeta[e];
         }
     }
+    init_lambda<T>(lam, lap_fac, mass_fac);
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
-        xs[l] = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
+        *reinterpret_cast<T *>(xb + 8 * l) = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
         ys[l] = 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const size_t e = e0 + threadIdx.x + (size_t)k * B;
-        if (e < e1) {
-            T xv[8], r[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) xv[j] = xs[li[k][j]];
-            element_child_frame<T>(mt[k], lap_fac, mass_fac, xv, r);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) atomicAdd(&ys[li[k][j]], (double)r[j]);
-        }
-    }
+    for (int k = 0; k < PER; ++k)
+        accumulate_element<T>(off[k], mt[k], e0 + threadIdx.x + k * B < ne, xb, yb, lam);
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
         if (g & SHARED_BIT) partial[bdst[n0 + l]] = (T)ys[l];
         else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : (T)ys[l];
     }
+}
+
+// Pass 1, persistent and software-pipelined, for blocks of at most NPT * B
+// nodes.  The grid is a multiple of 8 and workgroup w runs on XCD w % 8; each
+// XCD walks a contiguous range of element blocks, so neighbouring blocks
+// (which share nodes and x cache lines) meet in the same L2.  While block i is
+// computed, the element data, node lists and partial slots of block i+1 are
+// already in flight, and its x gather is issued before block i's results are
+// written out.
+template <typename T, bool DIR, int NPT>
+__global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ lconn,
+                                                   const uint16_t *__restrict__ cmeta, size_t ne, uint32_t nblk,
+                                                   const uint32_t *__restrict__ boff,
+                                                   const uint32_t *__restrict__ bnode,
+                                                   const uint32_t *__restrict__ bdst, const T *__restrict__ x,
+                                                   T *__restrict__ y, T *__restrict__ partial, double lap_fac,
+                                                   double mass_fac) {
+    constexpr int EB = PER * B, MAXN = NPT * B;
+    __shared__ __align__(16) double xs[MAXN];
+    __shared__ __align__(16) double ys[MAXN];
+    __shared__ T lam[32][4];
+    unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
+    unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
+    const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
+    const uint32_t lo = (uint32_t)((uint64_t)xcd * nblk / 8), hi = (uint32_t)((uint64_t)(xcd + 1) * nblk / 8);
+    uint32_t b = lo + (blockIdx.x >> 3);
+    if (b >= hi) return;  // uniform over the workgroup
+    init_lambda<T>(lam, lap_fac, mass_fac);
+
+    uint32_t off[PER][8], mt[PER], gv[NPT], sd[NPT];
+    T xv[NPT];
+    uint32_t n0, nb;
+    auto load_block = [&](uint32_t bb, uint32_t (*o)[8], uint32_t *m, uint32_t *g, uint32_t *d, uint32_t &s0,
+                          uint32_t &sn) {
+        const size_t e0 = (size_t)bb * EB;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const size_t e = e0 + threadIdx.x + k * B;
+            m[k] = 0;
+            if (e < ne) {
+                unpack_offsets(lconn, e, o[k]);
+                m[k] = cmeta[e];
+            }
+        }
+        s0 = boff[bb];
+        sn = boff[bb + 1] - s0;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const uint32_t l = threadIdx.x + i * B;
+            g[i] = l < sn ? bnode[s0 + l] : 0u;
+            d[i] = (l < sn && (g[i] & SHARED_BIT)) ? bdst[s0 + l] : 0u;
+        }
+    };
+    auto gather_x = [&](const uint32_t *g, uint32_t sn, T *v) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const uint32_t l = threadIdx.x + i * B;
+            v[i] = (l < sn && !(DIR && (g[i] & BDY_BIT))) ? x[g[i] & ID_MASK] : T(0);
+        }
+    };
+    load_block(b, off, mt, gv, sd, n0, nb);
+    gather_x(gv, nb, xv);
+    for (;;) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const uint32_t l = threadIdx.x + i * B;
+            if (l < nb) {
+                *reinterpret_cast<T *>(xb + 8 * l) = xv[i];
+                ys[l] = 0.0;
+            }
+        }
+        __syncthreads();
+        const uint32_t bn = b + per_xcd;
+        const bool more = bn < hi;
+        uint32_t off2[PER][8], mt2[PER], gv2[NPT], sd2[NPT], n02 = 0, nb2 = 0;
+        if (more) load_block(bn, off2, mt2, gv2, sd2, n02, nb2);
+        const size_t e0 = (size_t)b * EB;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            accumulate_element<T>(off[k], mt[k], e0 + threadIdx.x + k * B < ne, xb, yb, lam);
+        if (more) gather_x(gv2, nb2, xv);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const uint32_t l = threadIdx.x + i * B;
+            if (l < nb) {
+                const uint32_t g = gv[i];
+                if (g & SHARED_BIT) partial[sd[i]] = (T)ys[l];
+                else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : (T)ys[l];
+            }
+        }
+        if (!more) break;
+        __syncthreads();  // the LDS of block b is reused
+        b = bn;
+        n0 = n02;
+        nb = nb2;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            mt[k] = mt2[k];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) off[k][j] = off2[k][j];
+        }
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            gv[i] = gv2[i];
+            sd[i] = sd2[i];
+        }
+    }
+    (void)n0;
 }
 
 // Pass 2: shared nodes sum their partials in block order.
@@ -219,6 +436,33 @@ __global__ void __launch_bounds__(B) k_matvec_shared(const uint32_t *__restrict_
     y[g & ID_MASK] = acc;
 }
 
+// Persistent grid: resident workgroups per CU x CUs, a multiple of 8 (one
+// share per XCD), capped so that each workgroup gets at least one block.
+unsigned pipe_grid(const void *kernel, uint32_t nblk) {
+    static thread_local int cached_dev = -1, cus = 0;
+    static thread_local std::vector<std::pair<const void *, int>> occ;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev != cached_dev) {
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        occ.clear();
+        cached_dev = dev;
+    }
+    int per_cu = 0;
+    for (auto &kv : occ)
+        if (kv.first == kernel) per_cu = kv.second;
+    if (!per_cu) {
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, B, 0));
+        if (per_cu < 1) per_cu = 1;
+        occ.emplace_back(kernel, per_cu);
+    }
+    uint64_t g = (uint64_t)per_cu * (uint64_t)cus;
+    const uint64_t need = ((uint64_t)nblk + 7) / 8 * 8;
+    if (g > need) g = need;
+    g = (g + 7) / 8 * 8;
+    return (unsigned)std::max<uint64_t>(g, 8);
+}
+
 template <typename T, bool DIR>
 void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, hipStream_t s) {
     if (!da.n_elem) return;
@@ -226,11 +470,26 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
         dgx_set_error("matvec: DA built with element blocks of %u, kernel expects %d", da.elem_block, PER * B);
         throw DgxError();
     }
-    const size_t lds = (size_t)da.max_block_nodes * (sizeof(double) + sizeof(T));
     T *partial = reinterpret_cast<T *>(da.partial.p);
-    k_matvec_blk<T, DIR><<<da.n_blocks, B, lds, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.elem_block,
-                                                     da.bnode_off.p, da.bnode.p, da.bdst.p, da.max_block_nodes, x,
-                                                     y, partial, lap_fac, mass_fac);
+    const uint32_t mn = da.max_block_nodes;
+    if (mn <= 4 * B) {
+        const unsigned grid = pipe_grid(reinterpret_cast<const void *>(&k_matvec_pipe<T, DIR, 4>), da.n_blocks);
+        k_matvec_pipe<T, DIR, 4><<<grid, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.n_blocks, da.bnode_off.p,
+                                                    da.bnode.p, da.bdst.p, x, y, partial, (double)lap_fac,
+                                                    (double)mass_fac);
+    } else {
+#define DGX_MV_LAUNCH(MAXN)                                                                                     \
+    k_matvec_blk<T, DIR, MAXN><<<da.n_blocks, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.bnode_off.p,     \
+                                                          da.bnode.p, da.bdst.p, x, y, partial, (double)lap_fac, \
+                                                          (double)mass_fac)
+        if (mn <= 2048) DGX_MV_LAUNCH(2048);
+        else if (mn <= 4096) DGX_MV_LAUNCH(4096);
+        else {
+            dgx_set_error("matvec: block with %u nodes exceeds 4096", mn);
+            throw DgxError();
+        }
+    }
+#undef DGX_MV_LAUNCH
     if (da.n_shared)
         k_matvec_shared<T, DIR>
             <<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared, partial, x, y);

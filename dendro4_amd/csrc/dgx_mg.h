@@ -8,14 +8,15 @@
 // leaves is replaced by its parent, once.
 DevBuf<Key> coarsen_build(const Key *in, size_t n, hipStream_t s);
 
-// Interpolation from a coarse level to the next finer one, per fine node:
-// up to 8 (coarse node, weight) pairs (ELL, col = 0xffffffff when unused),
-// and the transpose as CSR over coarse nodes for the restriction.
+// Interpolation from a coarse level to the next finer one as CSR over fine
+// nodes (at most 8 coarse nodes per row), and its transpose as CSR over
+// coarse nodes for the restriction.
 struct Transfer {
     size_t nf = 0, nc = 0, nnz = 0;
-    DevBuf<uint32_t> pcol;    // [8][nf]
-    DevBuf<double> pval;      // [8][nf]
-    DevBuf<float> pval32;     // [8][nf]
+    DevBuf<uint32_t> poff;    // [nf+1]
+    DevBuf<uint32_t> pcol;    // [nnz]
+    DevBuf<double> pval;      // [nnz]
+    DevBuf<float> pval32;     // [nnz]
     DevBuf<uint32_t> roff;    // [nc+1]
     DevBuf<uint32_t> rcol;    // [nnz] fine node ids, ascending per coarse row
     DevBuf<double> rval;      // [nnz]

@@ -211,27 +211,54 @@ __global__ void k_residual(const T *b, const T *ax, uint32_t n, T *r) {
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i < n) r[i] = b[i] - ax[i];
 }
+// bc = R rf: 8 lanes per coarse row, strided over the row's entries, then a
+// butterfly sum inside the 8-lane group.
 template <typename T>
-__global__ void k_restrict(const uint32_t *roff, const uint32_t *rcol, const T *rval, uint32_t nc, const T *rf,
-                           T *bc) {
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (i >= nc) return;
+__global__ void __launch_bounds__(B) k_restrict(const uint32_t *__restrict__ roff, const uint32_t *__restrict__ rcol,
+                                                const T *__restrict__ rval, uint32_t nc, const T *__restrict__ rf,
+                                                T *__restrict__ bc) {
+    const uint32_t t = blockIdx.x * B + threadIdx.x, i = t >> 3, sub = t & 7u;
     T acc = T(0);
-    for (uint32_t p = roff[i]; p < roff[i + 1]; ++p) acc += rval[p] * rf[rcol[p]];
-    bc[i] = acc;
+    if (i < nc) {
+        const uint32_t p1 = roff[i + 1];
+        for (uint32_t p = roff[i] + sub; p < p1; p += 8) acc += rval[p] * rf[rcol[p]];
+    }
+    acc += __shfl_xor(acc, 4, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    acc += __shfl_xor(acc, 1, 64);
+    if (i < nc && sub == 0) bc[i] = acc;
 }
+// xf += P xc, P in CSR over fine nodes
 template <typename T>
-__global__ void k_prolong_add(const uint32_t *pcol, const T *pval, uint32_t nf, const T *xc, T *xf) {
-    uint32_t n = blockIdx.x * B + threadIdx.x;
+__global__ void __launch_bounds__(B) k_prolong_add(const uint32_t *__restrict__ poff, const uint32_t *__restrict__ pcol,
+                                                   const T *__restrict__ pval, uint32_t nf, const T *__restrict__ xc,
+                                                   T *__restrict__ xf) {
+    const uint32_t n = blockIdx.x * B + threadIdx.x;
     if (n >= nf) return;
     T acc = T(0);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t c = pcol[(size_t)k * nf + n];
-        if (c == NONE) break;
-        acc += pval[(size_t)k * nf + n] * xc[c];
-    }
+    for (uint32_t p = poff[n], p1 = poff[n + 1]; p < p1; ++p) acc += pval[p] * xc[pcol[p]];
     xf[n] += acc;
+}
+__global__ void k_ell_count(const uint32_t *ecol, uint32_t nf, uint32_t *cnt) {
+    const uint32_t n = blockIdx.x * B + threadIdx.x;
+    if (n >= nf) return;
+    uint32_t c = 0;
+    while (c < 8 && ecol[(size_t)c * nf + n] != NONE) ++c;
+    cnt[n] = c;
+}
+__global__ void k_ell_to_csr(const uint32_t *ecol, const double *eval, uint32_t nf, const uint32_t *off,
+                             uint32_t *pcol, double *pval, float *pval32) {
+    const uint32_t n = blockIdx.x * B + threadIdx.x;
+    if (n >= nf) return;
+    uint32_t o = off[n];
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t c = ecol[(size_t)k * nf + n];
+        if (c == NONE) break;
+        pcol[o] = c;
+        pval[o] = eval[(size_t)k * nf + n];
+        pval32[o] = (float)eval[(size_t)k * nf + n];
+        ++o;
+    }
 }
 // dense y = A x (n x n row-major), one wave per row
 template <typename T>
@@ -327,21 +354,31 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
     const uint32_t nf = f.n_nodes;
     t.nf = nf;
     t.nc = c.n_nodes;
-    t.pcol.alloc(8 * (size_t)nf, s);
-    t.pval.alloc(8 * (size_t)nf, s);
+    DevBuf<uint32_t> ecol(8 * (size_t)nf, s);  // ELL [8][nf], NONE-terminated rows
+    DevBuf<double> eval(8 * (size_t)nf, s);
     DevBuf<uint32_t> err(1, s);
     HIP_OK(hipMemsetAsync(err.p, 0, 4, s));
     k_prolong_rows<<<grid_for(nf, B), B, 0, s>>>(nf, f.n2l.p, f.x.p, f.y.p, f.z.p, f.bdy_node.p, c.key.p, c.n_elem,
                                                 c.x.p, c.y.p, c.z.p, c.flags.p, c.conn.p, c.meta.p, c.bdy_node.p,
-                                                md0 + 1, make_packed(curve), t.pcol.p, t.pval.p, err.p);
+                                                md0 + 1, make_packed(curve), ecol.p, eval.p, err.p);
     HIP_OK(hipGetLastError());
-    t.pval32.alloc(8 * (size_t)nf, s);
-    k_to_f32<<<grid_for(8 * (size_t)nf, B), B, 0, s>>>(t.pval.p, 8 * (size_t)nf, t.pval32.p);
+    {
+        DevBuf<uint32_t> cnt(nf, s);
+        k_ell_count<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, cnt.p);
+        t.poff.alloc((size_t)nf + 1, s);
+        const uint64_t pn = scan_exclusive_u32(cnt.p, t.poff.p, nf, s);
+        set_u32_async(t.poff.p + nf, (uint32_t)pn, s);
+        t.pcol.alloc(pn, s);
+        t.pval.alloc(pn, s);
+        t.pval32.alloc(pn, s);
+        k_ell_to_csr<<<grid_for(nf, B), B, 0, s>>>(ecol.p, eval.p, nf, t.poff.p, t.pcol.p, t.pval.p, t.pval32.p);
+        HIP_OK(hipGetLastError());
+    }
     // transpose into CSR over coarse nodes
     DevBuf<Key> keys(8 * (size_t)nf, s);
     DevBuf<uint32_t> vals(8 * (size_t)nf, s), cnt(1, s);
     HIP_OK(hipMemsetAsync(cnt.p, 0, 4, s));
-    k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(t.pcol.p, nf, keys.p, vals.p, cnt.p);
+    k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, keys.p, vals.p, cnt.p);
     uint32_t hcnt = 0, herr = 0;
     HIP_OK(hipMemcpyAsync(&hcnt, cnt.p, 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
@@ -358,12 +395,11 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
     DevBuf<uint32_t> rowcnt(t.nc, s);
     HIP_OK(hipMemsetAsync(rowcnt.p, 0, t.nc * 4, s));
     if (hcnt)
-        k_restrict_csr<<<grid_for(hcnt, B), B, 0, s>>>(keys.p, vals.p, hcnt, t.pval.p, t.rcol.p, t.rval.p,
+        k_restrict_csr<<<grid_for(hcnt, B), B, 0, s>>>(keys.p, vals.p, hcnt, eval.p, t.rcol.p, t.rval.p,
                                                       t.rval32.p, rowcnt.p);
     t.roff.alloc(t.nc + 1, s);
     uint64_t tot = scan_exclusive_u32(rowcnt.p, t.roff.p, t.nc, s);
-    uint32_t tail = (uint32_t)tot;
-    HIP_OK(hipMemcpyAsync(t.roff.p + t.nc, &tail, 4, hipMemcpyHostToDevice, s));
+    set_u32_async(t.roff.p + t.nc, (uint32_t)tot, s);
     HIP_OK(hipStreamSynchronize(s));
 }
 
@@ -490,9 +526,9 @@ void vcycle(MgHierarchy &mg, Work<T> &w, size_t k, const SolveOpts &o, hipStream
     matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
     k_residual<T><<<g, B, 0, s>>>(b, y, n, y);
     const Transfer &t = mg.lev[k + 1].to_finer;
-    k_restrict<T><<<grid_for(t.nc, B), B, 0, s>>>(t.roff.p, t.rcol.p, rval_of<T>(t), (uint32_t)t.nc, y, w.b[k + 1].p);
+    k_restrict<T><<<grid_for((size_t)t.nc * 8, B), B, 0, s>>>(t.roff.p, t.rcol.p, rval_of<T>(t), (uint32_t)t.nc, y, w.b[k + 1].p);
     vcycle<T>(mg, w, k + 1, o, s);
-    k_prolong_add<T><<<g, B, 0, s>>>(t.pcol.p, pval_of<T>(t), n, w.x[k + 1].p, x);
+    k_prolong_add<T><<<g, B, 0, s>>>(t.poff.p, t.pcol.p, pval_of<T>(t), n, w.x[k + 1].p, x);
     for (int it = 0; it < o.post; ++it) {
         matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
         k_jacobi<T><<<g, B, 0, s>>>(b, y, dinv, om, n, x);
@@ -550,6 +586,8 @@ struct MgWork {
     Work<float> w32;
     DevBuf<double> x, bvec, r, z, p, q, rold, part, sc;
     Graph g_precond, g_step, g_dir;
+    // no replay may still be running when the executable graphs are destroyed
+    ~MgWork() { (void)hipDeviceSynchronize(); }
 };
 MgHierarchy::~MgHierarchy() { delete work; }
 

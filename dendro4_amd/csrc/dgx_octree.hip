@@ -297,26 +297,12 @@ __global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, co
 }
 
 // ------------------------------------------------------------ balance ----
-__global__ void k_minmax_level(const Key *k, size_t n, uint32_t *mm) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t lo = 31, hi = 0;
-    if (i < n) lo = hi = key_level(k[i]);
-    for (int o = 32; o > 0; o >>= 1) {
-        lo = min(lo, (uint32_t)__shfl_xor(lo, o, 64));
-        hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
-    }
-    if (__lane_id() == 0) {
-        atomicMin(&mm[0], lo);
-        atomicMax(&mm[1], hi);
-    }
-}
-
 __global__ void __launch_bounds__(B) k_level_hist(const Key *k, size_t n, uint32_t *hist) {
     __shared__ uint32_t h[32];
     if (threadIdx.x < 32) h[threadIdx.x] = 0;
     __syncthreads();
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    if (i < n) atomicAdd(&h[key_level(k[i])], 1u);
+    wave_hist_add(h, i < n ? key_level(k[i]) : 0u, i < n);
     __syncthreads();
     if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
@@ -330,11 +316,9 @@ __global__ void __launch_bounds__(B) k_bucket_parents(const Key *k, size_t n, ui
     if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
     __syncthreads();
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    uint32_t x = 0, y = 0, z = 0, l = 0, loc = 0;
-    if (i < n) {
-        dev_decode(s_mc, k[i], md, x, y, z, l);
-        loc = atomicAdd(&cnt[l], 1u);
-    }
+    uint32_t x = 0, y = 0, z = 0, l = 0;
+    if (i < n) dev_decode(s_mc, k[i], md, x, y, z, l);
+    const uint32_t loc = wave_bucket_slot(cnt, l, i < n);
     __syncthreads();
     if (threadIdx.x < 32) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]) : 0u;
     __syncthreads();
@@ -574,22 +558,22 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         return std::move(out);
     };
     if (n < 2) return copy_input();  // comboRipple: in.size() < 2 (balance.cpp:806-808)
-    DevBuf<uint32_t> mm(2, s);
-    uint32_t init[2] = {31u, 0u};
-    HIP_OK(hipMemcpyAsync(mm.p, init, 8, hipMemcpyHostToDevice, s));
-    k_minmax_level<<<grid_for(n, B), B, 0, s>>>(in, n, mm.p);
-    uint32_t hmm[2];
-    HIP_OK(hipMemcpyAsync(hmm, mm.p, 8, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    if (hmm[1] - hmm[0] < 2) return copy_input();  // TreeNode.cpp:929-947
-    SfcPacked tab = make_packed(curve);
-    // bucket the parents of the inputs by input level
+    // level histogram of the inputs (also gives the level range)
     DevBuf<uint32_t> lh(32, s), cursor(32, s);
     HIP_OK(hipMemsetAsync(lh.p, 0, 128, s));
     k_level_hist<<<grid_for(n, B), B, 0, s>>>(in, n, lh.p);
     uint32_t hl[32];
     HIP_OK(hipMemcpyAsync(hl, lh.p, 128, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    uint32_t lmin = 31, lmax = 0;
+    for (uint32_t l = 0; l < 32; ++l)
+        if (hl[l]) {
+            lmin = std::min(lmin, l);
+            lmax = std::max(lmax, l);
+        }
+    if (lmax - lmin < 2) return copy_input();  // TreeNode.cpp:929-947
+    SfcPacked tab = make_packed(curve);
+    // bucket the parents of the inputs by input level
     uint32_t loff[33];
     loff[0] = 0;
     for (int l = 0; l < 32; ++l) loff[l + 1] = loff[l] + hl[l];
@@ -597,7 +581,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     DevBuf<Key> inpar(n, s);
     k_bucket_parents<<<grid_for(n, B), B, 0, s>>>(in, n, md, tab, cursor.p, inpar.p);
     HIP_OK(hipGetLastError());
-    const uint32_t maxL = hmm[1];
+    const uint32_t maxL = lmax;
     DevBuf<Key> prevQ;        // split octants one level finer (sorted unique Morton)
     size_t nprev = 0;
     DevBuf<Key> nbpar;        // parents of neighbours generated for the current level

@@ -111,12 +111,15 @@ __global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, u
     for (int i = threadIdx.x; i < 16 * 256; i += SB) h[i] = 0;
     __syncthreads();
     size_t stride = (size_t)gridDim.x * SB;
-    for (size_t i = (size_t)blockIdx.x * SB + threadIdx.x; i < n; i += stride) {
-        Key k = keys[i];
+    const size_t nround = (n + stride - 1) / stride * stride;  // whole waves stay in the loop
+    for (size_t i = (size_t)blockIdx.x * SB + threadIdx.x; i < nround; i += stride) {
+        const bool valid = i < n;
+        Key k{0ull, 0ull};
+        if (valid) k = keys[i];
 #pragma unroll
         for (int p = 0; p < 8; ++p) {
-            atomicAdd(&h[p * 256 + ((k.lo >> (8 * p)) & 255u)], 1u);
-            atomicAdd(&h[(p + 8) * 256 + ((k.hi >> (8 * p)) & 255u)], 1u);
+            wave_hist_add(&h[p * 256], (uint32_t)((k.lo >> (8 * p)) & 255u), valid);
+            wave_hist_add(&h[(p + 8) * 256], (uint32_t)((k.hi >> (8 * p)) & 255u), valid);
         }
     }
     __syncthreads();
@@ -133,7 +136,8 @@ __global__ void __launch_bounds__(SB) k_block_hist(const Key *keys, size_t n, in
 #pragma unroll
     for (int i = 0; i < SITEMS; ++i) {
         size_t idx = base + (size_t)i * SB + threadIdx.x;
-        if (idx < n) atomicAdd(&h[digit_of(keys[idx], bit)], 1u);
+        const bool valid = idx < n;
+        wave_hist_add(h, valid ? digit_of(keys[idx], bit) : 0u, valid);
     }
     __syncthreads();
     bhist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
@@ -182,7 +186,16 @@ __global__ void __launch_bounds__(SB) k_scatter(const Key *kin, const uint32_t *
     }
 }
 
+__global__ void k_set_value_u32(uint32_t *p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 }  // namespace
+
+void set_u32_async(uint32_t *p, uint32_t v, hipStream_t s) {
+    k_set_value_u32<<<1, 64, 0, s>>>(p, v);
+    HIP_OK(hipGetLastError());
+}
 
 void scan_exclusive_u32_async(const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total, hipStream_t s) {
     if (n == 0) {

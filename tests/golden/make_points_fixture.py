@@ -1,5 +1,5 @@
 """Writes tests/golden/points_sha256.json: SHA-256 of the first points of each
-benchmark configuration from dgx_gen_points, so that drift of the generator
+benchmark workload from dgx_gen_points, so that drift of the generator
 (or of libm) between machines is detected."""
 import hashlib
 import json
@@ -15,7 +15,7 @@ N = 100000
 
 
 def digests():
-    return {str(c): hashlib.sha256(dgx.config_points(c, N).tobytes()).hexdigest() for c in (1, 2, 3, 5)}
+    return {w: hashlib.sha256(dgx.workload_points(w, N).tobytes()).hexdigest() for w in dgx.WORKLOADS}
 
 
 if __name__ == "__main__":

@@ -129,7 +129,7 @@ def test_saturated_cells_random(dgx, oracle):
 def test_lognormal_corner_cluster(dgx, oracle, stencils):
     """Config-2 distribution: ~2% of coordinates clip to 1-eps, stacking points
     in the last cell at maxDepth; maxNumPts=8."""
-    pts = dgx.config_points(2, 400000)
+    pts = dgx.workload_points("lognormal-4M", 400000)
     assert (pts == np.nextafter(1.0, 0)).all(1).sum() > 0
     ou, ob, da = check_pipeline(dgx, oracle, pts, 30, 8, *stencils)
     print("stats", ou.stats(), flush=True)
@@ -144,7 +144,7 @@ def test_sample_sort(dgx, oracle):
 
 def test_config1_full_parity(dgx, oracle, stencils):
     """Config 1 at its full size: 262,144 uniform points, maxNumPts=1."""
-    pts = dgx.config_points(1)
+    pts = dgx.workload_points("uniform-256K")
     check_pipeline(dgx, oracle, pts, 30, 1, *stencils)
 
 
@@ -152,7 +152,7 @@ def test_config2_full_size_properties(dgx, oracle):
     """Config 2 at its full size (4,194,304 log-normal points, maxNumPts=8):
     octree and balanced octree bit-exact against the oracle, and the operator's
     conservation properties hold."""
-    pts = dgx.config_points(2)
+    pts = dgx.workload_points("lognormal-4M")
     ou = dgx.points2octree(pts, 30, 8)
     tu = ou.treenodes()
     ru = oracle.points2octree(pts, 30, 8)

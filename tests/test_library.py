@@ -41,12 +41,12 @@ def test_generator_is_deterministic_and_matches_fixture():
     fx = json.load(open(os.path.join(GOLDEN, "points_sha256.json")))
     n = fx["n"]
     for c, h in fx["sha256"].items():
-        pts = dgx.config_points(int(c), n)
+        pts = dgx.workload_points(c, n)
         assert hashlib.sha256(pts.tobytes()).hexdigest() == h, c
         assert (pts >= 0).all() and (pts < 1).all()
-    u = dgx.config_points(1, 200000)
+    u = dgx.workload_points("uniform-256K", 200000)
     assert abs(u.mean() - 0.5) < 0.01
-    ln = dgx.config_points(2, 200000)
+    ln = dgx.workload_points("lognormal-4M", 200000)
     frac_clipped = (ln == np.nextafter(1.0, 0)).mean()
     assert 0.015 < frac_clipped < 0.03  # P(exp(N(-1,.5)) >= 1) = P(N >= 2 sigma) = 0.0228
 
