@@ -33,7 +33,8 @@ EXPORTED = [
     "dgx_octree_import", "dgx_octree_size", "dgx_octree_max_depth", "dgx_octree_export", "dgx_octree_stats",
     "dgx_octree_free", "dgx_da_create", "dgx_da_get_info", "dgx_da_export", "dgx_da_matvec", "dgx_da_matvec_f32",
     "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points", "dgx_da_matvec_dirichlet", "dgx_mg_create", "dgx_mg_info",
-    "dgx_mg_finest_da", "dgx_mg_rhs", "dgx_mg_solve", "dgx_mg_free",
+    "dgx_mg_finest_da", "dgx_mg_rhs", "dgx_mg_solve", "dgx_mg_free", "dgx_points_to_octree_glens",
+    "dgx_sfc_tables", "dgx_da_export_anchors", "dgx_mg_set_operator",
 ]
 
 
@@ -91,6 +92,7 @@ def load_library():
         "dgx_memcpy_dtoh": ([vp, vp, sz], i32),
         "dgx_memset_device": ([vp, i32, sz], i32),
         "dgx_points_to_octree": ([vp, sz, u32, u32, i32, i32, C.POINTER(vp)], i32),
+        "dgx_points_to_octree_glens": ([vp, sz, vp, u32, u32, i32, i32, C.POINTER(vp)], i32),
         "dgx_balance_octree": ([vp, i32, C.POINTER(vp)], i32),
         "dgx_octree_import": ([vp, sz, u32, i32, i32, i32, C.POINTER(vp)], i32),
         "dgx_octree_size": ([vp, C.POINTER(sz)], i32),
@@ -238,16 +240,19 @@ class Octree:
             pass
 
 
-def points2octree(pts, max_depth=30, max_num_pts=1, curve=CURVE_HILBERT):
-    """ot::points2Octree on one rank.  pts: (n,3) float64 host array or a DeviceArray."""
+def points2octree(pts, max_depth=30, max_num_pts=1, curve=CURVE_HILBERT, glens=None):
+    """ot::points2Octree on one rank.  pts: (n,3) float64 host array or a
+    DeviceArray; glens: domain lengths (default: the unit cube)."""
     L = _lib_ready()
     h = C.c_void_p()
+    g = None if glens is None else np.ascontiguousarray(glens, np.float64).reshape(3)
+    gp = None if g is None else _ptr(g)
     if isinstance(pts, DeviceArray):
         n = pts.shape[0]
-        _check(L.dgx_points_to_octree(pts.ptr, n, max_depth, max_num_pts, curve, 1, C.byref(h)))
+        _check(L.dgx_points_to_octree_glens(pts.ptr, n, gp, max_depth, max_num_pts, curve, 1, C.byref(h)))
     else:
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
-        _check(L.dgx_points_to_octree(_ptr(pts), len(pts), max_depth, max_num_pts, curve, 0, C.byref(h)))
+        _check(L.dgx_points_to_octree_glens(_ptr(pts), len(pts), gp, max_depth, max_num_pts, curve, 0, C.byref(h)))
     return Octree(h)
 
 

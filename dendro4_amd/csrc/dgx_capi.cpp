@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <vector>
 #include <string>
 
 #include "../../include/dendro_mi355x.h"
@@ -157,8 +158,8 @@ int dgx_memset_device(void *dst, int value, size_t bytes) {
     API_END
 }
 
-int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts, int curve,
-                         int on_device, dgx_octree *out) {
+int dgx_points_to_octree_glens(const double *xyz, size_t n, const double *glens, unsigned max_depth,
+                               unsigned max_num_pts, int curve, int on_device, dgx_octree *out) {
     API_BEGIN
     ensure_ctx();
     if (!out || (n && !xyz)) { dgx_set_error("points2Octree: null argument"); return -1; }
@@ -166,6 +167,10 @@ int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsign
     if (max_num_pts == 0) { dgx_set_error("points2Octree: maxNumPts must be >= 1"); return -1; }
     if (curve != DGX_CURVE_HILBERT && curve != DGX_CURVE_MORTON) { dgx_set_error("unknown curve %d", curve); return -1; }
     if (n > 0xffffffffull) { dgx_set_error("points2Octree: %zu points exceed 2^32-1", n); return -1; }
+    if (glens && !(glens[0] > 0.0 && glens[1] > 0.0 && glens[2] > 0.0)) {
+        dgx_set_error("points2Octree: domain lengths must be positive");
+        return -1;
+    }
     std::unique_ptr<dgx_octree_s> o(new dgx_octree_s());
     o->md = max_depth;
     o->curve = curve;
@@ -177,7 +182,7 @@ int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsign
         d = tmp.p;
     }
     P2OStats st;
-    o->keys = p2o_build(d, n, max_depth, max_num_pts, curve, S(), &st);
+    o->keys = p2o_build(d, n, max_depth, max_num_pts, curve, S(), &st, glens);
     o->n = o->keys.n;
     o->stats[0] = st.textbook_leaves;
     o->stats[1] = st.triggers;
@@ -185,6 +190,11 @@ int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsign
     o->stats[3] = st.replayed_leaves;
     *out = o.release();
     API_END
+}
+
+int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts, int curve,
+                         int on_device, dgx_octree *out) {
+    return dgx_points_to_octree_glens(xyz, n, nullptr, max_depth, max_num_pts, curve, on_device, out);
 }
 
 int dgx_balance_octree(dgx_octree in, int inc_corner, dgx_octree *out) {
@@ -311,6 +321,21 @@ int dgx_da_export(dgx_da d, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels
     API_END
 }
 
+int dgx_da_export_anchors(dgx_da d, uint32_t *xyz) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !xyz) { dgx_set_error("null argument"); return -1; }
+    const DaDevice &a = d->get();
+    std::vector<uint32_t> t(a.n_local);
+    const uint32_t *src[3] = {a.x.p, a.y.p, a.z.p};
+    for (int c = 0; c < 3; ++c) {
+        if (a.n_local) HIP_OK(hipMemcpyAsync(t.data(), src[c], a.n_local * 4, hipMemcpyDeviceToHost, S()));
+        HIP_OK(hipStreamSynchronize(S()));
+        for (size_t i = 0; i < a.n_local; ++i) xyz[3 * i + c] = t[i];
+    }
+    API_END
+}
+
 int dgx_da_matvec(dgx_da d, const double *x, double *y, double lap_fac, double mass_fac) {
     API_BEGIN
     ensure_ctx();
@@ -349,6 +374,13 @@ int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fa
     m->mg = mg_build(finest->keys.p, finest->n, finest->md, finest->curve, nlevels, lap_fac, mass_fac, S());
     m->finest.borrowed = m->mg->lev[0].da.get();
     *out = m.release();
+    API_END
+}
+int dgx_mg_set_operator(dgx_mg m, double lap_fac, double mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!m) { dgx_set_error("DAMG: null argument"); return -1; }
+    mg_set_operator(*m->mg, lap_fac, mass_fac, S());
     API_END
 }
 int dgx_mg_info(dgx_mg m, int *nlevels, uint64_t *octants, uint64_t *nodes) {

@@ -45,6 +45,9 @@ struct MgHierarchy {
     ~MgHierarchy();
 };
 
+// Changes the operator factors: recomputes the level diagonals and the
+// coarse inverse, drops captured graphs.
+void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s);
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
                                       double lap_fac, double mass_fac, hipStream_t s);
 

@@ -472,6 +472,15 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     return mg;
 }
 
+void mg_reset_graphs(MgHierarchy &mg);
+void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s) {
+    mg.lap_fac = lap_fac;
+    mg.mass_fac = mass_fac;
+    for (auto &L : mg.lev) build_level_diag(L, lap_fac, mass_fac, s);
+    build_coarse_inverse(mg, s);
+    mg_reset_graphs(mg);
+}
+
 // ---------------------------------------------------------------- solve ----
 namespace {
 
@@ -590,6 +599,13 @@ struct MgWork {
     ~MgWork() { (void)hipDeviceSynchronize(); }
 };
 MgHierarchy::~MgHierarchy() { delete work; }
+void mg_reset_graphs(MgHierarchy &mg) {
+    if (!mg.work) return;
+    HIP_OK(hipDeviceSynchronize());
+    mg.work->g_precond.reset();
+    mg.work->g_step.reset();
+    mg.work->g_dir.reset();
+}
 
 namespace {
 

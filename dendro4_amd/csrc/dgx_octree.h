@@ -54,7 +54,7 @@ struct BalStats {
 
 // Sorted leaf keys of the octree built from n points (xyz interleaved, device).
 DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t max_num_pts, int curve, hipStream_t s,
-                      P2OStats *stats);
+                      P2OStats *stats, const double *glens = nullptr);
 // Sorted leaf keys of the minimal 2:1 (face, edge, corner) balanced refinement.
 DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipStream_t s, BalStats *stats);
 // Keys <-> ot::TreeNode arrays (7 x uint32 per octant, device).

@@ -50,13 +50,14 @@ namespace {
 
 constexpr int B = 256;
 
-__global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, SfcPacked tab, Key *keys,
-                                                uint32_t *bad) {
+__global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, double gx, double gy,
+                                                double gz, SfcPacked tab, Key *keys, uint32_t *bad) {
     LOAD_TABLES(tab)
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     const double scale = (double)(1u << md);
-    double fx = xyz[3 * i] * scale / 1.0, fy = xyz[3 * i + 1] * scale / 1.0, fz = xyz[3 * i + 2] * scale / 1.0;
+    // (unsigned)(x * 2^maxDepth / gLen), construct.cpp:933-941
+    double fx = xyz[3 * i] * scale / gx, fy = xyz[3 * i + 1] * scale / gy, fz = xyz[3 * i + 2] * scale / gz;
     uint32_t px = 0, py = 0, pz = 0;
     if (fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale) {
         px = (uint32_t)fx;
@@ -427,7 +428,7 @@ static void copy_keys(Key *dst, const Key *src, size_t n, hipStream_t s) {
 }
 
 DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, int curve, hipStream_t s,
-                      P2OStats *stats) {
+                      P2OStats *stats, const double *glens) {
     SfcPacked tab = make_packed(curve);
     if (md == 0 || n <= m) {
         DevBuf<Key> root(1, s);
@@ -440,7 +441,8 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
     DevBuf<Key> pk(n, s);
     DevBuf<uint32_t> bad(1, s);
     HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
-    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, tab, pk.p, bad.p);
+    const double gx = glens ? glens[0] : 1.0, gy = glens ? glens[1] : 1.0, gz = glens ? glens[2] : 1.0;
+    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p, bad.p);
     HIP_OK(hipGetLastError());
     radix_sort_keys(pk.p, nullptr, n, s, 0, 128);
     uint32_t hbad = 0;

@@ -51,6 +51,11 @@ int dgx_memset_device(void *dst, int value, size_t bytes);
  * a device pointer.  The result is sorted, linear and complete. */
 int dgx_points_to_octree(const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts, int curve,
                          int on_device, dgx_octree *out);
+/* Same with the domain lengths gLens[3] of points2Octree (octUtils.h:633):
+ * a point quantises to (unsigned)(x * 2^maxDepth / gLens[0]) per axis
+ * (construct.cpp:933-941) and must lie in [0, gLens).  glens NULL = unit cube. */
+int dgx_points_to_octree_glens(const double *xyz, size_t n, const double *glens, unsigned max_depth,
+                               unsigned max_num_pts, int curve, int on_device, dgx_octree *out);
 /* ot::balanceOctree on one rank (octUtils.h:692-694; balance.cpp:31-64):
  * 2:1 balance across faces, edges and corners.  inc_corner is accepted for
  * signature parity; in 3-D the reference always balances across corners
@@ -98,6 +103,10 @@ int dgx_da_get_info(dgx_da da, dgx_da_info *info);
  *   local_to_node: local index -> compact node id, 0xffffffff if not a node
  *     (the vecGetBuffer compaction, oda.cpp:863-882) */
 int dgx_da_export(dgx_da da, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels, uint32_t *local_to_node);
+/* Anchors of the local-buffer octants (3 x uint32 per octant, interleaved)
+ * in the embedded 2^(maxDepth+1) domain; with oct_levels they give the
+ * octant of DA::getChildNumber / getCurrentOffset (oda.tcc:1273-1300). */
+int dgx_da_export_anchors(dgx_da da, uint32_t *xyz);
 /* y = (lap_fac * K + mass_fac * M) x on compact node vectors (device pointers
  * of length n_nodes).  Jacobian1MatMult (examples/src/backend/odaJac.C:
  * 297-366) with type-2 hanging stencils; lapFac/massFac as read from the
@@ -125,6 +134,9 @@ typedef struct dgx_mg_s *dgx_mg;
  * restriction R = P^T between levels, Jacobi diagonals and the coarsest-level
  * FP64 direct solver.  The octree is not consumed. */
 int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fac, dgx_mg *out);
+/* The operator factors of DAMGSetKSP's Jacobian (-lapFac lap + massFac I,
+ * omgJac.C:958-1057): recomputes the level diagonals and the coarse solver. */
+int dgx_mg_set_operator(dgx_mg mg, double lap_fac, double mass_fac);
 /* Number of levels actually built and, per level (finest first), the number of
  * octants and of nodes (arrays of at least 32 entries, may be NULL). */
 int dgx_mg_info(dgx_mg mg, int *nlevels, uint64_t *octants, uint64_t *nodes);
@@ -159,6 +171,11 @@ int dgx_mg_free(dgx_mg mg);
  *           sigma sig_lo (or log-uniform in [sig_lo, sig_hi] when sig_hi > 0),
  *           equal weights (or Dirichlet(1) weights when dirichlet != 0), clipped.
  * SplitMix64 keyed by (seed, stream, counter); Box-Muller normals. */
+/* The SFC tables as used by the library (24 x 8 each): rank of Morton child
+ * m under rotation r, Morton child of rank k, rotation entered in Morton
+ * child m (HILBERT_TABLE / rotations, src/hcurvedata.cpp:329-546; identity
+ * for Morton).  Any pointer may be NULL. */
+int dgx_sfc_tables(int curve, uint8_t *rank192, uint8_t *morton192, uint8_t *crot192);
 int dgx_gen_points(int dist, uint64_t seed, size_t n, int n_gauss, double lo, double hi, double sig_lo, double sig_hi,
                    int dirichlet, double *out);
 
