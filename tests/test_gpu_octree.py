@@ -142,13 +142,13 @@ def test_sample_sort(dgx, oracle):
     assert np.array_equal(dgx.sample_sort(t[perm]), t)
 
 
-def test_config1_full_parity(dgx, oracle, stencils):
+def test_uniform_256k_full_parity(dgx, oracle, stencils):
     """Config 1 at its full size: 262,144 uniform points, maxNumPts=1."""
     pts = dgx.workload_points("uniform-256K")
     check_pipeline(dgx, oracle, pts, 30, 1, *stencils)
 
 
-def test_config2_full_size_properties(dgx, oracle):
+def test_lognormal_4m_full_size_properties(dgx, oracle):
     """Config 2 at its full size (4,194,304 log-normal points, maxNumPts=8):
     octree and balanced octree bit-exact against the oracle, and the operator's
     conservation properties hold."""
