@@ -35,18 +35,6 @@ void dgx_set_error(const char *fmt, ...) {
     g_err = buf;
 }
 
-void *dgx_alloc(size_t bytes, hipStream_t s) {
-    void *p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, s);
-    if (e != hipSuccess) {
-        dgx_set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
-        throw DgxError();
-    }
-    return p;
-}
-void dgx_free(void *p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
-}
 
 static void ensure_ctx() {
     if (!g_ctx.ready) {

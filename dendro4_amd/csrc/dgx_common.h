@@ -249,9 +249,12 @@ void dgx_set_error(const char *fmt, ...);
 
 struct DgxError {};
 
-// Stream-ordered device allocation from the library's memory pool.
+// Stream-ordered device allocation (dgx_alloc.cpp): a size-class cache in
+// front of the library's HIP memory pool.
 void *dgx_alloc(size_t bytes, hipStream_t s);
 void dgx_free(void *p, hipStream_t s);
+void dgx_alloc_trim();            // return every cached block to the pool
+size_t dgx_alloc_cached_bytes();  // bytes held in the cache
 
 template <typename T>
 struct DevBuf {
@@ -293,9 +296,13 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // Primitives (dgx_sort.hip)
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of 128-bit keys with an optional 32-bit payload.
-// Bits [begin_bit, end_bit) of the 128-bit value (hi:lo) are sorted; passes in
-// which every key has the same 8-bit digit are skipped.
-void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
+// Bits [begin_bit, end_bit) of the 128-bit value (hi:lo) are sorted.  With
+// probe, one histogram pass (and one host read) finds the 8-bit digit
+// positions in which every key agrees and skips them; without, every position
+// overlapping the range is sorted and nothing is read back.  At most 2048
+// keys without payload are sorted in one workgroup.
+void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128,
+                     bool probe = true);
 // Exclusive scan of n uint32 values; returns the total (synchronises the stream).
 // *p = v in stream order (a kernel, so it is safe under graph capture and
 // needs no host staging).

@@ -41,7 +41,7 @@ __device__ inline void dev_decode(const uint8_t *mc, const Key &k, uint32_t md, 
 }
 
 struct ReplayResult {
-    uint32_t nout, end, err, pad;
+    uint32_t nout, end, err, next_t;  // next_t: first textbook leaf at or inside resync
     Key resync;
 };
 

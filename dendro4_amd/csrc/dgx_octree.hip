@@ -244,6 +244,7 @@ __global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, co
     r.end = 0;
     r.err = 0;
     r.resync = Key{0, 0};
+    r.next_t = (uint32_t)nT;
     uint64_t s = arrival_cursor(T, tb, tc, ktrig, md);
     Key curr;
     if (!key_next(T[ktrig], curr)) {
@@ -284,6 +285,7 @@ __global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, co
     }
     if (synced) {
         r.resync = curr;
+        r.next_t = (uint32_t)lower_bound_lt(T, ktrig + 1, nT, curr);  // first textbook leaf in resync
     } else {  // tail loop (:1051-1061): coarsest completion to the end of the domain
         while (true) {
             if (r.nout >= cap) { r.err = 1; *res = r; return; }
@@ -339,11 +341,12 @@ __global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32
 
 // Parents of the 26 neighbours of every split octant: per axis the neighbours'
 // parents take at most two values, so at most 8 distinct parents per octant.
-__global__ void k_bal_nbr_parents(const Key *Q, size_t nq, uint32_t lvl, uint32_t md, Key *out, uint32_t *counter) {
+__global__ void k_bal_nbr_parents(const Key *Q, const uint32_t *d_nq, uint32_t lvl, uint32_t md, Key *out,
+                                  uint32_t *counter) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     Key loc[8];
     int nl = 0;
-    if (i < nq) {
+    if (i < *d_nq) {
         uint32_t x, y, z, l;
         morton_decode(Q[i], md, x, y, z, l);
         const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
@@ -367,9 +370,11 @@ __global__ void k_bal_nbr_parents(const Key *Q, size_t nq, uint32_t lvl, uint32_
 }
 
 // For every split octant at the finer level, mark its child bit in the parent (Qp sorted unique).
-__global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, size_t np, uint32_t *cmask) {
+__global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, const uint32_t *d_np,
+                                      uint32_t *cmask) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
+    const size_t np = *d_np;
     Key par = key_parent(Qc[i]);
     size_t j = lower_bound_lt(Qp, 0, np, par);
     if (j >= np || !key_eq(Qp[j], par)) return;  // cannot happen: split octants are closed under parents
@@ -377,17 +382,18 @@ __global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, s
     atomicOr(&cmask[j], 1u << d);
 }
 
-__global__ void k_leaf_count(const uint32_t *cmask, size_t nq, uint32_t *cnt) {
+// cnt[i] for i < cap: leaves under split octant i (0 beyond the device count).
+__global__ void k_leaf_count(const uint32_t *cmask, const uint32_t *d_nq, size_t cap, uint32_t *cnt) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    cnt[i] = 8 - __popc(cmask[i] & 255u);
+    if (i >= cap) return;
+    cnt[i] = i < *d_nq ? 8 - __popc(cmask[i] & 255u) : 0u;
 }
 
-__global__ void __launch_bounds__(B) k_leaf_emit(const Key *Q, size_t nq, const uint32_t *cmask, const uint32_t *off,
-                                                 uint32_t md, SfcPacked tab, Key *out) {
+__global__ void __launch_bounds__(B) k_leaf_emit(const Key *Q, const uint32_t *d_nq, const uint32_t *cmask,
+                                                 const uint32_t *off, uint32_t md, SfcPacked tab, Key *out) {
     LOAD_TABLES(tab)
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    if (i >= nq) return;
+    if (i >= *d_nq) return;
     uint32_t x, y, z, l;
     morton_decode(Q[i], md, x, y, z, l);
     uint32_t len = 1u << (md - l - 1), o = off[i], msk = cmask[i];
@@ -522,16 +528,7 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
         st.replayed_leaves += hr.nout;
         st.replays++;
         if (hr.end) { t_next = nT; break; }
-        // first textbook leaf at or inside the resync octant
-        std::vector<Key> hT;  // binary search on device would need another kernel; T is sorted, search via host copy of a window
-        size_t lo = k + 1, hi = nT;
-        while (lo < hi) {  // lower_bound on device array through small copies
-            size_t mid = lo + (hi - lo) / 2;
-            Key km;
-            HIP_OK(hipMemcpy(&km, T.p + mid, sizeof(Key), hipMemcpyDeviceToHost));
-            if (key_lt(km, hr.resync)) lo = mid + 1; else hi = mid;
-        }
-        t_next = lo;
+        t_next = hr.next_t;  // first textbook leaf at or inside the resync octant
     }
     if (t_next < nT) segs.push_back({0, t_next, nT - t_next});
     size_t total = 0;
@@ -584,73 +581,83 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     k_bucket_parents<<<grid_for(n, B), B, 0, s>>>(in, n, md, tab, cursor.p, inpar.p);
     HIP_OK(hipGetLastError());
     const uint32_t maxL = lmax;
-    DevBuf<Key> prevQ;        // split octants one level finer (sorted unique Morton)
+    // Per level: sort + unique the seeds W (parents at level l-1), emit the
+    // leaves of level l, generate the neighbour parents that seed level l-1.
+    // Counts stay on the device (cnt[0] split octants, cnt[1] leaves, cnt[2]
+    // neighbour parents); the host reads them once per level.
+    DevBuf<uint32_t> cnt(4, s);
+    uint32_t *d_nq = cnt.p, *d_nlv = cnt.p + 1, *d_nnb = cnt.p + 2;
+    DevBuf<Key> prevQ;  // split octants one level finer (sorted unique Morton)
     size_t nprev = 0;
-    DevBuf<Key> nbpar;        // parents of neighbours generated for the current level
-    size_t nnb = 0;
-    std::vector<DevBuf<Key>> leaves;
-    std::vector<size_t> nleaves;
-    DevBuf<uint32_t> counter(1, s);
+    DevBuf<Key> W;      // seeds of the current level: input parents, then neighbour parents
+    size_t nw = 0;
+    DevBuf<Key> leaves(2 * n + 4096, s);
+    size_t used = 0;
+    // seeds of the finest level: input parents only
+    nw = hl[maxL];
+    W.alloc(nw, s);
+    copy_keys(W.p, inpar.p + loff[maxL], nw, s);
     for (uint32_t l = maxL; l >= 1; --l) {
-        size_t nin = hl[l];
-        size_t nw = nin + nnb;
         if (nw == 0) {
             prevQ.release();
             nprev = 0;
+            if (l >= 2) {
+                nw = hl[l - 1];
+                W.alloc(nw, s);
+                copy_keys(W.p, inpar.p + loff[l - 1], nw, s);
+            }
             continue;
         }
-        DevBuf<Key> W(nw, s);
-        copy_keys(W.p, inpar.p + loff[l], nin, s);
-        copy_keys(W.p + nin, nbpar.p, nnb, s);
-        nbpar.release();
-        nnb = 0;
-        // parents live at level l-1: only digits 1..l-1 vary
-        radix_sort_keys(W.p, nullptr, nw, s, 0, 128);
+        // parents live at level lv = l-1: only digits 1..lv vary
+        const uint32_t lv = l - 1;
+        const int begin_bit = lv <= 21 ? 127 - 3 * (int)lv : 63 - 3 * ((int)lv - 21);
+        radix_sort_keys(W.p, nullptr, nw, s, begin_bit, 127, nw >= (size_t(1) << 18));
         DevBuf<uint32_t> f(nw, s), fp(nw, s);
         k_unique_flag<<<grid_for(nw, B), B, 0, s>>>(W.p, nw, f.p);
-        uint64_t nq = scan_exclusive_u32(f.p, fp.p, nw, s);
-        DevBuf<Key> Q(nq, s);
+        scan_exclusive_u32_async(f.p, fp.p, nw, d_nq, s);
+        DevBuf<Key> Q(nw, s);  // capacity; *d_nq used
         k_compact_keys<<<grid_for(nw, B), B, 0, s>>>(W.p, f.p, fp.p, nw, Q.p);
-        W.release();
-        f.release();
-        fp.release();
         // leaves at level l: children of Q not split themselves
-        DevBuf<uint32_t> cmask(nq, s), lc(nq, s), lo(nq, s);
-        HIP_OK(hipMemsetAsync(cmask.p, 0, nq * 4, s));
-        if (nprev) k_mark_split_children<<<grid_for(nprev, B), B, 0, s>>>(prevQ.p, nprev, Q.p, nq, cmask.p);
-        k_leaf_count<<<grid_for(nq, B), B, 0, s>>>(cmask.p, nq, lc.p);
-        uint64_t nlv = scan_exclusive_u32(lc.p, lo.p, nq, s);
-        DevBuf<Key> lv(nlv, s);
-        k_leaf_emit<<<grid_for(nq, B), B, 0, s>>>(Q.p, nq, cmask.p, lo.p, md, tab, lv.p);
-        HIP_OK(hipGetLastError());
-        leaves.push_back(std::move(lv));
-        nleaves.push_back(nlv);
-        // neighbours of the split octants seed the next coarser level
-        if (l >= 2) {
-            nbpar.alloc(8 * nq, s);
-            HIP_OK(hipMemsetAsync(counter.p, 0, 4, s));
-            k_bal_nbr_parents<<<grid_for(nq, B), B, 0, s>>>(Q.p, nq, l - 1, md, nbpar.p, counter.p);
-            uint32_t hc;
-            HIP_OK(hipMemcpyAsync(&hc, counter.p, 4, hipMemcpyDeviceToHost, s));
-            HIP_OK(hipStreamSynchronize(s));
-            nnb = hc;
+        DevBuf<uint32_t> cmask(nw, s), lc(nw, s), lo(nw, s);
+        HIP_OK(hipMemsetAsync(cmask.p, 0, nw * 4, s));
+        if (nprev) k_mark_split_children<<<grid_for(nprev, B), B, 0, s>>>(prevQ.p, nprev, Q.p, d_nq, cmask.p);
+        k_leaf_count<<<grid_for(nw, B), B, 0, s>>>(cmask.p, d_nq, nw, lc.p);
+        scan_exclusive_u32_async(lc.p, lo.p, nw, d_nlv, s);
+        if (leaves.n < used + 8 * nw) {  // grow: at most 8 leaves per split octant
+            DevBuf<Key> g(std::max(2 * leaves.n, used + 8 * nw), s);
+            copy_keys(g.p, leaves.p, used, s);
+            leaves = std::move(g);
         }
-        prevQ = std::move(Q);
-        nprev = nq;
+        k_leaf_emit<<<grid_for(nw, B), B, 0, s>>>(Q.p, d_nq, cmask.p, lo.p, md, tab, leaves.p + used);
+        HIP_OK(hipGetLastError());
+        // neighbours of the split octants seed the next coarser level
+        DevBuf<Key> Wn;
+        const size_t nin_next = l >= 2 ? hl[l - 1] : 0;
+        HIP_OK(hipMemsetAsync(d_nnb, 0, 4, s));
+        if (l >= 2) {
+            Wn.alloc(nin_next + 8 * nw, s);
+            copy_keys(Wn.p, inpar.p + loff[l - 1], nin_next, s);
+            k_bal_nbr_parents<<<grid_for(nw, B), B, 0, s>>>(Q.p, d_nq, l - 1, md, Wn.p + nin_next, d_nnb);
+            HIP_OK(hipGetLastError());
+        }
+        uint32_t hc[3];
+        HIP_OK(hipMemcpyAsync(hc, cnt.p, 12, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        used += hc[1];
         st.levels++;
         st.seeds += nw;
+        prevQ = std::move(Q);
+        nprev = hc[0];
+        W = std::move(Wn);
+        nw = nin_next + hc[2];
     }
-    size_t total = 0;
-    for (size_t v : nleaves) total += v;
-    out.alloc(total, s);
-    size_t o = 0;
-    for (size_t i = 0; i < leaves.size(); ++i) {
-        copy_keys(out.p + o, leaves[i].p, nleaves[i], s);
-        o += nleaves[i];
-    }
-    leaves.clear();
-    radix_sort_keys(out.p, nullptr, total, s, 0, 128);
+    W.release();
+    prevQ.release();
+    const size_t total = used;
+    radix_sort_keys(leaves.p, nullptr, total, s, 0, 128);
     HIP_OK(hipStreamSynchronize(s));
+    out = std::move(leaves);
+    out.n = total;
     st.leaves = total;
     if (stats) *stats = st;
     return out;

@@ -104,7 +104,64 @@ __global__ void __launch_bounds__(SCAN_B) k_scan_down(const uint32_t *in, uint32
     }
 }
 
+// One workgroup scans any n (in place allowed): 1024 threads x 16 items per
+// round, carry between rounds.  Used for small arrays, where the three-kernel
+// scan would be launch-bound.
+constexpr int SS_B = 1024, SS_ITEMS = 16;
+__global__ void __launch_bounds__(SS_B) k_scan_single(const uint32_t *in, uint32_t *out, size_t n,
+                                                      uint32_t *d_total) {
+    __shared__ uint32_t lds[SS_B / 64 + 1];
+    uint32_t carry = 0;
+    for (size_t base = 0; base < n; base += (size_t)SS_B * SS_ITEMS) {
+        const size_t b0 = base + (size_t)threadIdx.x * SS_ITEMS;
+        uint32_t v[SS_ITEMS];
+        uint32_t t = 0;
+#pragma unroll
+        for (int i = 0; i < SS_ITEMS; ++i) {
+            v[i] = b0 + i < n ? in[b0 + i] : 0u;
+            t += v[i];
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(t, lds, tot) + carry;
+#pragma unroll
+        for (int i = 0; i < SS_ITEMS; ++i) {
+            if (b0 + i < n) out[b0 + i] = ex;
+            ex += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
 // --------------------------------------------------------- radix sort ----
+// Bitonic sort of at most SMALL_N keys in one workgroup (LDS), padded with
+// all-ones keys.  Keys only: equal keys are identical, so stability is moot.
+constexpr int SMALL_N = 2048;
+__global__ void __launch_bounds__(1024) k_sort_small(Key *keys, uint32_t n) {
+    __shared__ Key sk[SMALL_N];
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += 1024) sk[i] = i < n ? keys[i] : Key{~0ull, ~0ull};
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += 1024) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const Key a = sk[i], b = sk[l];
+                    if (up ? key_lt(b, a) : key_lt(a, b)) {
+                        sk[i] = b;
+                        sk[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) keys[i] = sk[i];
+}
+
 // Global histograms of all 16 digit positions (16 x 256 counters).
 __global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, uint32_t *ghist) {
     __shared__ uint32_t h[16 * 256];
@@ -202,6 +259,11 @@ void scan_exclusive_u32_async(const uint32_t *in, uint32_t *out, size_t n, uint3
         if (d_total) HIP_OK(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
         return;
     }
+    if (n <= (size_t)SS_B * SS_ITEMS * 4) {
+        k_scan_single<<<1, SS_B, 0, s>>>(in, out, n, d_total);
+        HIP_OK(hipGetLastError());
+        return;
+    }
     size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     DevBuf<uint32_t> partial(nb, s);
     k_scan_reduce<<<(unsigned)nb, SCAN_B, 0, s>>>(in, n, partial.p);
@@ -219,28 +281,39 @@ uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStre
     return h;
 }
 
-void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit, int end_bit) {
+void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit, int end_bit, bool probe) {
     if (n < 2) return;
     if (n > 0xffffffffull) {
         dgx_set_error("radix_sort_keys: %zu keys exceed the 32-bit index range", n);
         throw DgxError();
     }
-    DevBuf<uint32_t> ghist(16 * 256, s);
-    HIP_OK(hipMemsetAsync(ghist.p, 0, 16 * 256 * sizeof(uint32_t), s));
-    unsigned gb = (unsigned)std::min<size_t>(grid_for(n, SB), 2048);
-    k_global_hist<<<gb, SB, 0, s>>>(keys, n, ghist.p);
-    HIP_OK(hipGetLastError());
-    std::vector<uint32_t> h(16 * 256);
-    HIP_OK(hipMemcpyAsync(h.data(), ghist.p, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    if (!vals && n <= (size_t)SMALL_N) {
+        k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
+        HIP_OK(hipGetLastError());
+        return;
+    }
     std::vector<int> passes;
-    for (int p = 0; p < 16; ++p) {
-        int bit = 8 * p;
-        if (bit + 8 <= begin_bit || bit >= end_bit) continue;
-        bool trivial = false;
-        for (int d = 0; d < 256; ++d)
-            if (h[p * 256 + d] == n) trivial = true;
-        if (!trivial) passes.push_back(bit);
+    if (probe) {
+        // skip digit positions that are constant over all keys (one host read)
+        DevBuf<uint32_t> ghist(16 * 256, s);
+        HIP_OK(hipMemsetAsync(ghist.p, 0, 16 * 256 * sizeof(uint32_t), s));
+        unsigned gb = (unsigned)std::min<size_t>(grid_for(n, SB), 2048);
+        k_global_hist<<<gb, SB, 0, s>>>(keys, n, ghist.p);
+        HIP_OK(hipGetLastError());
+        std::vector<uint32_t> h(16 * 256);
+        HIP_OK(hipMemcpyAsync(h.data(), ghist.p, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        for (int p = 0; p < 16; ++p) {
+            int bit = 8 * p;
+            if (bit + 8 <= begin_bit || bit >= end_bit) continue;
+            bool trivial = false;
+            for (int d = 0; d < 256; ++d)
+                if (h[p * 256 + d] == n) trivial = true;
+            if (!trivial) passes.push_back(bit);
+        }
+    } else {
+        for (int p = 0; p < 16; ++p)
+            if (!(8 * p + 8 <= begin_bit || 8 * p >= end_bit)) passes.push_back(8 * p);
     }
     if (passes.empty()) return;
     size_t nblocks = (n + STILE - 1) / STILE;
