@@ -34,7 +34,8 @@ EXPORTED = [
     "dgx_octree_free", "dgx_da_create", "dgx_da_get_info", "dgx_da_export", "dgx_da_matvec", "dgx_da_matvec_f32",
     "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points", "dgx_da_matvec_dirichlet", "dgx_mg_create", "dgx_mg_info",
     "dgx_mg_finest_da", "dgx_mg_rhs", "dgx_mg_solve", "dgx_mg_free", "dgx_points_to_octree_glens",
-    "dgx_sfc_tables", "dgx_da_export_anchors", "dgx_mg_set_operator",
+    "dgx_sfc_tables", "dgx_da_export_anchors", "dgx_mg_set_operator", "dgx_server_create", "dgx_server_submit",
+    "dgx_server_wait", "dgx_server_clock", "dgx_server_free", "dgx_server_peek",
 ]
 
 
@@ -62,6 +63,17 @@ class _DaInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("fp32_vcycle", C.c_int), ("pre", C.c_int), ("post", C.c_int), ("omega", C.c_double),
                 ("rtol", C.c_double), ("maxit", C.c_int), ("use_graph", C.c_int)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_uint), ("hbm_budget", C.c_uint64), ("nlevels", C.c_int), ("fp32_vcycle", C.c_int),
+                ("rtol", C.c_double), ("maxit", C.c_int), ("keep_resident", C.c_int), ("return_solution", C.c_int)]
+
+
+class Served(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
+                ("octants", C.c_uint64), ("nodes", C.c_uint64), ("levels", C.c_int), ("reused", C.c_int),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -93,6 +105,12 @@ def load_library():
         "dgx_memset_device": ([vp, i32, sz], i32),
         "dgx_points_to_octree": ([vp, sz, u32, u32, i32, i32, C.POINTER(vp)], i32),
         "dgx_points_to_octree_glens": ([vp, sz, vp, u32, u32, i32, i32, C.POINTER(vp)], i32),
+        "dgx_server_create": ([C.POINTER(ServerOpts), C.POINTER(vp)], i32),
+        "dgx_server_submit": ([vp, vp, sz, u32, u32, C.c_uint64, C.POINTER(C.c_uint64)], i32),
+        "dgx_server_wait": ([vp, C.c_uint64, C.POINTER(Served), vp, sz], i32),
+        "dgx_server_clock": ([vp], C.c_double),
+        "dgx_server_peek": ([vp, C.c_uint64, C.POINTER(Served)], i32),
+        "dgx_server_free": ([vp], i32),
         "dgx_balance_octree": ([vp, i32, C.POINTER(vp)], i32),
         "dgx_octree_import": ([vp, sz, u32, i32, i32, i32, C.POINTER(vp)], i32),
         "dgx_octree_size": ([vp, C.POINTER(sz)], i32),
@@ -444,3 +462,81 @@ def workload_points(name, n=None):
 
 def workload_max_num_pts(name):
     return WORKLOADS[name][1]
+
+
+class Server:
+    """Request server (dgx_server_*): streams of Poisson problems on point
+    clouds, `slots` at a time, admitted against an HBM budget, problems kept
+    resident for reuse."""
+
+    def __init__(self, slots=4, hbm_budget=200e9, nlevels=5, fp32=True, rtol=1e-8, maxit=200, keep_resident=True,
+                 return_solution=False):
+        self._o = ServerOpts(slots, int(hbm_budget), nlevels, 1 if fp32 else 0, rtol, maxit,
+                             1 if keep_resident else 0, 1 if return_solution else 0)
+        self._h = C.c_void_p()
+        _check(_lib_ready().dgx_server_create(C.byref(self._o), C.byref(self._h)))
+        self._sizes = {}
+
+    def submit(self, pts, max_depth=30, max_num_pts=1, seed=0):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        rid = C.c_uint64()
+        _check(_lib.dgx_server_submit(self._h, _ptr(pts), len(pts), max_depth, max_num_pts, seed, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, solution=False):
+        r = Served()
+        x = None
+        if solution:
+            _check(_lib.dgx_server_peek(self._h, rid, C.byref(r)))
+            x = np.empty(max(int(r.nodes), 1), np.float64)
+            rc = _lib.dgx_server_wait(self._h, rid, C.byref(r), _ptr(x), len(x))
+        else:
+            rc = _lib.dgx_server_wait(self._h, rid, C.byref(r), None, 0)
+        _check(rc)
+        out = {k: getattr(r, k) for k, _ in Served._fields_}
+        if solution:
+            out["x"] = x[: r.nodes].copy()
+        return out
+
+    def clock(self):
+        return _lib.dgx_server_clock(self._h)
+
+    def close(self):
+        if self._h:
+            _check(_lib.dgx_server_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# Served stream of Poisson problems -------------------------------------------
+SERVE_DISTS = ("uniform", "lognormal", "gauss4")
+
+
+def serve_workload(count=2000, seed=3, rate=200.0):
+    """Request list of the served stream: per request the point count is
+    log-uniform in [2^12, 2^22], the distribution uniform over {uniform [0,1)^3,
+    exp(N(-1,0.5)) clipped, 4 Gaussians sigma 0.02}, maxNumPts in {1,4,8};
+    arrivals are Poisson at `rate` per second.  Returns dicts with n, dist,
+    max_num_pts, arrival (seconds from the start) and the cloud's seed."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        n = int(round(2.0 ** rng.uniform(12.0, 22.0)))
+        dist = int(rng.integers(0, 3))
+        mnp = int(rng.choice([1, 4, 8]))
+        out.append({"n": n, "dist": dist, "max_num_pts": mnp, "arrival": t, "seed": (seed << 32) + i})
+    return out
+
+
+def serve_points(req):
+    """Point cloud of one served request."""
+    if req["dist"] == 2:
+        return gen_points(2, req["seed"], req["n"], n_gauss=4, lo=0.2, hi=0.8, sig_lo=0.02)
+    return gen_points(req["dist"], req["seed"], req["n"])

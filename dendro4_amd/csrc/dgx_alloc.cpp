@@ -104,3 +104,18 @@ size_t dgx_alloc_cached_bytes() {
     std::lock_guard<std::mutex> lk(g_mu);
     return g_cached_bytes;
 }
+
+void dgx_alloc_release_stream(hipStream_t s) {
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+        if (it->second.s == s) {
+            (void)hipFreeAsync(it->second.p, s);
+            g_cached_bytes -= it->first;
+            it = g_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    (void)hipStreamSynchronize(s);
+}

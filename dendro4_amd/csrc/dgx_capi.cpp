@@ -36,6 +36,13 @@ void dgx_set_error(const char *fmt, ...) {
 }
 
 
+// Device of the initialised context (internal, for the request server).
+bool dgx_ctx_device(int *device) {
+    if (!g_ctx.ready) return false;
+    *device = g_ctx.device;
+    return true;
+}
+
 static void ensure_ctx() {
     if (!g_ctx.ready) {
         dgx_set_error("dgx_init() has not been called");

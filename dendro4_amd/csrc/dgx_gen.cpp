@@ -3,9 +3,11 @@
 // under seed k is mix64(k * G1 + s * G2 + (i + 1) * G), so every coordinate is
 // a pure function of (seed, stream, counter) and the arrays are reproducible
 // on any host with the same libm.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/dendro_mi355x.h"
@@ -33,17 +35,37 @@ inline double clip01(double v) {
     return v < 0.0 ? 0.0 : (v > top ? top : v);
 }
 
+// Runs body(lo, hi) over [0, n) on up to 16 host threads (every value is a
+// pure function of its index, so the split does not change the result).
+template <typename F>
+void parallel_range(size_t n, F body) {
+    const size_t grain = size_t(1) << 16;
+    unsigned t = std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 2 * grain || t < 2) {
+        body(0, n);
+        return;
+    }
+    t = (unsigned)std::min<size_t>(t, n / grain);
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < t; ++k) th.emplace_back(body, n * k / t, n * (k + 1) / t);
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" int dgx_gen_points(int dist, uint64_t seed, size_t n, int n_gauss, double lo, double hi, double sig_lo,
                               double sig_hi, int dirichlet, double *out) {
     if (!out && n) return -1;
     if (dist == 0) {
-        for (size_t i = 0; i < 3 * n; ++i) out[i] = unif(seed, 0, i);
+        parallel_range(3 * n, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) out[i] = unif(seed, 0, i);
+        });
         return 0;
     }
     if (dist == 1) {
-        for (size_t i = 0; i < 3 * n; ++i) out[i] = clip01(std::exp(-1.0 + 0.5 * normal(seed, 0, i)));
+        parallel_range(3 * n, [&](size_t lo, size_t hi) {
+            for (size_t i = lo; i < hi; ++i) out[i] = clip01(std::exp(-1.0 + 0.5 * normal(seed, 0, i)));
+        });
         return 0;
     }
     if (dist != 2 || n_gauss <= 0) return -1;
@@ -65,11 +87,14 @@ extern "C" int dgx_gen_points(int dist, uint64_t seed, size_t n, int n_gauss, do
         cdf[k] = tot;
     }
     for (int k = 0; k < K; ++k) cdf[k] /= tot;
-    for (size_t i = 0; i < n; ++i) {
-        double u = unif(seed, 4, i);
-        int k = 0;
-        while (k < K - 1 && u >= cdf[k]) ++k;
-        for (int a = 0; a < 3; ++a) out[3 * i + a] = clip01(cen[3 * k + a] + sig[k] * normal(seed, 5, 3 * i + a));
-    }
+    parallel_range(n, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            double u = unif(seed, 4, i);
+            int k = 0;
+            while (k < K - 1 && u >= cdf[k]) ++k;
+            for (int a = 0; a < 3; ++a)
+                out[3 * i + a] = clip01(cen[3 * k + a] + sig[k] * normal(seed, 5, 3 * i + a));
+        }
+    });
     return 0;
 }

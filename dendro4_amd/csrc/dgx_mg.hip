@@ -20,7 +20,11 @@
 //     coarse solve stay FP64.
 #include "dgx_mg.h"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <chrono>
+#include <mutex>
 #include <cmath>
 
 namespace {
@@ -172,27 +176,11 @@ __global__ void k_dinv(const double *diag, const uint8_t *bdy, uint32_t n, doubl
 }
 
 // Gauss-Jordan inversion of an SPD matrix, one pivot per launch (no pivoting).
-__global__ void k_gj_scale(double *A, uint32_t n, uint32_t k, double *rowk) {
-    uint32_t j = blockIdx.x * B + threadIdx.x;
-    if (j >= n) return;
-    const double p = A[(size_t)k * n + k];
-    double v = (j == k) ? 1.0 / p : A[(size_t)k * n + j] / p;
-    rowk[j] = v;
-}
-__global__ void k_gj_update(double *A, uint32_t n, uint32_t k, const double *rowk, double *colk) {
-    const uint32_t j = blockIdx.x * B + threadIdx.x, i = blockIdx.y;
-    if (j >= n) return;
-    const double aik = colk[i];
-    if (i == k) {
-        A[(size_t)k * n + j] = rowk[j];
-    } else {
-        const double v = (j == k) ? 0.0 : A[(size_t)i * n + j];
-        A[(size_t)i * n + j] = v - aik * rowk[j];
-    }
-}
-__global__ void k_gj_col(const double *A, uint32_t n, uint32_t k, double *colk) {
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (i < n) colk[i] = A[(size_t)i * n + k];
+// Column-major lower triangle (row >= col) -> full symmetric matrix.
+__global__ void k_mirror_lower(double *A, uint32_t n) {
+    const uint32_t r = blockIdx.x * B + threadIdx.x, c = blockIdx.y;
+    if (r >= n || r <= c) return;
+    A[(size_t)r * n + c] = A[(size_t)c * n + r];
 }
 
 // -------------------------------------------------------- vector kernels ----
@@ -413,6 +401,22 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
     HIP_OK(hipGetLastError());
 }
 
+// Coarsest level: the dense Dirichlet matrix (SPD) is factored and inverted
+// in FP64 by rocSOLVER (Cholesky, potrf + potri), then mirrored into a full
+// symmetric inverse applied as a GEMV per V-cycle.  The reference factors its
+// coarsest matrix with PETSc LU (blockDiag.cpp:780-1047).
+static std::mutex g_blas_mu;
+static rocblas_handle blas_handle() {
+    static rocblas_handle h = nullptr;
+    if (!h) {
+        if (rocblas_create_handle(&h) != rocblas_status_success) {
+            dgx_set_error("multigrid: rocblas_create_handle failed");
+            throw DgxError();
+        }
+    }
+    return h;
+}
+
 static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     const DaDevice &d = *mg.lev.back().da;
     const uint32_t n = d.n_nodes;
@@ -425,14 +429,29 @@ static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     mg.nc = n;
     mg.cinv.alloc((size_t)n * n, s);
     da_assemble_dense_dirichlet(d, mg.lap_fac, mg.mass_fac, mg.cinv.p, s);
-    DevBuf<double> rowk(n, s), colk(n, s);
-    for (uint32_t k = 0; k < n; ++k) {
-        k_gj_scale<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k, rowk.p);
-        k_gj_col<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k, colk.p);
-        k_gj_update<<<dim3(grid_for(n, B), n), B, 0, s>>>(mg.cinv.p, n, k, rowk.p, colk.p);
+    DevBuf<rocblas_int> info(2, s);
+    {
+        std::lock_guard<std::mutex> lk(g_blas_mu);
+        rocblas_handle h = blas_handle();
+        if (rocblas_set_stream(h, s) != rocblas_status_success ||
+            rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, mg.cinv.p, (rocblas_int)n, info.p) !=
+                rocblas_status_success ||
+            rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, mg.cinv.p, (rocblas_int)n, info.p + 1) !=
+                rocblas_status_success) {
+            dgx_set_error("multigrid: rocSOLVER Cholesky of the %u x %u coarse matrix failed", n, n);
+            throw DgxError();
+        }
+        k_mirror_lower<<<dim3(grid_for(n, B), n), B, 0, s>>>(mg.cinv.p, n);
+        HIP_OK(hipGetLastError());
+        rocblas_int hinfo[2] = {0, 0};
+        HIP_OK(hipMemcpyAsync(hinfo, info.p, sizeof hinfo, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (hinfo[0] || hinfo[1]) {
+            dgx_set_error("multigrid: coarse matrix is not positive definite (potrf %d, potri %d)", (int)hinfo[0],
+                          (int)hinfo[1]);
+            throw DgxError();
+        }
     }
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(s));
 }
 
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
@@ -595,13 +614,18 @@ struct MgWork {
     Work<float> w32;
     DevBuf<double> x, bvec, r, z, p, q, rold, part, sc;
     Graph g_precond, g_step, g_dir;
-    // no replay may still be running when the executable graphs are destroyed
-    ~MgWork() { (void)hipDeviceSynchronize(); }
+    hipStream_t last = nullptr;  // stream of the last solve
+    // No replay may still be running when the executable graphs are destroyed.
+    // Only that stream is waited for: a device-wide wait would break graph
+    // captures running on other streams (request server).
+    ~MgWork() {
+        if (last) (void)hipStreamSynchronize(last);
+    }
 };
 MgHierarchy::~MgHierarchy() { delete work; }
 void mg_reset_graphs(MgHierarchy &mg) {
     if (!mg.work) return;
-    HIP_OK(hipDeviceSynchronize());
+    if (mg.work->last) HIP_OK(hipStreamSynchronize(mg.work->last));
     mg.work->g_precond.reset();
     mg.work->g_step.reset();
     mg.work->g_dir.reset();
@@ -697,6 +721,8 @@ double read_scalar(const double *p, hipStream_t s) {
 
 void mg_vcycle_apply(MgHierarchy &mg, const double *r, double *z, const SolveOpts &o, hipStream_t s) {
     MgWork &w = work_of(mg, s);
+    if (w.last && w.last != s) HIP_OK(hipStreamSynchronize(w.last));
+    w.last = s;
     const uint32_t n = mg.lev[0].da->n_nodes;
     k_copy<<<grid_for(n, B), B, 0, s>>>(r, n, w.r.p);
     run_graph(w.g_precond, o, o.use_graph, s, [&] { precond_body(mg, w, o, s); });
@@ -714,6 +740,8 @@ void mg_rhs_from_nodal(MgHierarchy &mg, const double *f, double *b, hipStream_t 
 void mg_solve(MgHierarchy &mg, const double *b, double *xout, const SolveOpts &o, SolveStats *st, hipStream_t s) {
     auto t0 = std::chrono::steady_clock::now();
     MgWork &w = work_of(mg, s);
+    if (w.last && w.last != s) HIP_OK(hipStreamSynchronize(w.last));  // work vectors are shared
+    w.last = s;
     const DaDevice &d = *mg.lev[0].da;
     const uint32_t n = d.n_nodes;
     const unsigned g = grid_for(n, B);

@@ -163,6 +163,50 @@ typedef struct {
 int dgx_mg_solve(dgx_mg mg, const double *b, double *u, const dgx_solve_opts *opts, dgx_solve_stats *stats);
 int dgx_mg_free(dgx_mg mg);
 
+/* ---- request server ----------------------------------------------------- */
+/* No counterpart in the reference (its drivers solve one problem per run):
+ * a server for streams of Poisson problems on point clouds.  Every request is
+ * points2Octree + balanceOctree + DAMGCreateAndSetDA + DAMGSolve of
+ * -lap u + u = f (homogeneous Dirichlet, f nodal N(0,1) from rhs_seed).
+ * `slots` requests run at once, each on its own stream, and a request is
+ * admitted when a slot is free and its memory estimate fits hbm_budget next to
+ * the running requests and the resident problems; finished problems stay
+ * resident (LRU, keyed by a hash of the points and parameters) when
+ * keep_resident is set, so a repeated request only re-solves. */
+typedef struct dgx_server_s *dgx_server;
+typedef struct {
+    unsigned slots;        /* 1..16 */
+    uint64_t hbm_budget;   /* bytes */
+    int nlevels;           /* multigrid levels requested */
+    int fp32_vcycle;       /* 1: FP32 V-cycle inside FP64 CG */
+    double rtol;
+    int maxit;
+    int keep_resident;
+    int return_solution;   /* 1: keep a host copy of each solution for dgx_server_wait */
+} dgx_server_opts;
+typedef struct {
+    uint64_t id;
+    int status;            /* 0 ok */
+    int iterations;
+    double rel_residual;
+    uint64_t octants, nodes;
+    int levels;
+    int reused;            /* 1: the hierarchy was resident */
+    double t_submit, t_start, t_done; /* seconds on the server clock */
+} dgx_served;
+int dgx_server_create(const dgx_server_opts *opts, dgx_server *out);
+/* Copies the n points; returns at once with the request id. */
+int dgx_server_submit(dgx_server sv, const double *xyz, size_t n, unsigned max_depth, unsigned max_num_pts,
+                      uint64_t rhs_seed, uint64_t *id);
+/* Blocks until request id is done; x (host, x_len >= nodes) receives the
+ * solution when return_solution is set, or may be NULL. */
+int dgx_server_wait(dgx_server sv, uint64_t id, dgx_served *res, double *x, size_t x_len);
+/* Blocks until request id is done and reports it without consuming it. */
+int dgx_server_peek(dgx_server sv, uint64_t id, dgx_served *res);
+double dgx_server_clock(dgx_server sv);
+/* Finishes the queued requests, then frees everything. */
+int dgx_server_free(dgx_server sv);
+
 /* ---- inputs ------------------------------------------------------------- */
 /* Deterministic synthetic point clouds (host memory, n*3 doubles):
  *   dist 0: i.i.d. uniform [0,1)^3

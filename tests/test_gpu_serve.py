@@ -1,0 +1,87 @@
+"""GPU tests of the request server (dgx_server_*): each request's result is
+the same alone and in a mix of concurrent requests (within the solver
+tolerance), resident problems are reused, the HBM budget gates admission,
+and concurrent requests overlap on the device."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def small_requests(dgx, count, seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(12, 15))
+        req = {"n": n, "dist": int(rng.integers(0, 3)), "max_num_pts": int(rng.choice([1, 4, 8])),
+               "seed": 1000 + i}
+        out.append((req, dgx.serve_points(req)))
+    return out
+
+
+def run_all(dgx, reqs, slots, fp32=True):
+    sv = dgx.Server(slots=slots, hbm_budget=64e9, nlevels=4, fp32=fp32, keep_resident=False, return_solution=True)
+    ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
+    res = [sv.wait(i, solution=True) for i in ids]
+    sv.close()
+    return res
+
+
+def test_result_is_the_same_alone_and_in_the_mix(dgx):
+    reqs = small_requests(dgx, 12)
+    mixed = run_all(dgx, reqs, slots=4)
+    alone = []
+    for rc in reqs:  # one at a time, nothing else on the device
+        alone += run_all(dgx, [rc], slots=1)
+    for a, m in zip(alone, mixed):
+        assert a["status"] == 0 and m["status"] == 0
+        assert a["nodes"] == m["nodes"] and a["octants"] == m["octants"] and a["levels"] == m["levels"]
+        assert a["rel_residual"] <= 1e-8 and m["rel_residual"] <= 1e-8
+        assert abs(a["iterations"] - m["iterations"]) <= 1
+        # LDS accumulation order is not fixed: solutions agree to the solver tolerance
+        d = np.linalg.norm(a["x"] - m["x"]) / np.linalg.norm(a["x"])
+        assert d <= 1e-8, d
+
+
+def test_resident_problem_is_reused(dgx):
+    (req, pts), = small_requests(dgx, 1, seed=5)
+    sv = dgx.Server(slots=2, hbm_budget=64e9, nlevels=4, keep_resident=True, return_solution=True)
+    a = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 7), solution=True)
+    b = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 7), solution=True)
+    c = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 8), solution=True)
+    sv.close()
+    assert a["reused"] == 0 and b["reused"] == 1 and c["reused"] == 1
+    assert np.linalg.norm(a["x"] - b["x"]) <= 1e-10 * np.linalg.norm(a["x"])
+    assert np.linalg.norm(a["x"] - c["x"]) > 1e-3 * np.linalg.norm(a["x"])  # another right-hand side
+
+
+def test_budget_gates_admission(dgx):
+    reqs = small_requests(dgx, 6, seed=3)
+    est = max(r["n"] for r, _ in reqs) * 1600 + (64 << 20)
+    sv = dgx.Server(slots=4, hbm_budget=est + 1, nlevels=4, keep_resident=False)
+    ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
+    res = sorted((sv.wait(i) for i in ids), key=lambda x: x["t_start"])
+    sv.close()
+    for a, b in zip(res, res[1:]):  # one request at a time fits the budget
+        assert b["t_start"] >= a["t_done"] - 1e-6
+
+
+def test_concurrent_requests_overlap(dgx):
+    reqs = small_requests(dgx, 8, seed=4)
+    sv = dgx.Server(slots=4, hbm_budget=64e9, nlevels=4, keep_resident=False)
+    ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
+    res = [sv.wait(i) for i in ids]
+    sv.close()
+    overlaps = sum(1 for i in range(len(res)) for j in range(i + 1, len(res))
+                   if res[i]["t_start"] < res[j]["t_done"] and res[j]["t_start"] < res[i]["t_done"])
+    assert overlaps >= 1
+    assert all(r["status"] == 0 and r["rel_residual"] <= 1e-8 for r in res)
+
+
+def test_bad_requests_fail_loudly(dgx):
+    sv = dgx.Server(slots=1, hbm_budget=8e9, nlevels=3)
+    pts = np.full((100, 3), 1.5)  # outside the unit cube
+    rid = sv.submit(pts, 30, 1, 0)
+    with pytest.raises(dgx.DgxError):
+        sv.wait(rid)
+    sv.close()

@@ -61,3 +61,13 @@ def test_library_calls_fail_loudly_without_init():
     pts = np.zeros((4, 3))
     rc = L.dgx_points_to_octree(pts.ctypes.data_as(C.c_void_p), 4, 30, 1, 0, 0, C.byref(h))
     assert rc != 0 and b"dgx_init" in L.dgx_last_error()
+
+
+def test_server_requires_init():
+    import ctypes as C
+    import dendro4_amd as dgx
+    L = dgx.load_library()
+    o = dgx.ServerOpts(2, 1 << 30, 3, 1, 1e-8, 100, 0, 0)
+    h = C.c_void_p()
+    assert L.dgx_server_create(C.byref(o), C.byref(h)) != 0
+    assert b"dgx_init" in L.dgx_last_error()
