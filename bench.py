@@ -15,6 +15,10 @@ Also reported on the same line:
                 solve, error of the mixed solve against the FP64 one; on the
                 lognormal-4M octree and on gauss4-16M (16,777,216 points,
                 4 Gaussians sigma 0.02, seed 2, maxNumPts 4);
+  serve         a served stream of 2,000 Poisson problems (point counts
+                log-uniform in [2^12, 2^22], three distributions, maxNumPts
+                1/4/8, Poisson arrivals at 200/s, 200 GB HBM budget, problems
+                kept resident): solves/s and p50/p99 latency, FP64 and FP32-MG;
   cpu_baseline  the oracle (CPU restatement, one thread) timed on the first
                 --cpu-sample points of the headline workload.
 """
@@ -113,6 +117,43 @@ def solve_section(dgx, ob, nlevels, solves, warmup, seed, graph=True):
     return out
 
 
+def serve_section(dgx, count, slots, budget):
+    """Served stream: `count` requests (log-uniform 2^12..2^22 points, three
+    distributions, maxNumPts 1/4/8, seed 3) submitted at their Poisson arrival
+    times (200/s); latency = completion - arrival, on the server clock."""
+    reqs = dgx.serve_workload(count)
+    t0 = time.perf_counter()
+    clouds = [dgx.serve_points(r) for r in reqs]
+    t_gen = time.perf_counter() - t0
+    out = {"requests": count, "rate_per_s": 200.0, "slots": slots, "hbm_budget_GB": budget / 1e9,
+           "points_total": int(sum(r["n"] for r in reqs)), "generate_s": t_gen}
+    for name, fp32 in (("fp64", False), ("fp32_mg", True)):
+        sv = dgx.Server(slots=slots, hbm_budget=budget, nlevels=5, fp32=fp32, keep_resident=True)
+        start = time.perf_counter()
+        ids = []
+        for r, c in zip(reqs, clouds):
+            dt = r["arrival"] - (time.perf_counter() - start)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(sv.submit(c, 30, r["max_num_pts"], r["seed"]))
+        res = [sv.wait(i) for i in ids]
+        elapsed = time.perf_counter() - start
+        sv.close()
+        lat = np.array([x["t_done"] - x["t_submit"] for x in res])
+        run = np.array([x["t_done"] - x["t_start"] for x in res])
+        out[name] = {"solves_per_s": len(res) / elapsed, "elapsed_s": elapsed,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "service_p50_ms": float(np.percentile(run, 50) * 1e3),
+                     "service_p99_ms": float(np.percentile(run, 99) * 1e3),
+                     "max_rel_residual": float(max(x["rel_residual"] for x in res)),
+                     "iterations_max": int(max(x["iterations"] for x in res)),
+                     "octants_total": int(sum(x["octants"] for x in res))}
+        log(f"serve {name}: {json.dumps(out[name])}")
+    out["speedup_fp32_mg_vs_fp64"] = out["fp32_mg"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    del clouds
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +164,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Poisson solves")
     ap.add_argument("--no-large", action="store_true", help="skip the gauss4-16M solve")
     ap.add_argument("--no-graph", action="store_true", help="solve without hipGraph replay (for tracing runs)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the served stream (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--cpu-sample", type=int, default=524288, help="points in the oracle's CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--matvec-only", action="store_true",
@@ -224,6 +267,10 @@ def main():
             log(f"solve {W3}: {json.dumps(s3)}")
             del o3
 
+    serve = None
+    if args.serve_requests > 0:
+        serve = serve_section(dgx, args.serve_requests, args.serve_slots, 200e9)
+
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as O
@@ -265,6 +312,7 @@ def main():
         "build_balance": {"s_per_step": t_build, "p2o_s": t_p2o, "algorithmic_bytes": build_bytes,
                           "GB_per_s": build_bytes / t_build / 1e9},
         "solve": solve,
+        "serve": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

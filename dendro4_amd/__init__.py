@@ -73,7 +73,9 @@ class ServerOpts(C.Structure):
 class Served(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
                 ("octants", C.c_uint64), ("nodes", C.c_uint64), ("levels", C.c_int), ("reused", C.c_int),
-                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("peak_bytes", C.c_uint64), ("resident_bytes", C.c_uint64), ("s_octree", C.c_double),
+                ("s_hierarchy", C.c_double), ("s_solve", C.c_double)]
 
 
 class SolveStats(C.Structure):

@@ -4,8 +4,14 @@
 // each costs tens of microseconds of host time per pair, so freed blocks are
 // kept here and handed out again to the next request of the same size class
 // on the same stream.  Reuse is safe in stream order: a block freed on stream
-// s is only reused by work queued later on s.  Blocks beyond the cache cap,
-// and all cached blocks when an allocation fails, go back to the HIP pool.
+// s is only reused by work queued later on s.  Blocks beyond the cache cap go
+// back to the HIP pool on the stream they were freed on; when an allocation
+// fails, the blocks cached for the allocating stream are returned and the
+// allocation is retried.
+//
+// Accounting: every allocation carries the calling thread's current tag
+// (dgx_alloc_set_tag); live and peak bytes are kept per tag, whichever thread
+// frees a block, so a caller can measure what one piece of work holds.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -20,12 +26,24 @@ struct Cached {
     void *p;
     hipStream_t s;
 };
+struct Live {
+    size_t bytes;
+    hipStream_t s;
+    uint64_t tag;
+};
+struct TagUse {
+    int64_t live = 0, peak = 0;
+};
 
 std::mutex g_mu;
-std::unordered_map<void *, std::pair<size_t, hipStream_t>> g_live;  // pointer -> (class bytes, stream)
-std::multimap<size_t, Cached> g_cache;                              // class bytes -> free block
+std::unordered_map<void *, Live> g_live;
+std::unordered_map<uint64_t, TagUse> g_tags;  // tag 0 (untagged) is not tracked
+std::multimap<size_t, Cached> g_cache;        // class bytes -> free block
 size_t g_cached_bytes = 0;
-constexpr size_t kCacheCap = size_t(16) << 30;  // bytes held back from the pool at most
+constexpr size_t kCacheCap = size_t(8) << 30;  // bytes held back from the pool at most
+
+thread_local hipStream_t t_free_override = nullptr;  // see dgx_alloc_free_stream
+thread_local uint64_t t_tag = 0;                     // see dgx_alloc_set_tag
 
 size_t size_class(size_t b) {
     if (b == 0) b = 1;
@@ -34,10 +52,26 @@ size_t size_class(size_t b) {
     return (b + g - 1) / g * g;
 }
 
-void trim_locked() {
-    for (auto &kv : g_cache) (void)hipFreeAsync(kv.second.p, kv.second.s);
-    g_cache.clear();
-    g_cached_bytes = 0;
+void note_alloc_locked(void *p, size_t cls, hipStream_t s) {
+    g_live[p] = Live{cls, s, t_tag};
+    if (t_tag) {
+        TagUse &u = g_tags[t_tag];
+        u.live += (int64_t)cls;
+        if (u.live > u.peak) u.peak = u.live;
+    }
+}
+
+// Return the blocks cached for stream s to the pool (queued on s).
+void trim_stream_locked(hipStream_t s) {
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+        if (it->second.s == s) {
+            (void)hipFreeAsync(it->second.p, s);
+            g_cached_bytes -= it->first;
+            it = g_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
 }
 
 }  // namespace
@@ -50,7 +84,7 @@ void *dgx_alloc(size_t bytes, hipStream_t s) {
         for (auto it = g_cache.lower_bound(cls); it != g_cache.end() && it->first <= cls + cls / 4; ++it) {
             if (it->second.s != s) continue;
             void *p = it->second.p;
-            g_live[p] = {it->first, s};
+            note_alloc_locked(p, it->first, s);
             g_cached_bytes -= it->first;
             g_cache.erase(it);
             return p;
@@ -62,17 +96,18 @@ void *dgx_alloc(size_t bytes, hipStream_t s) {
         (void)hipGetLastError();
         {
             std::lock_guard<std::mutex> lk(g_mu);
-            trim_locked();
+            trim_stream_locked(s);
         }
         (void)hipStreamSynchronize(s);
         e = hipMallocAsync(&p, cls, s);
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             dgx_set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
             throw DgxError();
         }
     }
     std::lock_guard<std::mutex> lk(g_mu);
-    g_live[p] = {cls, s};
+    note_alloc_locked(p, cls, s);
     return p;
 }
 
@@ -81,11 +116,18 @@ void dgx_free(void *p, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_live.find(p);
     if (it == g_live.end()) {  // not ours
-        (void)hipFreeAsync(p, s);
+        (void)hipFreeAsync(p, t_free_override ? t_free_override : s);
         return;
     }
-    const size_t cls = it->second.first;
-    const hipStream_t as = it->second.second;
+    const size_t cls = it->second.bytes;
+    const hipStream_t as = t_free_override ? t_free_override : it->second.s;
+    if (it->second.tag) {
+        auto t = g_tags.find(it->second.tag);
+        if (t != g_tags.end()) {
+            t->second.live -= (int64_t)cls;
+            if (t->second.live <= 0) g_tags.erase(t);
+        }
+    }
     g_live.erase(it);
     if (g_cached_bytes + cls > kCacheCap) {
         (void)hipFreeAsync(p, as);
@@ -97,7 +139,9 @@ void dgx_free(void *p, hipStream_t s) {
 
 void dgx_alloc_trim() {
     std::lock_guard<std::mutex> lk(g_mu);
-    trim_locked();
+    for (auto &kv : g_cache) (void)hipFreeAsync(kv.second.p, kv.second.s);
+    g_cache.clear();
+    g_cached_bytes = 0;
 }
 
 size_t dgx_alloc_cached_bytes() {
@@ -107,15 +151,32 @@ size_t dgx_alloc_cached_bytes() {
 
 void dgx_alloc_release_stream(hipStream_t s) {
     (void)hipStreamSynchronize(s);
-    std::lock_guard<std::mutex> lk(g_mu);
-    for (auto it = g_cache.begin(); it != g_cache.end();) {
-        if (it->second.s == s) {
-            (void)hipFreeAsync(it->second.p, s);
-            g_cached_bytes -= it->first;
-            it = g_cache.erase(it);
-        } else {
-            ++it;
-        }
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        trim_stream_locked(s);
     }
     (void)hipStreamSynchronize(s);
+}
+
+hipStream_t dgx_alloc_free_stream(hipStream_t s) {
+    hipStream_t old = t_free_override;
+    t_free_override = s;
+    return old;
+}
+hipStream_t dgx_alloc_free_stream_get() { return t_free_override; }
+
+uint64_t dgx_alloc_set_tag(uint64_t tag) {
+    uint64_t old = t_tag;
+    t_tag = tag;
+    return old;
+}
+int64_t dgx_alloc_tag_live(uint64_t tag) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tags.find(tag);
+    return it == g_tags.end() ? 0 : it->second.live;
+}
+int64_t dgx_alloc_tag_peak(uint64_t tag) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tags.find(tag);
+    return it == g_tags.end() ? 0 : it->second.peak;
 }

@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <csignal>
+#include <cstdlib>
+#include <execinfo.h>
+#include <unistd.h>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -82,7 +86,23 @@ struct dgx_mg_s {
 
 extern "C" {
 
+// With DGX_SEGV_BACKTRACE set, a fatal signal prints the native stack to
+// stderr before the default action (diagnostics for host-side crashes).
+static void segv_backtrace(int sig) {
+    void *fr[64];
+    const int n = backtrace(fr, 64);
+    const char msg[] = "dendro_mi355x: fatal signal, native backtrace:\n";
+    (void)!write(2, msg, sizeof msg - 1);
+    backtrace_symbols_fd(fr, n, 2);
+    signal(sig, SIG_DFL);
+    raise(sig);
+}
+
 int dgx_init(int device) {
+    if (std::getenv("DGX_SEGV_BACKTRACE")) {
+        signal(SIGSEGV, segv_backtrace);
+        signal(SIGABRT, segv_backtrace);
+    }
     std::lock_guard<std::mutex> lk(g_init_mu);
     API_BEGIN
     if (g_ctx.ready) {
@@ -101,7 +121,11 @@ int dgx_init(int device) {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     HIP_OK(hipDeviceGetDefaultMemPool(&g_ctx.pool, device));
-    uint64_t thr = UINT64_MAX;
+    // The pool keeps up to 4 GiB of freed memory mapped; beyond that it returns
+    // memory to the device at synchronisation points, so the runtime's own
+    // allocations (kernel scratch, library workspaces) still find room.  The
+    // library's caching allocator absorbs the churn of temporaries.
+    uint64_t thr = uint64_t(4) << 30;
     HIP_OK(hipMemPoolSetAttribute(g_ctx.pool, hipMemPoolAttrReleaseThreshold, &thr));
     g_ctx.device = device;
     g_ctx.ready = true;

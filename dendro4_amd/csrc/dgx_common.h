@@ -258,6 +258,18 @@ size_t dgx_alloc_cached_bytes();  // bytes held in the cache
 // Return the cached blocks of stream s to the pool before s is destroyed
 // (no live allocation may still be tagged with s).
 void dgx_alloc_release_stream(hipStream_t s);
+// While set on this thread, frees are queued on / cached for stream s instead
+// of the stream a block was allocated on (for objects whose work has finished
+// and whose own stream may be in use by another thread, e.g. under graph
+// capture).  Returns the previous setting.
+hipStream_t dgx_alloc_free_stream(hipStream_t s);
+hipStream_t dgx_alloc_free_stream_get();
+// Accounting tags: allocations made while a thread's tag is set are counted
+// under it (live and peak bytes), whichever thread frees them; tag 0 = none.
+// The counters of a tag disappear when its live bytes return to zero.
+uint64_t dgx_alloc_set_tag(uint64_t tag);  // returns the previous tag
+int64_t dgx_alloc_tag_live(uint64_t tag);
+int64_t dgx_alloc_tag_peak(uint64_t tag);
 
 template <typename T>
 struct DevBuf {

@@ -198,8 +198,10 @@ __device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t
     const bool wave_hanging = __ballot(valid && mask != 0u) != 0ull;
     if (!valid) return;
     T u[8];
+    // x slots are sizeof(T) bytes, accumulators 8 bytes: off[] addresses the latter
+    constexpr int XSH = sizeof(T) == 4 ? 1 : 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) u[j] = *reinterpret_cast<const T *>(xb + off[j]);
+    for (int j = 0; j < 8; ++j) u[j] = *reinterpret_cast<const T *>(xb + (off[j] >> XSH));
     bool h1 = false, h2 = false, h4 = false, h3 = false, h5 = false, h6 = false;
     if (wave_hanging) {
         // hanging vertices in the child frame: edge midpoints 1,2,4 (mean of
@@ -245,22 +247,26 @@ __device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t
     for (int j = 0; j < 8; ++j) atomicAdd(reinterpret_cast<double *>(yb + off[j]), (double)u[j]);
 }
 
+// The element record holds 8-byte slot offsets; SH = 1 turns them into
+// 4-byte slot offsets (FP32 slots).
+template <int SH>
 __device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, uint32_t *off) {
     const uint4 v = *reinterpret_cast<const uint4 *>(lconn + e * 8);
-    off[0] = v.x & 0xffffu;
-    off[1] = v.x >> 16;
-    off[2] = v.y & 0xffffu;
-    off[3] = v.y >> 16;
-    off[4] = v.z & 0xffffu;
-    off[5] = v.z >> 16;
-    off[6] = v.w & 0xffffu;
-    off[7] = v.w >> 16;
+    off[0] = (v.x & 0xffffu) >> SH;
+    off[1] = (v.x >> 16) >> SH;
+    off[2] = (v.y & 0xffffu) >> SH;
+    off[3] = (v.y >> 16) >> SH;
+    off[4] = (v.z & 0xffffu) >> SH;
+    off[5] = (v.z >> 16) >> SH;
+    off[6] = (v.w & 0xffffu) >> SH;
+    off[7] = (v.w >> 16) >> SH;
 }
 
 // Pass 1, one workgroup per element block (any block size up to MAXN nodes).
 // Element data are loaded first, the block's nodes are gathered into LDS,
 // element contributions are accumulated there in FP64 (ds_add_f64 for both
-// precisions), and each node sum goes to y (node touched by this block only)
+// precisions: on gfx950 it runs well ahead of ds_add_f32), and each node sum
+// goes to y (node touched by this block only)
 // or to its partial slot (shared node).  Node entries: bits 0-29 node id,
 // bit 30 Dirichlet boundary, bit 31 shared.  With DIR, boundary columns are
 // dropped and boundary rows are the identity.  LDS: x slots (8 bytes each)
@@ -274,8 +280,8 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
                                                   const T *__restrict__ x, T *__restrict__ y, T *__restrict__ partial,
                                                   double lap_fac, double mass_fac) {
     constexpr int EB = PER * B;
-    __shared__ __align__(16) double xs[MAXN];  // x of the block's nodes (T in the low bytes)
-    __shared__ __align__(16) double ys[MAXN];  // FP64 accumulators
+    __shared__ __align__(16) T xs[MAXN];       // x of the block's nodes
+    __shared__ __align__(16) double ys[MAXN];  // FP64 accumulators (ds_add_f64 outruns ds_add_f32 here)
     __shared__ T lam[32][4];
     unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
     unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
@@ -287,14 +293,14 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
         const size_t e = e0 + threadIdx.x + k * B;
         mt[k] = 0;
         if (e < ne) {
-            unpack_offsets(lconn, e, off[k]);
+            unpack_offsets<0>(lconn, e, off[k]);
             mt[k] = cmeta[e];
         }
     }
     init_lambda<T>(lam, lap_fac, mass_fac);
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
-        *reinterpret_cast<T *>(xb + 8 * l) = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
+        xs[l] = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
         ys[l] = 0.0;
     }
     __syncthreads();
@@ -325,7 +331,7 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
                                                    T *__restrict__ y, T *__restrict__ partial, double lap_fac,
                                                    double mass_fac) {
     constexpr int EB = PER * B, MAXN = NPT * B;
-    __shared__ __align__(16) double xs[MAXN];
+    __shared__ __align__(16) T xs[MAXN];
     __shared__ __align__(16) double ys[MAXN];
     __shared__ T lam[32][4];
     unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
@@ -347,7 +353,7 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
             const size_t e = e0 + threadIdx.x + k * B;
             m[k] = 0;
             if (e < ne) {
-                unpack_offsets(lconn, e, o[k]);
+                unpack_offsets<0>(lconn, e, o[k]);
                 m[k] = cmeta[e];
             }
         }
@@ -374,7 +380,7 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
-                *reinterpret_cast<T *>(xb + 8 * l) = xv[i];
+                xs[l] = xv[i];
                 ys[l] = 0.0;
             }
         }

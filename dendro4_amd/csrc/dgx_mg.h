@@ -45,6 +45,9 @@ struct MgHierarchy {
     ~MgHierarchy();
 };
 
+// The caller has waited for every solve on this hierarchy: later solves on
+// other streams need not wait for the previous stream.
+void mg_mark_idle(MgHierarchy &mg);
 // Changes the operator factors: recomputes the level diagonals and the
 // coarse inverse, drops captured graphs.
 void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s);

@@ -137,21 +137,13 @@ __global__ void __launch_bounds__(B) k_prolong_rows(uint32_t nf, const uint32_t 
 }
 __global__ void k_prolong_keys(const uint32_t *pcol, uint32_t nf, Key *keys, uint32_t *vals, uint32_t *cnt) {
     uint32_t n = blockIdx.x * B + threadIdx.x;
-    uint32_t nloc = 0;
-    Key loc[8];
-    uint32_t lv[8];
+    uint32_t nloc = 0;  // rows are filled from slot 0; NONE ends them
     if (n < nf)
-        for (int k = 0; k < 8; ++k) {
-            uint32_t c = pcol[(size_t)k * nf + n];
-            if (c == NONE) break;
-            loc[nloc] = Key{(uint64_t)c, (uint64_t)n};
-            lv[nloc] = (uint32_t)k * nf + n;
-            ++nloc;
-        }
-    uint32_t base = wave_reserve(cnt, nloc);
-    for (uint32_t j = 0; j < nloc; ++j) {
-        keys[base + j] = loc[j];
-        vals[base + j] = lv[j];
+        while (nloc < 8 && pcol[(size_t)nloc * nf + n] != NONE) ++nloc;
+    uint32_t base = wave_reserve(cnt, nloc);  // every lane of the wave reaches this call
+    for (uint32_t k = 0; k < nloc; ++k) {
+        keys[base + k] = Key{(uint64_t)pcol[(size_t)k * nf + n], (uint64_t)n};
+        vals[base + k] = k * nf + n;
     }
 }
 __global__ void k_restrict_csr(const Key *keys, const uint32_t *vals, uint32_t nnz, const double *pval, uint32_t *rcol,
@@ -469,7 +461,9 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     HIP_OK(hipMemcpyAsync(L0.keys.p, finest, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
     L0.n_oct = n;
     L0.da = da_build(L0.keys.p, n, md, curve, s);
-    const uint32_t kMaxDense = 12288;
+    // the coarsest level must fit the dense FP64 coarse solve; its Cholesky and
+    // inverse cost n^3, so coarsening goes on until it has at most 2048 nodes
+    const uint32_t kMaxDense = 2048;
     for (int k = 1; k < 30; ++k) {
         // nlevels is honoured, but the hierarchy keeps coarsening while the
         // coarsest level is too large for the dense FP64 coarse solve
@@ -619,10 +613,14 @@ struct MgWork {
     // Only that stream is waited for: a device-wide wait would break graph
     // captures running on other streams (request server).
     ~MgWork() {
-        if (last) (void)hipStreamSynchronize(last);
+        // when frees are redirected the owner has already waited for this work
+        if (last && !dgx_alloc_free_stream_get()) (void)hipStreamSynchronize(last);
     }
 };
 MgHierarchy::~MgHierarchy() { delete work; }
+void mg_mark_idle(MgHierarchy &mg) {
+    if (mg.work) mg.work->last = nullptr;
+}
 void mg_reset_graphs(MgHierarchy &mg) {
     if (!mg.work) return;
     if (mg.work->last) HIP_OK(hipStreamSynchronize(mg.work->last));

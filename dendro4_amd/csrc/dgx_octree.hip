@@ -344,29 +344,34 @@ __global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32
 __global__ void k_bal_nbr_parents(const Key *Q, const uint32_t *d_nq, uint32_t lvl, uint32_t md, Key *out,
                                   uint32_t *counter) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    Key loc[8];
-    int nl = 0;
+    // per axis: the octant's own parent coordinate and the neighbouring one
+    int64_t cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
+    uint32_t valid = 0;  // bit (bz*4 + by*2 + bx): candidate parent inside the domain
     if (i < *d_nq) {
         uint32_t x, y, z, l;
         morton_decode(Q[i], md, x, y, z, l);
         const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
-        int64_t c[3][2];
-        const int64_t v[3] = {x, y, z};
-        for (int a = 0; a < 3; ++a) {
-            int64_t own = (v[a] / plen) * plen;
-            c[a][0] = own;
-            c[a][1] = (v[a] % plen) ? own + plen : own - plen;
-        }
-        for (int bz = 0; bz < 2; ++bz)
-            for (int by = 0; by < 2; ++by)
-                for (int bx = 0; bx < 2; ++bx) {
-                    int64_t px = c[0][bx], py = c[1][by], pz = c[2][bz];
-                    if (px < 0 || py < 0 || pz < 0 || px >= dom || py >= dom || pz >= dom) continue;
-                    loc[nl++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
-                }
+        cx0 = ((int64_t)x / plen) * plen;
+        cy0 = ((int64_t)y / plen) * plen;
+        cz0 = ((int64_t)z / plen) * plen;
+        cx1 = ((int64_t)x % plen) ? cx0 + plen : cx0 - plen;
+        cy1 = ((int64_t)y % plen) ? cy0 + plen : cy0 - plen;
+        cz1 = ((int64_t)z % plen) ? cz0 + plen : cz0 - plen;
+        const uint32_t okx = 1u | ((cx1 >= 0 && cx1 < dom) ? 2u : 0u);
+        const uint32_t oky = 1u | ((cy1 >= 0 && cy1 < dom) ? 2u : 0u);
+        const uint32_t okz = 1u | ((cz1 >= 0 && cz1 < dom) ? 2u : 0u);
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if (((okx >> (t & 1)) & 1u) && ((oky >> ((t >> 1) & 1)) & 1u) && ((okz >> ((t >> 2) & 1)) & 1u))
+                valid |= 1u << t;
     }
-    uint32_t base = wave_reserve(counter, (uint32_t)nl);  // every lane of the wave reaches this call
-    for (int j = 0; j < nl; ++j) out[base + j] = loc[j];
+    uint32_t base = wave_reserve(counter, (uint32_t)__popc(valid));  // every lane of the wave reaches this call
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+        if ((valid >> t) & 1u) {
+            const int64_t px = (t & 1) ? cx1 : cx0, py = (t & 2) ? cy1 : cy0, pz = (t & 4) ? cz1 : cz0;
+            out[base++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
+        }
 }
 
 // For every split octant at the finer level, mark its child bit in the parent (Qp sorted unique).

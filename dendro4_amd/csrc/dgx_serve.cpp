@@ -14,6 +14,7 @@
 // Resident problems are evicted, oldest first, when admission needs room.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -69,7 +70,7 @@ uint64_t hash_request(const double *xyz, size_t n, unsigned md, unsigned m) {
 }
 
 struct Request {
-    uint64_t id = 0, key = 0, seed = 0, est = 0;
+    uint64_t id = 0, key = 0, seed = 0, est = 0, npts = 0;
     std::vector<double> xyz;
     unsigned md = 30, mnp = 1;
     double t_submit = 0;
@@ -107,9 +108,16 @@ struct dgx_server_s {
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 
-    // bytes a request may hold on the device: points, keys, the octree, the
-    // DA maps and the multigrid hierarchy scale with the number of points
-    static uint64_t estimate(size_t n) { return (uint64_t)n * 1600ull + (64ull << 20); }
+    // Bytes a request may hold on the device (points, keys, octree, DA maps,
+    // hierarchy and their temporaries scale with the number of octants, i.e.
+    // with the number of points over maxNumPts): the largest peak per point
+    // seen so far for the request's maxNumPts class (1, 2-4, more), starting
+    // from 14000 / 6000 / 3000 B, plus 64 MiB.
+    double bytes_per_point[3] = {14000.0, 6000.0, 3000.0};  // under mu
+    static int mclass(unsigned m) { return m <= 1 ? 0 : (m <= 4 ? 1 : 2); }
+    uint64_t estimate(size_t n, unsigned m) const {
+        return (uint64_t)((double)n * bytes_per_point[mclass(m)]) + (64ull << 20);
+    }
 
     // Under mu: make room for `need` by evicting idle resident problems,
     // least recently used first.
@@ -130,6 +138,10 @@ struct dgx_server_s {
         (void)hipSetDevice(device);
         hipStream_t s = nullptr;
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
+        // Resident problems evicted by this thread were built on other slots'
+        // streams, which may be capturing a graph right now: their (finished)
+        // memory is returned on this thread's stream instead.
+        dgx_alloc_free_stream(s);
         for (;;) {
             Request rq;
             {
@@ -150,8 +162,11 @@ struct dgx_server_s {
             d.r.t_submit = rq.t_submit;
             d.r.t_start = now();
             uint64_t keep_bytes = 0;
+            // device memory of this request is counted under its id
+            const uint64_t tag = (1ull << 63) | rq.id;
+            dgx_alloc_set_tag(tag);
             try {
-                run(rq, s, d, keep_bytes);
+                run(rq, s, d, keep_bytes, tag);
             } catch (DgxError &) {
                 d.r.status = -1;
                 d.err = dgx_last_error();
@@ -159,23 +174,29 @@ struct dgx_server_s {
                 d.r.status = -1;
                 d.err = e.what();
             }
+            dgx_alloc_set_tag(0);
             (void)hipStreamSynchronize(s);
             d.r.t_done = now();
+            d.r.resident_bytes = keep_bytes;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 reserved -= rq.est;
                 reserved += keep_bytes;
+                if (rq.npts >= 65536 && d.r.peak_bytes > 0)  // small ones are covered by the 64 MiB term
+                    bytes_per_point[mclass(rq.mnp)] =
+                        std::max(bytes_per_point[mclass(rq.mnp)], (double)d.r.peak_bytes / (double)rq.npts);
                 done[rq.id] = std::move(d);
             }
             cv_done.notify_all();
             cv_work.notify_all();
         }
         (void)hipStreamSynchronize(s);
+        dgx_alloc_free_stream(nullptr);
         std::lock_guard<std::mutex> lk(mu);
         streams.push_back(s);  // destroyed by the server after the resident problems
     }
 
-    void run(Request &rq, hipStream_t s, Done &d, uint64_t &keep_bytes) {
+    void run(Request &rq, hipStream_t s, Done &d, uint64_t &keep_bytes, uint64_t tag) {
         // resident hierarchy for this request?
         rq.key = hash_request(rq.xyz.data(), rq.xyz.size() / 3, rq.md, rq.mnp);
         Resident *res = nullptr;
@@ -191,12 +212,14 @@ struct dgx_server_s {
         }
         std::unique_ptr<MgHierarchy> own;
         MgHierarchy *mg = nullptr;
+
         uint64_t octants = 0;
         if (res) {
             mg = res->mg.get();
             octants = res->octants;
             d.r.reused = 1;
         } else {
+            const double ta = now();
             const size_t n = rq.xyz.size() / 3;
             DevBuf<double> dxyz(3 * n, s);
             if (n) HIP_OK(hipMemcpyAsync(dxyz.p, rq.xyz.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
@@ -205,10 +228,17 @@ struct dgx_server_s {
             DevBuf<Key> b = balance_build(u.p, u.n, rq.md, DGX_CURVE_HILBERT, s, nullptr);
             u.release();
             octants = b.n;
+            HIP_OK(hipStreamSynchronize(s));
+            const double tb = now();
+            d.r.s_octree = tb - ta;
             own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s);
             mg = own.get();
+            b.release();
+            HIP_OK(hipStreamSynchronize(s));
+            d.r.s_hierarchy = now() - tb;
         }
         std::vector<double>().swap(rq.xyz);
+        const double tc = now();
         const uint32_t nn = mg->lev[0].da->n_nodes;
         DevBuf<double> f(nn, s), bvec(nn, s), x(nn, s);
         if (nn) {
@@ -221,7 +251,7 @@ struct dgx_server_s {
         so.rtol = o.rtol;
         so.maxit = o.maxit;
         SolveStats st;
-        mg_solve(*mg, bvec.p, x.p, so, &st, s);
+        mg_solve(*mg, bvec.p, x.p, so, &st, s);  // allocates the hierarchy's work vectors once
         d.r.iterations = st.iterations;
         d.r.rel_residual = st.rel_residual;
         d.r.octants = octants;
@@ -232,17 +262,25 @@ struct dgx_server_s {
             HIP_OK(hipMemcpyAsync(d.x.data(), x.p, (size_t)nn * sizeof(double), hipMemcpyDeviceToHost, s));
         }
         HIP_OK(hipStreamSynchronize(s));
+        d.r.s_solve = now() - tc;
+        mg_mark_idle(*mg);
+        d.r.peak_bytes = (uint64_t)std::max<int64_t>(0, dgx_alloc_tag_peak(tag));
+        f.release();
+        bvec.release();
+        x.release();
+        // what is still live under this request's tag is the new hierarchy
+        const int64_t hier_bytes = res ? 0 : dgx_alloc_tag_live(tag);
         std::lock_guard<std::mutex> lk(mu);
         if (res) {
             res->busy = false;
         } else if (o.keep_resident) {
             Resident r;
             r.key = rq.key;
-            r.bytes = rq.est;
+            r.bytes = (uint64_t)std::max<int64_t>(hier_bytes, 1 << 20);  // measured footprint of the hierarchy
             r.octants = octants;
             r.mg = std::move(own);
+            keep_bytes = r.bytes;
             cache.push_front(std::move(r));
-            keep_bytes = rq.est;
         }
     }
 };
@@ -263,8 +301,15 @@ int dgx_server_create(const dgx_server_opts *o, dgx_server *out) {
         dgx_set_error("server: bad options");
         return -1;
     }
+    size_t dfree = 0, dtotal = 0;
+    if (hipMemGetInfo(&dfree, &dtotal) != hipSuccess) {
+        dgx_set_error("server: hipMemGetInfo failed");
+        return -1;
+    }
     auto *sv = new dgx_server_s();
     sv->o = *o;
+    // leave a quarter of the device to the runtime, libraries and caches
+    sv->o.hbm_budget = std::min<uint64_t>(o->hbm_budget, dtotal / 4 * 3);
     sv->device = dev;
     sv->t0 = std::chrono::steady_clock::now();
     for (unsigned i = 0; i < o->slots; ++i) sv->workers.emplace_back(&dgx_server_s::worker, sv, i);
@@ -284,10 +329,14 @@ int dgx_server_submit(dgx_server sv, const double *xyz, size_t n, unsigned max_d
     }
     Request rq;
     rq.xyz.assign(xyz, xyz + 3 * n);
+    rq.npts = n;
     rq.md = max_depth;
     rq.mnp = max_num_pts;
     rq.seed = rhs_seed;
-    rq.est = dgx_server_s::estimate(n);
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        rq.est = sv->estimate(n, max_num_pts);
+    }
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         rq.id = sv->next_id++;

@@ -259,7 +259,7 @@ void scan_exclusive_u32_async(const uint32_t *in, uint32_t *out, size_t n, uint3
         if (d_total) HIP_OK(hipMemsetAsync(d_total, 0, sizeof(uint32_t), s));
         return;
     }
-    if (n <= (size_t)SS_B * SS_ITEMS * 4) {
+    if (n <= (size_t)SS_B * SS_ITEMS) {  // one round of the single-workgroup scan
         k_scan_single<<<1, SS_B, 0, s>>>(in, out, n, d_total);
         HIP_OK(hipGetLastError());
         return;

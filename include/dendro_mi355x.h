@@ -129,8 +129,8 @@ typedef struct dgx_mg_s *dgx_mg;
  * blockDiag.cpp:780-1047, 1312-2190) on one rank: nlevels levels built by
  * coarsenOctree + balanceOctree from the finest octree (stopping early below
  * 500 octants, blockDiag.cpp:1417-1420; coarsening continues past nlevels
- * while the coarsest level has more than 12288 nodes, so that the dense FP64
- * coarse solve fits), a DA per level, interpolation P and
+ * while the coarsest level has more than 2048 nodes, so that the dense FP64
+ * coarse solve stays cheap), a DA per level, interpolation P and
  * restriction R = P^T between levels, Jacobi diagonals and the coarsest-level
  * FP64 direct solver.  The octree is not consumed. */
 int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fac, dgx_mg *out);
@@ -193,6 +193,9 @@ typedef struct {
     int levels;
     int reused;            /* 1: the hierarchy was resident */
     double t_submit, t_start, t_done; /* seconds on the server clock */
+    uint64_t peak_bytes;     /* peak device memory the request held */
+    uint64_t resident_bytes; /* device memory of the problem kept resident (0 if not kept) */
+    double s_octree, s_hierarchy, s_solve; /* seconds in the octree build, the MG set-up and the solve */
 } dgx_served;
 int dgx_server_create(const dgx_server_opts *opts, dgx_server *out);
 /* Copies the n points; returns at once with the request id. */

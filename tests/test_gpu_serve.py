@@ -57,8 +57,10 @@ def test_resident_problem_is_reused(dgx):
 
 def test_budget_gates_admission(dgx):
     reqs = small_requests(dgx, 6, seed=3)
-    est = max(r["n"] for r, _ in reqs) * 1600 + (64 << 20)
-    sv = dgx.Server(slots=4, hbm_budget=est + 1, nlevels=4, keep_resident=False)
+    # every request's estimate (>= 64 MiB + 3000 B per point) exceeds this
+    # budget, so a request is admitted only when nothing else holds memory
+    budget = (64 << 20) + min(r["n"] for r, _ in reqs) * 3000
+    sv = dgx.Server(slots=4, hbm_budget=budget, nlevels=4, keep_resident=False)
     ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
     res = sorted((sv.wait(i) for i in ids), key=lambda x: x["t_start"])
     sv.close()

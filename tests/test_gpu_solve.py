@@ -122,3 +122,18 @@ def test_graph_replay_matches_eager(dgx, problem):
             u = du.to_host()
             assert np.linalg.norm(u - u0) / np.linalg.norm(u0) <= 1e-9, (fp32, rep)
             assert st["rel_residual"] <= 1e-8
+
+
+def test_fp32_matvec_matches_fp64_to_single_precision(dgx, problem):
+    """The FP32 operator of the V-cycle (FP32 LDS slots and accumulation)
+    against the FP64 operator on the same DA, both plain and Dirichlet."""
+    ob, rda, A, bd = problem
+    da = dgx.DA(ob)
+    x = np.random.default_rng(2).standard_normal(da.n_nodes)
+    y64 = da.matvec_host(x)
+    dx = dgx.DeviceArray.from_host(x.astype(np.float32))
+    dy = dgx.DeviceArray((da.n_nodes,), np.float32)
+    da.matvec(dx, dy)
+    y32 = dy.to_host().astype(np.float64)
+    assert np.abs(y32 - y64).max() / np.abs(y64).max() < 1e-5
+    assert np.abs(y64 - A @ x).max() / np.abs(A @ x).max() < 1e-12
