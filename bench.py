@@ -15,7 +15,8 @@ Also reported on the same line:
                 solve, error of the mixed solve against the FP64 one; on the
                 lognormal-4M octree and on gauss4-16M (16,777,216 points,
                 4 Gaussians sigma 0.02, seed 2, maxNumPts 4);
-  serve         a served stream of 2,000 Poisson problems (point counts
+  serve         a served stream of Poisson problems: the first --serve-requests
+                (default 400) of a 2,000-request stream (point counts
                 log-uniform in [2^12, 2^22], three distributions, maxNumPts
                 1/4/8, Poisson arrivals at 200/s, 200 GB HBM budget, problems
                 kept resident): solves/s and p50/p99 latency, FP64 and FP32-MG;
@@ -121,7 +122,7 @@ def serve_section(dgx, count, slots, budget):
     """Served stream: `count` requests (log-uniform 2^12..2^22 points, three
     distributions, maxNumPts 1/4/8, seed 3) submitted at their Poisson arrival
     times (200/s); latency = completion - arrival, on the server clock."""
-    reqs = dgx.serve_workload(count)
+    reqs = dgx.serve_workload(2000)[:count]
     t0 = time.perf_counter()
     clouds = [dgx.serve_points(r) for r in reqs]
     t_gen = time.perf_counter() - t0
@@ -131,12 +132,18 @@ def serve_section(dgx, count, slots, budget):
         sv = dgx.Server(slots=slots, hbm_budget=budget, nlevels=5, fp32=fp32, keep_resident=True)
         start = time.perf_counter()
         ids = []
-        for r, c in zip(reqs, clouds):
+        for k, (r, c) in enumerate(zip(reqs, clouds)):
             dt = r["arrival"] - (time.perf_counter() - start)
             if dt > 0:
                 time.sleep(dt)
             ids.append(sv.submit(c, 30, r["max_num_pts"], r["seed"]))
-        res = [sv.wait(i) for i in ids]
+            if (k + 1) % 500 == 0:
+                log(f"serve {name}: {k + 1}/{count} submitted")
+        res = []
+        for k, i in enumerate(ids):
+            res.append(sv.wait(i))
+            if (k + 1) % 100 == 0:
+                log(f"serve {name}: {k + 1}/{count} done at {time.perf_counter() - start:.1f} s")
         elapsed = time.perf_counter() - start
         sv.close()
         lat = np.array([x["t_done"] - x["t_submit"] for x in res])
@@ -164,8 +171,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Poisson solves")
     ap.add_argument("--no-large", action="store_true", help="skip the gauss4-16M solve")
     ap.add_argument("--no-graph", action="store_true", help="solve without hipGraph replay (for tracing runs)")
-    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the served stream (0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests of the served stream, a prefix of the 2,000-request stream (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=524288, help="points in the oracle's CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--matvec-only", action="store_true",
