@@ -26,9 +26,10 @@ struct DaDevice {
     //   of block b holds lconn[(b*elem_block + k)*8 + j] = 8 x (local index of vertex
     //   j ^ c in the block's list), c the child number -- the "child frame", where
     //   the parent corner is vertex 0 -- and cmeta[b*elem_block + k] = hanging mask
-    //   in the child frame | original level << 8.
+    //   in the child frame | original level << 8.  Both hold whole blocks (the slots past
+    //   n_elem in the last block are allocated and never used).
     //   shared nodes: snode[s], partial slots soff[s]..soff[s+1] (ordered by block).
-    uint32_t elem_block = 512;  // elements per block (8 x elem_block byte offsets must fit 16 bits)
+    uint32_t elem_block = 512;  // elements per block (a block may touch at most 8191 nodes)
     uint32_t n_blocks = 0, max_block_nodes = 0, n_shared = 0;
     uint64_t n_pairs = 0, partial_slots = 0;
     DevBuf<uint32_t> bnode_off, bnode, bdst, soff, snode;

@@ -528,6 +528,94 @@ __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
     uint32_t v = ids[p];
     if (bdy[v & 0x3fffffffu]) ids[p] = v | 0x40000000u;
 }
+// Node numbering inside a block against LDS bank conflicts.  The matvec adds
+// element results into the block's FP64 LDS slots with ds_add_f64, which
+// serves 16 lanes per pass and runs without conflicts (8 clocks per wave
+// instruction on gfx950) only when their slots differ mod 16; an unordered
+// numbering costs about 3x that.  The 16 lanes of one pass are 16 consecutive
+// element slots of the block at one record position p, a "lane group".  The
+// block's nodes are coloured greedily with 16 colours (colour = slot mod 16):
+// each node, in list order, takes the colour used least so far by the lane
+// groups it appears in (ties rotate with the node index), colour c holding
+// exactly n/16 + (c < n mod 16) nodes, so that slot = 16 x (rank inside its
+// colour) + colour is again a permutation of 0..n-1.  The records and the
+// node list are then renumbered.  Deterministic; one wave per block.  Blocks
+// past this kernel's LDS tables (more than kColMaxN nodes or kColMaxE
+// elements) keep the sorted numbering.
+constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024;
+__global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_t *boff, uint32_t *bnode, size_t ne,
+                                                  uint32_t eb) {
+    __shared__ uint32_t deg[kColMaxN];             // degree, then fill position
+    __shared__ uint16_t ptr[kColMaxN + 1];         // adjacency offsets
+    __shared__ uint16_t adj[8 * kColMaxE];         // lane groups of each node's uses
+    __shared__ uint8_t cnt[(kColMaxE / 16) * 8 * 16];  // [lane group][colour] uses so far
+    __shared__ uint16_t newidx[kColMaxN];
+    __shared__ uint32_t tmp[kColMaxN];
+    __shared__ uint32_t used[16];
+    const uint32_t b = blockIdx.x, lane = threadIdx.x;
+    const uint32_t s0 = boff[b], n = boff[b + 1] - s0;
+    const size_t e0 = (size_t)b * eb;
+    if (n > kColMaxN || eb > kColMaxE || e0 >= ne) return;  // uniform over the wave
+    const uint32_t nbe = (uint32_t)min((size_t)eb, ne - e0), nuse = 8 * nbe;
+    const uint32_t ngroups = ((nbe + 15) / 16) * 8;
+    uint16_t *rec = lconn + e0 * 8;
+    for (uint32_t i = lane; i < n; i += 64) deg[i] = 0;
+    for (uint32_t i = lane; i < ngroups * 16; i += 64) cnt[i] = 0;
+    if (lane < 16) used[lane] = 0;
+    __syncthreads();
+    for (uint32_t t = lane; t < nuse; t += 64) atomicAdd(&deg[rec[t] >> 3], 1u);
+    __syncthreads();
+    {  // exclusive scan of the degrees, one chunk per lane
+        const uint32_t ch = (n + 63) / 64, i0 = min(n, lane * ch), i1 = min(n, i0 + ch);
+        uint32_t sum = 0;
+        for (uint32_t i = i0; i < i1; ++i) sum += deg[i];
+        uint32_t incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += v;
+        }
+        uint32_t run = incl - sum;
+        for (uint32_t i = i0; i < i1; ++i) {
+            const uint32_t d = deg[i];
+            ptr[i] = (uint16_t)run;
+            deg[i] = run;
+            run += d;
+        }
+        if (lane == 63) ptr[n] = (uint16_t)incl;
+    }
+    __syncthreads();
+    for (uint32_t t = lane; t < nuse; t += 64) {
+        const uint32_t q = atomicAdd(&deg[rec[t] >> 3], 1u);
+        adj[q] = (uint16_t)((t >> 7) * 8 + (t & 7));  // element slot t/8, lane group (t/8)/16, position t%8
+    }
+    __syncthreads();
+    const uint32_t base = n >> 4, extra = n & 15u;
+    for (uint32_t l = 0; l < n; ++l) {
+        const uint32_t a = ptr[l], z = ptr[l + 1];
+        uint32_t key = 0xffffffffu;
+        if (lane < 16) {
+            const uint32_t c = (lane + l) & 15u;
+            if (used[c] < base + (c < extra ? 1u : 0u)) {
+                uint32_t pen = 0;
+                for (uint32_t q = a; q < z; ++q) pen += cnt[adj[q] * 16u + c];
+                key = (pen << 4) | lane;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o, 64));
+        const uint32_t c = ((key & 15u) + l) & 15u;
+        for (uint32_t q = a + lane; q < z; q += 64) cnt[adj[q] * 16u + c] += 1;
+        __syncthreads();
+        if (lane == 0) {
+            newidx[l] = (uint16_t)(used[c] * 16u + c);
+            used[c] += 1;
+        }
+        __syncthreads();
+    }
+    for (uint32_t t = lane; t < nuse; t += 64) rec[t] = (uint16_t)(newidx[rec[t] >> 3] << 3);
+    for (uint32_t i = lane; i < n; i += 64) tmp[i] = bnode[s0 + i];
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += 64) bnode[s0 + newidx[i]] = tmp[i];
+}
 __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t v = b < nblk ? boff[b + 1] - boff[b] : 0u;
@@ -540,7 +628,7 @@ __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx
 void da_build_blocks(DaDevice &da, hipStream_t s) {
     const size_t E = da.n_elem, T = 8 * E;
     const uint32_t eb = da.elem_block;
-    if (eb == 0 || (size_t)eb * 8 * 8 > 65535) {
+    if (eb == 0 || eb > 8192) {
         dgx_set_error("DA: element block of %u elements is out of range", eb);
         throw DgxError();
     }
@@ -567,8 +655,9 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
         da.lconn.alloc((size_t)nblk * 8 * eb, s);
         k_blk_lconn<<<grid_for(T, B), B, 0, s>>>(da.conn.p, da.meta.p, E, eb, da.bnode_off.p, da.bnode.p, epos.p,
                                                 da.lconn.p);
-        da.cmeta.alloc(E, s);
+        da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
         k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
+        k_blk_color<<<nblk, 64, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
         HIP_OK(hipGetLastError());
     }
     // shared nodes and their node-major partial slots
@@ -598,6 +687,10 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
     uint32_t hmx = 0;
     HIP_OK(hipMemcpyAsync(&hmx, mx.p, 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    if ((uint64_t)hmx * 8 > 65535) {  // lconn holds 16-bit byte offsets of 8-byte slots
+        dgx_set_error("DA: an element block touches %u nodes, more than the 8191 its records can address", hmx);
+        throw DgxError();
+    }
     da.n_blocks = nblk;
     da.n_pairs = np;
     da.max_block_nodes = hmx;

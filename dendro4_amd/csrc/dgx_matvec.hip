@@ -250,8 +250,7 @@ __device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t
 // The element record holds 8-byte slot offsets; SH = 1 turns them into
 // 4-byte slot offsets (FP32 slots).
 template <int SH>
-__device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, uint32_t *off) {
-    const uint4 v = *reinterpret_cast<const uint4 *>(lconn + e * 8);
+__device__ __forceinline__ void unpack_record(const uint4 v, uint32_t *off) {
     off[0] = (v.x & 0xffffu) >> SH;
     off[1] = (v.x >> 16) >> SH;
     off[2] = (v.y & 0xffffu) >> SH;
@@ -260,6 +259,10 @@ __device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, 
     off[5] = (v.z >> 16) >> SH;
     off[6] = (v.w & 0xffffu) >> SH;
     off[7] = (v.w >> 16) >> SH;
+}
+template <int SH>
+__device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, uint32_t *off) {
+    unpack_record<SH>(*reinterpret_cast<const uint4 *>(lconn + e * 8), off);
 }
 
 // Pass 1, one workgroup per element block (any block size up to MAXN nodes).
@@ -318,10 +321,14 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
 // Pass 1, persistent and software-pipelined, for blocks of at most NPT * B
 // nodes.  The grid is a multiple of 8 and workgroup w runs on XCD w % 8; each
 // XCD walks a contiguous range of element blocks, so neighbouring blocks
-// (which share nodes and x cache lines) meet in the same L2.  While block i is
-// computed, the element data, node lists and partial slots of block i+1 are
-// already in flight, and its x gather is issued before block i's results are
-// written out.
+// (which share nodes and x cache lines) meet in the same L2.  A workgroup
+// works on block b while the loads of the blocks after it are in flight:
+// the x values, element records and partial slots of b+1, the node entries of
+// b+2 and the node offsets of b+3 (b+1 = the workgroup's next block).  No load in the
+// loop depends on a load issued in the same iteration, so the element phase
+// never waits for memory.  Loads past a block's node count are clamped to its
+// last entry and element records are read for whole blocks (the record
+// buffers are padded to whole blocks); the values loaded there are not used.
 template <typename T, bool DIR, int NPT>
 __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ lconn,
                                                    const uint16_t *__restrict__ cmeta, size_t ne, uint32_t nblk,
@@ -336,92 +343,125 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
     __shared__ T lam[32][4];
     unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
     unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
-    const uint32_t xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
+    const uint32_t xcd = blockIdx.x & 7u, step = gridDim.x >> 3;
     const uint32_t lo = (uint32_t)((uint64_t)xcd * nblk / 8), hi = (uint32_t)((uint64_t)(xcd + 1) * nblk / 8);
     uint32_t b = lo + (blockIdx.x >> 3);
     if (b >= hi) return;  // uniform over the workgroup
     init_lambda<T>(lam, lap_fac, mass_fac);
 
-    uint32_t off[PER][8], mt[PER], gv[NPT], sd[NPT];
-    T xv[NPT];
-    uint32_t n0, nb;
-    auto load_block = [&](uint32_t bb, uint32_t (*o)[8], uint32_t *m, uint32_t *g, uint32_t *d, uint32_t &s0,
-                          uint32_t &sn) {
-        const size_t e0 = (size_t)bb * EB;
+    auto load_elems = [&](uint32_t bb, uint4 *ev, uint32_t *mt) {
+        const size_t e = (size_t)bb * EB + threadIdx.x;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const size_t e = e0 + threadIdx.x + k * B;
-            m[k] = 0;
-            if (e < ne) {
-                unpack_offsets<0>(lconn, e, o[k]);
-                m[k] = cmeta[e];
-            }
-        }
-        s0 = boff[bb];
-        sn = boff[bb + 1] - s0;
-#pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const uint32_t l = threadIdx.x + i * B;
-            g[i] = l < sn ? bnode[s0 + l] : 0u;
-            d[i] = (l < sn && (g[i] & SHARED_BIT)) ? bdst[s0 + l] : 0u;
+            ev[k] = *reinterpret_cast<const uint4 *>(lconn + (e + k * B) * 8);
+            mt[k] = cmeta[e + k * B];
         }
     };
-    auto gather_x = [&](const uint32_t *g, uint32_t sn, T *v) {
+    auto load_entries = [&](uint32_t s0, uint32_t n, uint32_t *g) {
 #pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const uint32_t l = threadIdx.x + i * B;
-            v[i] = (l < sn && !(DIR && (g[i] & BDY_BIT))) ? x[g[i] & ID_MASK] : T(0);
-        }
+        for (int i = 0; i < NPT; ++i) g[i] = bnode[s0 + min(threadIdx.x + i * B, n - 1u)];
     };
-    load_block(b, off, mt, gv, sd, n0, nb);
-    gather_x(gv, nb, xv);
+    auto gather = [&](const uint32_t *g, T *v) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) v[i] = x[g[i] & ID_MASK];
+    };
+    // partial slots (read for every entry, used by the shared ones)
+    auto load_slots = [&](uint32_t s0, uint32_t n, uint32_t *d) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) d[i] = bdst[s0 + min(threadIdx.x + i * B, n - 1u)];
+    };
+
+    // block b: element records, node entries, x values
+    uint32_t s0 = boff[b], nb = boff[b + 1] - s0;
+    uint4 ev[PER];
+    uint32_t mt[PER], g[NPT], sd[NPT];
+    T xv[NPT];
+    load_elems(b, ev, mt);
+    load_entries(s0, nb, g);
+    load_slots(s0, nb, sd);
+    gather(g, xv);
+    // block bA = b + step: node entries; block bB: node offsets
+    uint32_t bA = b + step, s0A = 0, nA = 1, gA[NPT];
+    if (bA < hi) {
+        s0A = boff[bA];
+        nA = boff[bA + 1] - s0A;
+        load_entries(s0A, nA, gA);
+    }
+    uint32_t bB = bA + step, s0B = 0, nB = 1;
+    if (bB < hi) {
+        s0B = boff[bB];
+        nB = boff[bB + 1] - s0B;
+    }
+    // Drain the prologue loads here: the loop body's wait counts then only
+    // see the loads issued inside the loop, and the element phase does not
+    // wait for the loads issued ahead of it in the same iteration.
+    __builtin_amdgcn_s_waitcnt(0);
     for (;;) {
+        T xr[NPT];  // x of the block's nodes, for the identity rows of boundary nodes
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
-                xs[l] = xv[i];
+                xs[l] = (DIR && (g[i] & BDY_BIT)) ? T(0) : xv[i];
                 ys[l] = 0.0;
             }
+            if (DIR) xr[i] = xv[i];
         }
         __syncthreads();
-        const uint32_t bn = b + per_xcd;
-        const bool more = bn < hi;
-        uint32_t off2[PER][8], mt2[PER], gv2[NPT], sd2[NPT], n02 = 0, nb2 = 0;
-        if (more) load_block(bn, off2, mt2, gv2, sd2, n02, nb2);
-        const size_t e0 = (size_t)b * EB;
+        const bool moreA = bA < hi;
+        uint4 evA[PER];
+        uint32_t mtA[PER], gB[NPT], sdA[NPT];
+        if (moreA) {
+            gather(gA, xv);
+            load_elems(bA, evA, mtA);
+            load_slots(s0A, nA, sdA);
+        }
+        if (bB < hi) load_entries(s0B, nB, gB);
+        const uint32_t bC = bB + step;
+        uint32_t s0C = 0, nC = 1;
+        if (bC < hi) {
+            s0C = boff[bC];
+            nC = boff[bC + 1] - s0C;
+        }
+        const size_t e0 = (size_t)b * EB + threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < PER; ++k)
-            accumulate_element<T>(off[k], mt[k], e0 + threadIdx.x + k * B < ne, xb, yb, lam);
-        if (more) gather_x(gv2, nb2, xv);
+        for (int k = 0; k < PER; ++k) {
+            uint32_t off[8];
+            unpack_record<0>(ev[k], off);
+            accumulate_element<T>(off, mt[k], e0 + k * B < ne, xb, yb, lam);
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
-                const uint32_t g = gv[i];
-                if (g & SHARED_BIT) partial[sd[i]] = (T)ys[l];
-                else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : (T)ys[l];
+                if (g[i] & SHARED_BIT) partial[sd[i]] = (T)ys[l];
+                else y[g[i] & ID_MASK] = (DIR && (g[i] & BDY_BIT)) ? xr[i] : (T)ys[l];
             }
         }
-        if (!more) break;
+        if (!moreA) break;
         __syncthreads();  // the LDS of block b is reused
-        b = bn;
-        n0 = n02;
-        nb = nb2;
+        b = bA;
+        s0 = s0A;
+        nb = nA;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            mt[k] = mt2[k];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) off[k][j] = off2[k][j];
+            ev[k] = evA[k];
+            mt[k] = mtA[k];
         }
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
-            gv[i] = gv2[i];
-            sd[i] = sd2[i];
+            g[i] = gA[i];
+            sd[i] = sdA[i];
+            gA[i] = gB[i];
         }
+        bA = bB;
+        s0A = s0B;
+        nA = nB;
+        bB = bC;
+        s0B = s0C;
+        nB = nC;
     }
-    (void)n0;
 }
 
 // Pass 2: shared nodes sum their partials in block order.
