@@ -35,7 +35,9 @@ EXPORTED = [
     "dgx_da_diagonal", "dgx_da_free", "dgx_gen_points", "dgx_da_matvec_dirichlet", "dgx_mg_create", "dgx_mg_info",
     "dgx_mg_finest_da", "dgx_mg_rhs", "dgx_mg_solve", "dgx_mg_free", "dgx_points_to_octree_glens",
     "dgx_sfc_tables", "dgx_da_export_anchors", "dgx_mg_set_operator", "dgx_server_create", "dgx_server_submit",
-    "dgx_server_wait", "dgx_server_clock", "dgx_server_free", "dgx_server_peek",
+    "dgx_server_wait", "dgx_server_clock", "dgx_server_free", "dgx_server_peek", "dgx_read_pts", "dgx_write_pts",
+    "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
+    "dgx_write_nodes_binary", "dgx_write_vtk",
 ]
 
 
@@ -136,6 +138,15 @@ def load_library():
         "dgx_mg_rhs": ([vp, vp, vp], i32),
         "dgx_mg_solve": ([vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)], i32),
         "dgx_mg_free": ([vp], i32),
+        "dgx_read_pts": ([C.c_char_p, vp, sz, C.POINTER(sz)], i32),
+        "dgx_write_pts": ([C.c_char_p, vp, sz], i32),
+        "dgx_read_data_pts": ([C.c_char_p, vp, vp, sz, C.POINTER(sz)], i32),
+        "dgx_write_data_pts": ([C.c_char_p, vp, vp, sz], i32),
+        "dgx_read_nodes": ([C.c_char_p, vp, sz, C.POINTER(sz)], i32),
+        "dgx_write_nodes": ([C.c_char_p, vp, sz], i32),
+        "dgx_read_nodes_binary": ([C.c_char_p, vp, sz, C.POINTER(sz)], i32),
+        "dgx_write_nodes_binary": ([C.c_char_p, vp, sz], i32),
+        "dgx_write_vtk": ([vp, sz, i32, C.c_char_p, i32, i32], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -436,6 +447,79 @@ def sfc_tables(curve=CURVE_HILBERT):
     c = np.zeros(192, np.uint8)
     L.dgx_sfc_tables(curve, _ptr(r), _ptr(m), _ptr(c))
     return r.reshape(24, 8), m.reshape(24, 8), c.reshape(24, 8)
+
+
+# On-disk formats (host only, no init needed) -------------------------------
+def _path(p):
+    return os.fsencode(p)
+
+
+def _read_two_call(fn, path, width, dtype, extra=False):
+    L = load_library()
+    n = C.c_size_t()
+    if extra:
+        _check(fn(_path(path), None, None, 0, C.byref(n)))
+    else:
+        _check(fn(_path(path), None, 0, C.byref(n)))
+    out = np.empty((n.value, width), dtype)
+    if not extra:
+        _check(fn(_path(path), _ptr(out), n.value, C.byref(n)))
+        return out
+    vals = np.empty(n.value, np.float64)
+    _check(fn(_path(path), _ptr(out), _ptr(vals), n.value, C.byref(n)))
+    return out, vals
+
+
+def read_pts(path):
+    """ot::readPtsFromFile: (N,3) float64 points of a .pts file."""
+    return _read_two_call(load_library().dgx_read_pts, path, 3, np.float64)
+
+
+def write_pts(path, pts):
+    """ot::writePtsToFile: u32 N then the coordinates; an empty file for N = 0."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    _check(load_library().dgx_write_pts(_path(path), _ptr(pts), len(pts)))
+
+
+def read_data_pts(path):
+    """ot::readDataPtsFromFile: ((N,3) points, (N,) values)."""
+    return _read_two_call(load_library().dgx_read_data_pts, path, 3, np.float64, extra=True)
+
+
+def write_data_pts(path, pts, vals=None):
+    """ot::writeDataPtsToFile: like the reference, writes N and the points only."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    _check(load_library().dgx_write_data_pts(_path(path), _ptr(pts), None, len(pts)))
+
+
+def read_nodes(path):
+    """ot::readNodesFromFile: (N,7) uint32 TreeNode rows of a .ot text file."""
+    return _read_two_call(load_library().dgx_read_nodes, path, 7, np.uint32)
+
+
+def write_nodes(path, tn):
+    """ot::writeNodesToFile: 'dim maxDepth', N, then 'x y z level' lines."""
+    tn = np.ascontiguousarray(tn, np.uint32).reshape(-1, 7)
+    _check(load_library().dgx_write_nodes(_path(path), _ptr(tn), len(tn)))
+
+
+def read_nodes_binary(path):
+    """ot::readNodesFromFile_binary: (N,7) uint32 rows of a raw 28-byte dump."""
+    return _read_two_call(load_library().dgx_read_nodes_binary, path, 7, np.uint32)
+
+
+def write_nodes_binary(path, tn):
+    """ot::writeNodesToFile_binary: raw 28-byte TreeNode records."""
+    tn = np.ascontiguousarray(tn, np.uint32).reshape(-1, 7)
+    _check(load_library().dgx_write_nodes_binary(_path(path), _ptr(tn), len(tn)))
+
+
+def write_vtk(tn, name, rank=0, hsorted=False, curve=CURVE_HILBERT):
+    """treeNodesTovtk: sorts a copy of tn along the curve and writes
+    <name>_H_<rank>.vtk (or _M_ for Morton); returns the sorted rows."""
+    tn = np.array(tn, np.uint32).reshape(-1, 7)
+    _check(load_library().dgx_write_vtk(_ptr(tn), len(tn), rank, _path(name), 1 if hsorted else 0, curve))
+    return tn
 
 
 # Benchmark workloads (point clouds) ----------------------------------------

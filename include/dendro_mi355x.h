@@ -210,6 +210,38 @@ double dgx_server_clock(dgx_server sv);
 /* Finishes the queued requests, then frees everything. */
 int dgx_server_free(dgx_server sv);
 
+/* ---- on-disk formats (host only; need no dgx_init) ---------------------- */
+/* Readers use two calls: with a NULL buffer (or cap below the count) they only
+ * set *n; then call again with a buffer of cap >= *n entries.  Octants are
+ * ot::TreeNode rows of 7 x u32 (x, y, z, level|flags, weight, dim, maxDepth). */
+/* ot::readPtsFromFile (src/octFunctions.cpp:319-348): u32 N, then 3N doubles.
+ * An empty file holds no points (what writePtsToFile writes for none). */
+int dgx_read_pts(const char *path, double *pts, size_t cap, size_t *n);
+/* ot::writePtsToFile (src/octFunctions.cpp:380-395): nothing when n == 0. */
+int dgx_write_pts(const char *path, const double *pts, size_t n);
+/* ot::readDataPtsFromFile (src/octFunctions.cpp:352-378): u32 N, 3N doubles of
+ * points, N doubles of values. */
+int dgx_read_data_pts(const char *path, double *pts, double *vals, size_t cap, size_t *n);
+/* ot::writeDataPtsToFile (src/octFunctions.cpp:398-412): writes N and the
+ * points; like the reference it leaves the values out (vals is ignored). */
+int dgx_write_data_pts(const char *path, const double *pts, const double *vals, size_t n);
+/* ot::readNodesFromFile (src/octFunctions.cpp:433-453): text "dim maxDepth N"
+ * then "x y z level" per octant; weight 1, no flags (TreeNode.cpp:650-662). */
+int dgx_read_nodes(const char *path, uint32_t *tn, size_t cap, size_t *n);
+/* ot::writeNodesToFile (src/octFunctions.cpp:469-486); all octants must share
+ * dim and maxDepth; the level is written without flags. */
+int dgx_write_nodes(const char *path, const uint32_t *tn, size_t n);
+/* ot::readNodesFromFile_binary (src/octFunctions.cpp:415-431): 28-byte
+ * records.  Returns the N records of the file; the reference's feof loop
+ * appends a copy of the last one. */
+int dgx_read_nodes_binary(const char *path, uint32_t *tn, size_t cap, size_t *n);
+/* ot::writeNodesToFile_binary (src/octFunctions.cpp:455-467). */
+int dgx_write_nodes_binary(const char *path, const uint32_t *tn, size_t n);
+/* treeNodesTovtk (src/treenode2vtk.cpp:4-134): sorts tn in place along the
+ * curve (as the reference's std::sort does), then writes
+ * <name>_H_<rank>.vtk (Hilbert) or <name>_M_<rank>.vtk (Morton). */
+int dgx_write_vtk(uint32_t *tn, size_t n, int rank, const char *name, int hsorted, int curve);
+
 /* ---- inputs ------------------------------------------------------------- */
 /* Deterministic synthetic point clouds (host memory, n*3 doubles):
  *   dist 0: i.i.d. uniform [0,1)^3

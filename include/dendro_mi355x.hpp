@@ -215,6 +215,78 @@ inline int balanceOctree(std::vector<TreeNode> &in, std::vector<TreeNode> &out, 
     return 1;
 }
 
+// ---- on-disk formats (octFunctions.cpp:319-488); host only ----------------
+namespace detail_io {
+template <typename T, typename Read>
+inline void read_all(const char *what, std::vector<T> &v, size_t width, Read rd) {
+    size_t n = 0;
+    dgx::check(rd(nullptr, 0, &n), what);
+    v.resize(width * n);
+    if (n) dgx::check(rd(v.data(), n, &n), what);
+}
+}  // namespace detail_io
+
+// ot::readPtsFromFile (octFunctions.cpp:319-348): pts receives 3N coordinates.
+inline int readPtsFromFile(const char *filename, std::vector<double> &pts) {
+    detail_io::read_all("readPtsFromFile", pts, 3, [&](double *p, size_t cap, size_t *n) {
+        return dgx_read_pts(filename, p, cap, n);
+    });
+    return 1;
+}
+// ot::readDataPtsFromFile (octFunctions.cpp:352-378).
+inline int readDataPtsFromFile(const char *filename, std::vector<double> &pts, std::vector<double> &ptVals) {
+    size_t n = 0;
+    dgx::check(dgx_read_data_pts(filename, nullptr, nullptr, 0, &n), "readDataPtsFromFile");
+    pts.resize(3 * n);
+    ptVals.resize(n);
+    if (n) dgx::check(dgx_read_data_pts(filename, pts.data(), ptVals.data(), n, &n), "readDataPtsFromFile");
+    return 1;
+}
+// ot::writePtsToFile (octFunctions.cpp:380-395).
+inline int writePtsToFile(const char *filename, std::vector<double> &pts) {
+    dgx::check(dgx_write_pts(filename, pts.data(), pts.size() / 3), "writePtsToFile");
+    return 1;
+}
+// ot::writeDataPtsToFile (octFunctions.cpp:398-412): N and the points; the
+// values are left out, as in the reference.
+inline int writeDataPtsToFile(const char *filename, std::vector<double> &pts, std::vector<double> &data) {
+    dgx::check(dgx_write_data_pts(filename, pts.data(), data.data(), pts.size() / 3), "writeDataPtsToFile");
+    return 1;
+}
+// ot::readNodesFromFile (octFunctions.cpp:433-453): the ".ot" text format.
+inline int readNodesFromFile(const char *filename, std::vector<TreeNode> &nodes) {
+    size_t n = 0;
+    dgx::check(dgx_read_nodes(filename, nullptr, 0, &n), "readNodesFromFile");
+    nodes.resize(n);
+    if (n) dgx::check(dgx_read_nodes(filename, reinterpret_cast<uint32_t *>(nodes.data()), n, &n), "readNodesFromFile");
+    return 1;
+}
+// ot::readNodesFromFile_binary (octFunctions.cpp:415-431): appends the N
+// records of the file (the reference's feof loop appends the last one twice)
+// and returns the new size.
+inline int readNodesFromFile_binary(const char *filename, std::vector<TreeNode> &nodes) {
+    size_t n = 0;
+    dgx::check(dgx_read_nodes_binary(filename, nullptr, 0, &n), "readNodesFromFile_binary");
+    const size_t old = nodes.size();
+    nodes.resize(old + n);
+    if (n)
+        dgx::check(dgx_read_nodes_binary(filename, reinterpret_cast<uint32_t *>(nodes.data() + old), n, &n),
+                   "readNodesFromFile_binary");
+    return (int)nodes.size();
+}
+// ot::writeNodesToFile_binary (octFunctions.cpp:455-467).
+inline int writeNodesToFile_binary(const char *filename, const std::vector<TreeNode> &nodes) {
+    dgx::check(dgx_write_nodes_binary(filename, reinterpret_cast<const uint32_t *>(nodes.data()), nodes.size()),
+               "writeNodesToFile_binary");
+    return 0;
+}
+// ot::writeNodesToFile (octFunctions.cpp:469-486).
+inline int writeNodesToFile(const char *filename, const std::vector<TreeNode> &nodes) {
+    dgx::check(dgx_write_nodes(filename, reinterpret_cast<const uint32_t *>(nodes.data()), nodes.size()),
+               "writeNodesToFile");
+    return 1;
+}
+
 namespace DA_FLAGS {
 // Element loops (oda.h); on one rank every element is writable and
 // independent, DEPENDENT loops are empty.
@@ -549,6 +621,15 @@ inline int DAMGDestroy(DAMG *damg) {
 #define DAMGGetUser(ctx, level) ((ctx)[level]->user)
 #define DAMGGetLevels(ctx) (ctx)[0]->nlevels
 #define DAMGGetDAMG(ctx) (ctx)[(ctx)[0]->nlevels - 1]
+
+// treeNodesTovtk (treenode2vtk.h:19, treenode2vtk.cpp:4-134): sorts nodes
+// along the curve and writes <name>_H_<rank>.vtk (_M_ for Morton).
+inline void treeNodesTovtk(std::vector<ot::TreeNode> &nodes, int mpi_rank, std::string vtk_file_name,
+                           bool hsorted = false) {
+    dgx::check(dgx_write_vtk(reinterpret_cast<uint32_t *>(nodes.data()), nodes.size(), mpi_rank,
+                             vtk_file_name.c_str(), hsorted ? 1 : 0, dgx::curve()),
+               "treeNodesTovtk");
+}
 
 namespace par {
 // par::sampleSort (parUtils.h:424-425) on one rank: a device sort of the
