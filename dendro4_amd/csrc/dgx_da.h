@@ -59,5 +59,13 @@ void da_matvec_dirichlet_f64(const DaDevice &da, const double *x, double *y, dou
                              hipStream_t s);
 void da_matvec_dirichlet_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac,
                              hipStream_t s);
+// Smoother and residual of the Dirichlet operator with the update fused into
+// the matvec's stores: x += omega dinv (b - A x) in place (scratch: n_nodes
+// entries, used only for DAs whose blocks exceed 1024 nodes), r = b - A x.
+template <typename T>
+void da_dirichlet_jacobi(const DaDevice &da, T *x, const T *b, const T *dinv, T omega, T lap_fac, T mass_fac,
+                         T *scratch, hipStream_t s);
+template <typename T>
+void da_dirichlet_residual(const DaDevice &da, const T *x, const T *b, T *r, T lap_fac, T mass_fac, hipStream_t s);
 // Dense n_nodes x n_nodes matrix of the Dirichlet operator (row-major, FP64).
 void da_assemble_dense_dirichlet(const DaDevice &da, double lap_fac, double mass_fac, double *A, hipStream_t s);

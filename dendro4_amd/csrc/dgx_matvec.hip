@@ -173,6 +173,7 @@ __device__ __forceinline__ void haar3(T *u) {
 
 constexpr int PER = 2;  // elements per thread (element block = PER * B)
 constexpr uint32_t ID_MASK = 0x3fffffffu, BDY_BIT = 0x40000000u, SHARED_BIT = 0x80000000u;
+enum { EPI_Y = 0, EPI_JACOBI = 1, EPI_RESID = 2 };
 
 // Haar eigenvalues / 8 per level (h = 2^-level), lam[level][nd].
 template <typename T>
@@ -329,14 +330,19 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
 // never waits for memory.  Loads past a block's node count are clamped to its
 // last entry and element records are read for whole blocks (the record
 // buffers are padded to whole blocks); the values loaded there are not used.
-template <typename T, bool DIR, int NPT>
+// EPI selects what the writeout stores for a node with row value v = (A x)_i:
+// EPI_Y: y_i = v; EPI_JACOBI: x_i += omega dinv_i (b_i - v), in place (y is
+// x); EPI_RESID: y_i = b_i - v.  In place is safe: a block writes only nodes
+// no other block reads, and shared nodes are finished by the second pass.
+template <typename T, bool DIR, int NPT, int EPI>
 __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ lconn,
                                                    const uint16_t *__restrict__ cmeta, size_t ne, uint32_t nblk,
                                                    const uint32_t *__restrict__ boff,
                                                    const uint32_t *__restrict__ bnode,
-                                                   const uint32_t *__restrict__ bdst, const T *__restrict__ x,
-                                                   T *__restrict__ y, T *__restrict__ partial, double lap_fac,
-                                                   double mass_fac) {
+                                                   const uint32_t *__restrict__ bdst, const T *x, T *y,
+                                                   T *__restrict__ partial, double lap_fac, double mass_fac,
+                                                   const T *__restrict__ bvec, const T *__restrict__ dinv,
+                                                   T omega) {
     constexpr int EB = PER * B, MAXN = NPT * B;
     __shared__ __align__(16) T xs[MAXN];
     __shared__ __align__(16) double ys[MAXN];
@@ -397,7 +403,10 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
     // wait for the loads issued ahead of it in the same iteration.
     __builtin_amdgcn_s_waitcnt(0);
     for (;;) {
-        T xr[NPT];  // x of the block's nodes, for the identity rows of boundary nodes
+        // x of the block's nodes for the identity rows of boundary nodes (the
+        // Jacobi writeout reads x back from the LDS slots instead and reloads
+        // it for boundary nodes, whose slots hold 0)
+        T xr[NPT];
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
@@ -405,7 +414,14 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
                 xs[l] = (DIR && (g[i] & BDY_BIT)) ? T(0) : xv[i];
                 ys[l] = 0.0;
             }
-            if (DIR) xr[i] = xv[i];
+            if (DIR && EPI != EPI_JACOBI) xr[i] = xv[i];
+        }
+        // right-hand side and inverse diagonal of the block's nodes, for the writeout
+        T bv[NPT], dv[NPT];
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            if (EPI != EPI_Y) bv[i] = bvec[g[i] & ID_MASK];
+            if (EPI == EPI_JACOBI) dv[i] = dinv[g[i] & ID_MASK];
         }
         __syncthreads();
         const bool moreA = bA < hi;
@@ -435,8 +451,17 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
-                if (g[i] & SHARED_BIT) partial[sd[i]] = (T)ys[l];
-                else y[g[i] & ID_MASK] = (DIR && (g[i] & BDY_BIT)) ? xr[i] : (T)ys[l];
+                if (g[i] & SHARED_BIT) {
+                    partial[sd[i]] = (T)ys[l];
+                } else if (EPI == EPI_JACOBI) {
+                    const T xi = (DIR && (g[i] & BDY_BIT)) ? x[g[i] & ID_MASK] : xs[l];
+                    const T v = (DIR && (g[i] & BDY_BIT)) ? xi : (T)ys[l];
+                    y[g[i] & ID_MASK] = xi + omega * dv[i] * (bv[i] - v);
+                } else {
+                    const T v = (DIR && (g[i] & BDY_BIT)) ? xr[i] : (T)ys[l];
+                    if (EPI == EPI_Y) y[g[i] & ID_MASK] = v;
+                    else y[g[i] & ID_MASK] = bv[i] - v;
+                }
             }
         }
         if (!moreA) break;
@@ -464,22 +489,38 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
     }
 }
 
-// Pass 2: shared nodes sum their partials in block order.
-template <typename T, bool DIR>
+// Pass 2: shared nodes sum their partials in block order and store as the
+// first pass does (EPI).
+template <typename T, bool DIR, int EPI>
 __global__ void __launch_bounds__(B) k_matvec_shared(const uint32_t *__restrict__ soff,
                                                      const uint32_t *__restrict__ snode, uint32_t ns,
-                                                     const T *__restrict__ partial, const T *__restrict__ x,
-                                                     T *__restrict__ y) {
+                                                     const T *__restrict__ partial, const T *x, T *y,
+                                                     const T *__restrict__ bvec, const T *__restrict__ dinv,
+                                                     T omega) {
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i >= ns) return;
-    const uint32_t g = snode[i];
+    const uint32_t g = snode[i], id = g & ID_MASK;
+    T v;
     if (DIR && (g & BDY_BIT)) {
-        y[g & ID_MASK] = x[g & ID_MASK];
-        return;
+        v = x[id];
+    } else {
+        v = T(0);
+        for (uint32_t p = soff[i]; p < soff[i + 1]; ++p) v += partial[p];
     }
-    T acc = T(0);
-    for (uint32_t p = soff[i]; p < soff[i + 1]; ++p) acc += partial[p];
-    y[g & ID_MASK] = acc;
+    if (EPI == EPI_Y) y[id] = v;
+    else if (EPI == EPI_RESID) y[id] = bvec[id] - v;
+    else y[id] = x[id] + omega * dinv[id] * (bvec[id] - v);
+}
+// Stand-alone writeouts for the blocks the pipelined kernel does not take.
+template <typename T>
+__global__ void k_after_jacobi(const T *bvec, const T *ax, const T *dinv, T omega, uint32_t n, T *x) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) x[i] = x[i] + omega * dinv[i] * (bvec[i] - ax[i]);
+}
+template <typename T>
+__global__ void k_after_resid(const T *bvec, uint32_t n, T *y) {
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (i < n) y[i] = bvec[i] - y[i];
 }
 
 // Persistent grid: resident workgroups per CU x CUs, a multiple of 8 (one
@@ -509,8 +550,12 @@ unsigned pipe_grid(const void *kernel, uint32_t nblk) {
     return (unsigned)std::max<uint64_t>(g, 8);
 }
 
-template <typename T, bool DIR>
-void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, hipStream_t s) {
+// EPI as for k_matvec_pipe; with EPI_JACOBI, y is x and scratch (n_nodes
+// entries) holds A x on the path for DAs whose blocks exceed the pipelined
+// kernel's 1024 nodes.
+template <typename T, bool DIR, int EPI>
+void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, const T *bvec, const T *dinv,
+                   T omega, T *scratch, hipStream_t s) {
     if (!da.n_elem) return;
     if (da.elem_block != PER * B) {
         dgx_set_error("matvec: DA built with element blocks of %u, kernel expects %d", da.elem_block, PER * B);
@@ -519,45 +564,72 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
     T *partial = reinterpret_cast<T *>(da.partial.p);
     const uint32_t mn = da.max_block_nodes;
     if (mn <= 4 * B) {
-        const unsigned grid = pipe_grid(reinterpret_cast<const void *>(&k_matvec_pipe<T, DIR, 4>), da.n_blocks);
-        k_matvec_pipe<T, DIR, 4><<<grid, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.n_blocks, da.bnode_off.p,
-                                                    da.bnode.p, da.bdst.p, x, y, partial, (double)lap_fac,
-                                                    (double)mass_fac);
-    } else {
+        const unsigned grid =
+            pipe_grid(reinterpret_cast<const void *>(&k_matvec_pipe<T, DIR, 4, EPI>), da.n_blocks);
+        k_matvec_pipe<T, DIR, 4, EPI><<<grid, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.n_blocks,
+                                                         da.bnode_off.p, da.bnode.p, da.bdst.p, x, y, partial,
+                                                         (double)lap_fac, (double)mass_fac, bvec, dinv, omega);
+        if (da.n_shared)
+            k_matvec_shared<T, DIR, EPI><<<grid_for(da.n_shared, B), B, 0, s>>>(
+                da.soff.p, da.snode.p, da.n_shared, partial, x, y, bvec, dinv, omega);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    T *ax = EPI == EPI_JACOBI ? scratch : y;
 #define DGX_MV_LAUNCH(MAXN)                                                                                     \
     k_matvec_blk<T, DIR, MAXN><<<da.n_blocks, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.bnode_off.p,     \
-                                                          da.bnode.p, da.bdst.p, x, y, partial, (double)lap_fac, \
+                                                          da.bnode.p, da.bdst.p, x, ax, partial, (double)lap_fac, \
                                                           (double)mass_fac)
-        if (mn <= 2048) DGX_MV_LAUNCH(2048);
-        else if (mn <= 4096) DGX_MV_LAUNCH(4096);
-        else {
-            dgx_set_error("matvec: block with %u nodes exceeds 4096", mn);
-            throw DgxError();
-        }
+    if (mn <= 2048) DGX_MV_LAUNCH(2048);
+    else if (mn <= 4096) DGX_MV_LAUNCH(4096);
+    else {
+        dgx_set_error("matvec: block with %u nodes exceeds 4096", mn);
+        throw DgxError();
     }
 #undef DGX_MV_LAUNCH
     if (da.n_shared)
-        k_matvec_shared<T, DIR>
-            <<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared, partial, x, y);
+        k_matvec_shared<T, DIR, EPI_Y><<<grid_for(da.n_shared, B), B, 0, s>>>(da.soff.p, da.snode.p, da.n_shared,
+                                                                              partial, x, ax, nullptr, nullptr, T(0));
+    const unsigned g = grid_for(da.n_nodes, B);
+    if (EPI == EPI_RESID) k_after_resid<T><<<g, B, 0, s>>>(bvec, da.n_nodes, y);
+    if (EPI == EPI_JACOBI) k_after_jacobi<T><<<g, B, 0, s>>>(bvec, ax, dinv, omega, da.n_nodes, y);
     HIP_OK(hipGetLastError());
 }
 
 }  // namespace
 
 void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s) {
-    matvec_blocks<double, false>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<double, false, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0, nullptr, s);
 }
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s) {
-    matvec_blocks<float, false>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<float, false, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0f, nullptr, s);
 }
 void da_matvec_dirichlet_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac,
                              hipStream_t s) {
-    matvec_blocks<double, true>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<double, true, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0, nullptr, s);
 }
 void da_matvec_dirichlet_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac,
                              hipStream_t s) {
-    matvec_blocks<float, true>(da, x, y, lap_fac, mass_fac, s);
+    matvec_blocks<float, true, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0f, nullptr, s);
 }
+template <typename T>
+void da_dirichlet_jacobi(const DaDevice &da, T *x, const T *b, const T *dinv, T omega, T lap_fac, T mass_fac,
+                         T *scratch, hipStream_t s) {
+    matvec_blocks<T, true, EPI_JACOBI>(da, x, x, lap_fac, mass_fac, b, dinv, omega, scratch, s);
+}
+template <typename T>
+void da_dirichlet_residual(const DaDevice &da, const T *x, const T *b, T *r, T lap_fac, T mass_fac,
+                           hipStream_t s) {
+    matvec_blocks<T, true, EPI_RESID>(da, x, r, lap_fac, mass_fac, b, nullptr, T(0), nullptr, s);
+}
+template void da_dirichlet_jacobi<double>(const DaDevice &, double *, const double *, const double *, double, double,
+                                          double, double *, hipStream_t);
+template void da_dirichlet_jacobi<float>(const DaDevice &, float *, const float *, const float *, float, float, float,
+                                         float *, hipStream_t);
+template void da_dirichlet_residual<double>(const DaDevice &, const double *, const double *, double *, double, double,
+                                            hipStream_t);
+template void da_dirichlet_residual<float>(const DaDevice &, const float *, const float *, float *, float, float,
+                                           hipStream_t);
 
 namespace {
 

@@ -181,16 +181,6 @@ __global__ void k_jacobi_first(const T *b, const T *dinv, T omega, uint32_t n, T
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i < n) x[i] = omega * dinv[i] * b[i];
 }
-template <typename T>
-__global__ void k_jacobi(const T *b, const T *ax, const T *dinv, T omega, uint32_t n, T *x) {
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (i < n) x[i] += omega * dinv[i] * (b[i] - ax[i]);
-}
-template <typename T>
-__global__ void k_residual(const T *b, const T *ax, uint32_t n, T *r) {
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (i < n) r[i] = b[i] - ax[i];
-}
 // bc = R rf: 8 lanes per coarse row, strided over the row's entries, then a
 // butterfly sum inside the 8-lane group.
 template <typename T>
@@ -521,12 +511,6 @@ const double *rval_of<double>(const Transfer &t) { return t.rval.p; }
 template <>
 const float *rval_of<float>(const Transfer &t) { return t.rval32.p; }
 
-inline void matvec_dir(const DaDevice &d, const double *x, double *y, double lap, double mass, hipStream_t s) {
-    da_matvec_dirichlet_f64(d, x, y, lap, mass, s);
-}
-inline void matvec_dir(const DaDevice &d, const float *x, float *y, double lap, double mass, hipStream_t s) {
-    da_matvec_dirichlet_f32(d, x, y, (float)lap, (float)mass, s);
-}
 
 template <typename T>
 void vcycle(MgHierarchy &mg, Work<T> &w, size_t k, const SolveOpts &o, hipStream_t s) {
@@ -538,23 +522,19 @@ void vcycle(MgHierarchy &mg, Work<T> &w, size_t k, const SolveOpts &o, hipStream
         k_dense_gemv<T><<<grid_for(n, B / 64), B, 0, s>>>(mg.cinv.p, n, b, x);
         return;
     }
-    const T om = (T)o.omega;
+    const T om = (T)o.omega, lap = (T)mg.lap_fac, mass = (T)mg.mass_fac;
     const T *dinv = dinv_of<T>(L);
+    // Jacobi sweeps x += omega D^-1 (b - A x) and the residual b - A x, each
+    // one matvec with the update fused into its stores; the first sweep
+    // starts from x = 0 and needs no matvec
     k_jacobi_first<T><<<g, B, 0, s>>>(b, dinv, om, n, x);
-    for (int it = 1; it < o.pre; ++it) {
-        matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
-        k_jacobi<T><<<g, B, 0, s>>>(b, y, dinv, om, n, x);
-    }
-    matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
-    k_residual<T><<<g, B, 0, s>>>(b, y, n, y);
+    for (int it = 1; it < o.pre; ++it) da_dirichlet_jacobi<T>(*L.da, x, b, dinv, om, lap, mass, y, s);
+    da_dirichlet_residual<T>(*L.da, x, b, y, lap, mass, s);
     const Transfer &t = mg.lev[k + 1].to_finer;
     k_restrict<T><<<grid_for((size_t)t.nc * 8, B), B, 0, s>>>(t.roff.p, t.rcol.p, rval_of<T>(t), (uint32_t)t.nc, y, w.b[k + 1].p);
     vcycle<T>(mg, w, k + 1, o, s);
     k_prolong_add<T><<<g, B, 0, s>>>(t.poff.p, t.pcol.p, pval_of<T>(t), n, w.x[k + 1].p, x);
-    for (int it = 0; it < o.post; ++it) {
-        matvec_dir(*L.da, x, y, mg.lap_fac, mg.mass_fac, s);
-        k_jacobi<T><<<g, B, 0, s>>>(b, y, dinv, om, n, x);
-    }
+    for (int it = 0; it < o.post; ++it) da_dirichlet_jacobi<T>(*L.da, x, b, dinv, om, lap, mass, y, s);
 }
 
 template <typename T>
@@ -766,8 +746,7 @@ void mg_solve(MgHierarchy &mg, const double *b, double *xout, const SolveOpts &o
         run_graph(w.g_dir, o, o.use_graph, s, [&] { direction_body(mg, w, o, s); });
     }
     // true residual in FP64
-    da_matvec_dirichlet_f64(d, w.x.p, w.q.p, mg.lap_fac, mg.mass_fac, s);
-    k_residual<double><<<g, B, 0, s>>>(w.bvec.p, w.q.p, n, w.r.p);
+    da_dirichlet_residual<double>(d, w.x.p, w.bvec.p, w.r.p, mg.lap_fac, mg.mass_fac, s);
     dot_dev(w.r.p, w.r.p, n, w.part.p, w.sc.p + 6, s);
     res.rel_residual = std::sqrt(read_scalar(w.sc.p + 6, s)) / bnorm;
     k_copy<<<g, B, 0, s>>>(w.x.p, n, xout);
