@@ -590,18 +590,16 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
     }
     __syncthreads();
     const uint32_t base = n >> 4, extra = n & 15u;
+    // lane = 16 x k + j: candidate colour (j + l) mod 16, uses a + k, a + k + 4, ...
+    const uint32_t j = lane & 15u, k = lane >> 4;
     for (uint32_t l = 0; l < n; ++l) {
-        const uint32_t a = ptr[l], z = ptr[l + 1];
-        uint32_t key = 0xffffffffu;
-        if (lane < 16) {
-            const uint32_t c = (lane + l) & 15u;
-            if (used[c] < base + (c < extra ? 1u : 0u)) {
-                uint32_t pen = 0;
-                for (uint32_t q = a; q < z; ++q) pen += cnt[adj[q] * 16u + c];
-                key = (pen << 4) | lane;
-            }
-        }
-        for (int o = 32; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o, 64));
+        const uint32_t a = ptr[l], z = ptr[l + 1], cj = (j + l) & 15u;
+        uint32_t pen = 0;
+        for (uint32_t q = a + k; q < z; q += 4) pen += cnt[adj[q] * 16u + cj];
+        pen += __shfl_xor(pen, 16, 64);
+        pen += __shfl_xor(pen, 32, 64);
+        uint32_t key = used[cj] < base + (cj < extra ? 1u : 0u) ? ((pen << 4) | j) : 0xffffffffu;
+        for (int o = 8; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o, 64));
         const uint32_t c = ((key & 15u) + l) & 15u;
         for (uint32_t q = a + lane; q < z; q += 64) cnt[adj[q] * 16u + c] += 1;
         __syncthreads();
