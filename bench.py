@@ -84,16 +84,21 @@ def log(msg):
 
 def solve_section(dgx, ob, nlevels, solves, warmup, seed, graph=True):
     """FP64 and FP32-V-cycle solves of the Dirichlet problem on octree ob."""
-    t0 = time.perf_counter()
-    mg = dgx.MG(ob, nlevels)
-    dgx.synchronize()
-    t_setup = time.perf_counter() - t0
+    # the hierarchy is built twice: the first build in a process also pays
+    # the one-time start-up of rocSOLVER (coarse Cholesky)
+    setups = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        mg = dgx.MG(ob, nlevels)
+        dgx.synchronize()
+        setups.append(time.perf_counter() - t0)
+    t_setup = setups[-1]
     n = mg.n_nodes
     f = dgx.DeviceArray.from_host(np.random.default_rng(seed).standard_normal(n))
     b = dgx.DeviceArray((n,), np.float64)
     mg.rhs(f, b)
     out = {"levels": mg.nlevels, "level_octants": mg.level_octants, "level_nodes": mg.level_nodes,
-           "mg_setup_s": t_setup}
+           "mg_setup_s": t_setup, "mg_first_setup_s": setups[0]}
     sol = {}
     for name, fp32 in (("fp64", False), ("fp32_mg", True)):
         u = dgx.DeviceArray((n,), np.float64)
