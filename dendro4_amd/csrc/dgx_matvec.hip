@@ -17,6 +17,7 @@
 #include "dgx_da.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <utility>
 #include <vector>
 
@@ -563,7 +564,10 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
     }
     T *partial = reinterpret_cast<T *>(da.partial.p);
     const uint32_t mn = da.max_block_nodes;
-    if (mn <= 4 * B) {
+    // DGX_MATVEC_BLOCKED=1 takes the path of DAs with large blocks for every
+    // DA (a test hook: real meshes rarely reach it)
+    const char *forced = getenv("DGX_MATVEC_BLOCKED");
+    if (mn <= 4 * B && !(forced && forced[0] == '1')) {
         const unsigned grid =
             pipe_grid(reinterpret_cast<const void *>(&k_matvec_pipe<T, DIR, 4, EPI>), da.n_blocks);
         k_matvec_pipe<T, DIR, 4, EPI><<<grid, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.n_blocks,

@@ -78,6 +78,42 @@ def test_pipeline_morton_curve(dgx, oracle, stencils):
     check_pipeline(dgx, oracle, pts, 30, 4, *stencils, curve=1)
 
 
+@pytest.mark.parametrize("dist,seed,n,md,m", [(0, 3, 20000, 6, 1), (0, 5, 5000, 5, 2), (1, 4, 20000, 8, 2),
+                                             (1, 4, 20000, 10, 1), (2, 11, 20000, 16, 4)])
+def test_max_depth_variants(dgx, oracle, stencils, dist, seed, n, md, m):
+    """Shallow trees: most level-maxDepth cells hold more than maxNumPts
+    points, so p2oLocal's saturated-cell path runs all over the domain."""
+    pts = (dgx.gen_points(dist, seed, n, n_gauss=3, lo=0.3, hi=0.7, sig_lo=0.002) if dist == 2
+           else dgx.gen_points(dist, seed, n))
+    check_pipeline(dgx, oracle, pts, md, m, *stencils)
+
+
+@pytest.mark.parametrize("md,m", [(4, 1), (8, 2)])
+def test_stale_cursor_at_domain_end_is_an_error_in_both(dgx, oracle, md, m):
+    """Clustered points at a shallow maxDepth leave the p2oLocal cursor stale
+    at the end of the domain, where the reference compares against the getNext
+    sentinel; the oracle and the library both report it."""
+    pts = dgx.gen_points(2, 11, 20000, n_gauss=3, lo=0.3, hi=0.7, sig_lo=0.002)
+    with pytest.raises(ValueError):
+        oracle.points2octree(pts, md, m)
+    with pytest.raises(dgx.DgxError):
+        dgx.points2octree(pts, md, m)
+
+
+def test_domain_lengths(dgx, oracle, stencils):
+    """points2Octree over [0, gLens): px = (uint)(x 2^D / gLen).  Scaling by
+    2^D is exact, so this equals the unit-cube quantisation of x / gLen."""
+    g = np.array([2.0, 1.0, 0.5])
+    pts = dgx.gen_points(1, 5, 20000) * g
+    ou = dgx.points2octree(pts, 30, 4, glens=g)
+    ru = oracle.points2octree(pts / g, 30, 4)
+    assert np.array_equal(ou.treenodes(), ru)
+    ob = dgx.balance_octree(ou)
+    assert np.array_equal(ob.treenodes(), oracle.balance(ru))
+    with pytest.raises(dgx.DgxError):  # a point at x = gLen is outside
+        dgx.points2octree(np.vstack([pts, [[2.0, 0.5, 0.25]]]), 30, 4, glens=g)
+
+
 def test_degenerate_inputs(dgx, oracle):
     pts = np.array([[0.25, 0.5, 0.75]])
     assert dgx.points2octree(pts, 30, 1).treenodes().tolist() == [[0, 0, 0, 0, 1, 3, 30]]

@@ -137,3 +137,46 @@ def test_fp32_matvec_matches_fp64_to_single_precision(dgx, problem):
     y32 = dy.to_host().astype(np.float64)
     assert np.abs(y32 - y64).max() / np.abs(y64).max() < 1e-5
     assert np.abs(y64 - A @ x).max() / np.abs(A @ x).max() < 1e-12
+
+
+def solve_against_direct(dgx, ob, rda, bd, L, M, lap, mass, nlevels=4):
+    mg = dgx.MG(ob, nlevels, lap_fac=lap, mass_fac=mass)
+    n = mg.n_nodes
+    f = np.random.default_rng(5).standard_normal(n)
+    df = dgx.DeviceArray.from_host(f)
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    br = db.to_host()
+    AD = dirichlet(assemble(rda, L, M, lap=lap, mass=mass), bd)
+    uref = spla.spsolve(AD.tocsc(), br)
+    for fp32 in (False, True):
+        du = dgx.DeviceArray((n,), np.float64)
+        st = mg.solve(db, du, fp32=fp32, rtol=1e-8, maxit=300)
+        u = du.to_host()
+        err = np.linalg.norm(u - uref) / np.linalg.norm(uref)
+        true_res = np.linalg.norm(br - AD @ u) / np.linalg.norm(br)
+        print(f"lap={lap} mass={mass} fp32={fp32} it={st['iterations']} err={err:.2e} res={true_res:.2e}", flush=True)
+        assert st["rel_residual"] <= 1e-8 and true_res <= 1.5e-8 and err <= 1e-7
+
+
+@pytest.mark.parametrize("lap,mass", [(2.0, 0.5), (1.0, 0.0), (0.25, 4.0)])
+def test_mg_solve_operator_factors(dgx, problem, stencils, lap, mass):
+    """DAMGSetKSP's lapFac / massFac: -lapFac lap + massFac I, including the
+    pure Dirichlet Laplacian (massFac 0)."""
+    ob, rda, A, bd = problem
+    solve_against_direct(dgx, ob, rda, bd, *stencils, lap, mass)
+
+
+def test_matvec_path_for_large_blocks(dgx, problem, stencils, monkeypatch):
+    """The matvec path taken by DAs whose element blocks touch more than 1024
+    nodes (one workgroup per block, then separate smoother / residual
+    updates), forced on this DA: matvecs and MG solves against the references."""
+    monkeypatch.setenv("DGX_MATVEC_BLOCKED", "1")
+    ob, rda, A, bd = problem
+    da = dgx.DA(ob)
+    x = np.random.default_rng(0).standard_normal(da.n_nodes)
+    AD = dirichlet(A, bd)
+    yr = AD @ x
+    assert np.abs(da.matvec_dirichlet_host(x) - yr).max() / np.abs(yr).max() < 1e-12
+    assert np.abs(da.matvec_host(x) - A @ x).max() / np.abs(A @ x).max() < 1e-12
+    solve_against_direct(dgx, ob, rda, bd, *stencils, 1.0, 1.0)
