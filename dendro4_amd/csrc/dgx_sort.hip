@@ -200,15 +200,32 @@ __global__ void __launch_bounds__(SB) k_block_hist(const Key *keys, size_t n, in
     bhist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one tile (STILE keys).  Keys are ranked inside the tile
+// (8 ballots per wave, LDS counters across the 4 waves, rounds in order),
+// staged in LDS in digit order, and then written run by run: consecutive
+// threads store consecutive addresses of one digit's output run.  The tile's
+// count of digit d is the difference of consecutive entries of the scanned
+// digit-major histogram.
 template <bool HAS_VALS>
 __global__ void __launch_bounds__(SB) k_scatter(const Key *kin, const uint32_t *vin, Key *kout, uint32_t *vout,
                                                 size_t n, int bit, const uint32_t *boff, uint32_t nblocks) {
-    __shared__ uint32_t run_off[256];
+    __shared__ uint32_t run_off[256];   // output position of the tile's first key of digit d
+    __shared__ uint32_t tile_off[256];  // start of digit d in the staged tile
+    __shared__ uint32_t seen[256];      // keys of digit d staged so far
     __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t scan_tmp[SB / 64 + 1];
+    __shared__ Key sk[STILE];
+    __shared__ uint32_t sv[HAS_VALS ? STILE : 1];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    run_off[tid] = boff[(size_t)tid * nblocks + blockIdx.x];
+    const size_t hi_idx = (size_t)tid * nblocks + blockIdx.x;
+    const uint32_t cur = boff[hi_idx];
+    const uint32_t nxt = hi_idx + 1 < (size_t)256 * nblocks ? boff[hi_idx + 1] : (uint32_t)n;
+    run_off[tid] = cur;
+    uint32_t total;
+    tile_off[tid] = block_exclusive_scan(nxt - cur, scan_tmp, total);
+    seen[tid] = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    size_t base = (size_t)blockIdx.x * STILE;
+    const size_t base = (size_t)blockIdx.x * STILE;
     for (int r = 0; r < SITEMS; ++r) {
         size_t idx = base + (size_t)r * SB + tid;
         bool valid = idx < n;
@@ -232,14 +249,22 @@ __global__ void __launch_bounds__(SB) k_scatter(const Key *kin, const uint32_t *
         if (valid && rank == 0) wcnt[wid][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = run_off[d] + rank;
+            uint32_t pos = tile_off[d] + seen[d] + rank;
             for (int w = 0; w < wid; ++w) pos += wcnt[w][d];
-            kout[pos] = k;
-            if (HAS_VALS) vout[pos] = v;
+            sk[pos] = k;
+            if (HAS_VALS) sv[pos] = v;
         }
         __syncthreads();
-        run_off[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        __syncthreads();
+        seen[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    }
+    __syncthreads();
+    const uint32_t tn = (uint32_t)min((size_t)STILE, n - base);
+    for (uint32_t i = tid; i < tn; i += SB) {
+        const Key k = sk[i];
+        const uint32_t d = digit_of(k, bit);
+        const uint32_t pos = run_off[d] + (i - tile_off[d]);
+        kout[pos] = k;
+        if (HAS_VALS) vout[pos] = sv[i];
     }
 }
 
