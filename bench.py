@@ -330,7 +330,7 @@ def main():
                    "points": N, "octants": U, "balanced_octants": Bn, "elements": E, "nodes": Nn},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "kernel": "FP64 -lap+I matvec: k_matvec_pipe<double,false,4> + k_matvec_shared<double,false>",
+                     "kernel": "FP64 -lap+I matvec: k_matvec_pipe<double,false,4,0> + k_matvec_shared<double,false,0>",
                      "algorithmic_bytes": mv_bytes, "avg_launch_ms": mv_avg_ms},
         "matvec": {"da_blocks": block_info, "avg_ms": mv_avg_ms, "bracketed_avg_ms": mv_bracket_ms,
                    "bracketed_min_ms": float(np.min(mv_ms)),
