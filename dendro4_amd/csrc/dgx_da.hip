@@ -535,8 +535,8 @@ __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
 // numbering costs about 3x that.  The 16 lanes of one pass are 16 consecutive
 // element slots of the block at one record position p, a "lane group".  The
 // block's nodes are coloured greedily with 16 colours (colour = slot mod 16):
-// each node, in list order, takes the colour used least so far by the lane
-// groups it appears in (ties rotate with the node index), colour c holding
+// each node takes the colour used least so far by the lane groups it appears
+// in (ties rotate with the node index), colour c holding
 // exactly n/16 + (c < n mod 16) nodes, so that slot = 16 x (rank inside its
 // colour) + colour is again a permutation of 0..n-1.  The records and the
 // node list are then renumbered.  Deterministic; one wave per block.  Blocks
@@ -552,6 +552,7 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
     __shared__ uint16_t newidx[kColMaxN];
     __shared__ uint32_t tmp[kColMaxN];
     __shared__ uint32_t used[16];
+    __shared__ uint32_t pick[4];
     const uint32_t b = blockIdx.x, lane = threadIdx.x;
     const uint32_t s0 = boff[b], n = boff[b + 1] - s0;
     const size_t e0 = (size_t)b * eb;
@@ -590,23 +591,51 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
     }
     __syncthreads();
     const uint32_t base = n >> 4, extra = n & 15u;
-    // lane = 16 x k + j: candidate colour (j + l) mod 16, uses a + k, a + k + 4, ...
-    const uint32_t j = lane & 15u, k = lane >> 4;
-    for (uint32_t l = 0; l < n; ++l) {
-        const uint32_t a = ptr[l], z = ptr[l + 1], cj = (j + l) & 15u;
+    // Four nodes per step, one per 16-lane group, taken from the four quarters
+    // of the list (far apart, so they rarely share lane groups); lane j of a
+    // group scores candidate colour (j + l) mod 16.  Lane 0 then commits the
+    // four choices in order, moving a choice whose colour has filled up in
+    // this step to the next colour with room.
+    const uint32_t g = lane >> 4, j = lane & 15u, quarter = (n + 3) / 4;
+    for (uint32_t t = 0; t < quarter; ++t) {
+        const uint32_t l = g * quarter + t;
+        const bool has = l < n;
+        const uint32_t a = has ? ptr[l] : 0u, z = has ? ptr[l + 1] : 0u, cj = (j + l) & 15u;
         uint32_t pen = 0;
-        for (uint32_t q = a + k; q < z; q += 4) pen += cnt[adj[q] * 16u + cj];
-        pen += __shfl_xor(pen, 16, 64);
-        pen += __shfl_xor(pen, 32, 64);
-        uint32_t key = used[cj] < base + (cj < extra ? 1u : 0u) ? ((pen << 4) | j) : 0xffffffffu;
+        for (uint32_t q = a; q < z; q += 4) {  // four independent loads per step
+            uint32_t u[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[r] = q + r < z ? adj[q + r] : 0xffffu;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (u[r] != 0xffffu) pen += cnt[u[r] * 16u + cj];
+        }
+        uint32_t key = has && used[cj] < base + (cj < extra ? 1u : 0u) ? ((pen << 4) | j) : 0xffffffffu;
         for (int o = 8; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o, 64));
-        const uint32_t c = ((key & 15u) + l) & 15u;
-        for (uint32_t q = a + lane; q < z; q += 64) cnt[adj[q] * 16u + c] += 1;
+        if (j == 0) pick[g] = key == 0xffffffffu ? 0xffffffffu : ((key & 15u) + l) & 15u;
         __syncthreads();
         if (lane == 0) {
-            newidx[l] = (uint16_t)(used[c] * 16u + c);
-            used[c] += 1;
+            for (uint32_t h = 0; h < 4; ++h) {
+                const uint32_t lh = h * quarter + t;
+                if (lh >= n) continue;
+                uint32_t c = pick[h];
+                if (c == 0xffffffffu || used[c] >= base + (c < extra ? 1u : 0u)) {
+                    for (uint32_t d = 0; d < 16; ++d) {
+                        const uint32_t c2 = (lh + d) & 15u;
+                        if (used[c2] < base + (c2 < extra ? 1u : 0u)) {
+                            c = c2;
+                            break;
+                        }
+                    }
+                }
+                pick[h] = c;
+                newidx[lh] = (uint16_t)(used[c] * 16u + c);
+                used[c] += 1;
+            }
         }
+        __syncthreads();
+        if (has)
+            for (uint32_t q = a + j; q < z; q += 16) cnt[adj[q] * 16u + pick[g]] += 1;
         __syncthreads();
     }
     for (uint32_t t = lane; t < nuse; t += 64) rec[t] = (uint16_t)(newidx[rec[t] >> 3] << 3);
