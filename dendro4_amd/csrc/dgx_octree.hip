@@ -333,17 +333,26 @@ __global__ void k_unique_flag(const Key *k, size_t n, uint32_t *flag) {
     if (j >= n) return;
     flag[j] = (j == 0 || !key_eq(k[j], k[j - 1])) ? 1u : 0u;
 }
-__global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32_t *pos, size_t n, Key *out) {
+// Compaction of the flagged keys; also clears zmask[0..n) and *zcount for the
+// kernels that follow (one launch instead of two fills).
+__global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32_t *pos, size_t n, Key *out,
+                               uint32_t *zmask, uint32_t *zcount) {
     size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) *zcount = 0;
     if (j >= n) return;
+    zmask[j] = 0;
     if (flag[j]) out[pos[j]] = in[j];
 }
 
 // Parents of the 26 neighbours of every split octant: per axis the neighbours'
 // parents take at most two values, so at most 8 distinct parents per octant.
+// They go after the next level's nseed input parents, which the same launch
+// copies to out[0..nseed).
 __global__ void k_bal_nbr_parents(const Key *Q, const uint32_t *d_nq, uint32_t lvl, uint32_t md, Key *out,
-                                  uint32_t *counter) {
+                                  uint32_t *counter, const Key *seed, size_t nseed) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nseed) out[i] = seed[i];
+    out += nseed;
     // per axis: the octant's own parent coordinate and the neighbouring one
     int64_t cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
     uint32_t valid = 0;  // bit (bz*4 + by*2 + bx): candidate parent inside the domain
@@ -621,10 +630,9 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         k_unique_flag<<<grid_for(nw, B), B, 0, s>>>(W.p, nw, f.p);
         scan_exclusive_u32_async(f.p, fp.p, nw, d_nq, s);
         DevBuf<Key> Q(nw, s);  // capacity; *d_nq used
-        k_compact_keys<<<grid_for(nw, B), B, 0, s>>>(W.p, f.p, fp.p, nw, Q.p);
-        // leaves at level l: children of Q not split themselves
         DevBuf<uint32_t> cmask(nw, s), lc(nw, s), lo(nw, s);
-        HIP_OK(hipMemsetAsync(cmask.p, 0, nw * 4, s));
+        k_compact_keys<<<grid_for(nw, B), B, 0, s>>>(W.p, f.p, fp.p, nw, Q.p, cmask.p, d_nnb);
+        // leaves at level l: children of Q not split themselves
         if (nprev) k_mark_split_children<<<grid_for(nprev, B), B, 0, s>>>(prevQ.p, nprev, Q.p, d_nq, cmask.p);
         k_leaf_count<<<grid_for(nw, B), B, 0, s>>>(cmask.p, d_nq, nw, lc.p);
         scan_exclusive_u32_async(lc.p, lo.p, nw, d_nlv, s);
@@ -638,11 +646,10 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         // neighbours of the split octants seed the next coarser level
         DevBuf<Key> Wn;
         const size_t nin_next = l >= 2 ? hl[l - 1] : 0;
-        HIP_OK(hipMemsetAsync(d_nnb, 0, 4, s));
         if (l >= 2) {
             Wn.alloc(nin_next + 8 * nw, s);
-            copy_keys(Wn.p, inpar.p + loff[l - 1], nin_next, s);
-            k_bal_nbr_parents<<<grid_for(nw, B), B, 0, s>>>(Q.p, d_nq, l - 1, md, Wn.p + nin_next, d_nnb);
+            k_bal_nbr_parents<<<grid_for(std::max(nw, nin_next), B), B, 0, s>>>(Q.p, d_nq, l - 1, md, Wn.p, d_nnb,
+                                                                             inpar.p + loff[l - 1], nin_next);
             HIP_OK(hipGetLastError());
         }
         uint32_t hc[3];
