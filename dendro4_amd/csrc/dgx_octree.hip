@@ -440,6 +440,202 @@ __global__ void __launch_bounds__(B) k_import_tn(const uint32_t *tn, size_t n, u
     k[i] = dev_encode(s_rc, x, y, z, l, md);
 }
 
+// ---- consecutive small balance levels in one workgroup -------------------
+// The level step of balance_build for levels whose seeds fit kFuseCap keys,
+// run by one workgroup for as many consecutive levels as stay that small:
+// sort + unique the seeds (bitonic in LDS), mark the children that are split
+// themselves, emit the leaves, and collect the next level's seeds (the input
+// parents of that level, then the neighbour parents).  Leaves and neighbour
+// parents may come out in any order: seeds are sorted per level and the
+// leaves once at the end, exactly as in the multi-kernel step.
+constexpr uint32_t kFuseCap = 2048, kFuseB = 1024;
+struct FuseState {
+    uint32_t level;      // next level to process (0: all done)
+    uint32_t nw;         // its seeds: in wbuf[wsel], or the input parents of that level when from_inpar
+    uint32_t wsel, from_inpar;
+    uint32_t nprev;      // split octants of the last processed level, in qbuf[qsel]
+    uint32_t qsel;
+    uint32_t used;       // leaves emitted so far
+    uint32_t levels, seeds;
+};
+
+__device__ __forceinline__ uint32_t fuse_block_scan(uint32_t v, uint32_t *tmp, uint32_t &total) {
+    // exclusive scan over the kFuseB threads; tmp holds kFuseB/64 + 1 entries
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t w = 0; w < kFuseB / 64; ++w) {
+            const uint32_t t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+        }
+        tmp[kFuseB / 64] = acc;
+    }
+    __syncthreads();
+    const uint32_t r = x - v + tmp[wid];
+    total = tmp[kFuseB / 64];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *win, uint32_t nw0, const Key *prevq,
+                                                     uint32_t nprev0, const Key *inpar, const uint32_t *loff,
+                                                     uint32_t md, SfcPacked tab, Key *leaves, uint32_t used0,
+                                                     Key *wbuf0, Key *wbuf1, Key *qbuf0, Key *qbuf1, FuseState *st) {
+    LOAD_TABLES(tab)
+    __shared__ Key sk[kFuseCap];
+    __shared__ uint32_t cm[kFuseCap];
+    __shared__ uint32_t tmp[kFuseB / 64 + 1];
+    const uint32_t t = threadIdx.x;
+    uint32_t l = l0, nw = nw0, nprev = nprev0, used = used0, levels = 0, seeds = 0, k = 0;
+    const Key *W = win, *PQ = prevq;
+    bool from_inpar = false;
+    for (;;) {
+        // seeds of level l (1 <= nw <= kFuseCap): sort, then unique in place -> Q = sk[0..nq)
+        uint32_t P = 1;
+        while (P < nw) P <<= 1;
+        for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? W[i] : Key{~0ull, ~0ull};
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = t; i < P; i += kFuseB) {
+                    const uint32_t o = i ^ j;
+                    if (o > i) {
+                        const bool up = (i & kk) == 0;
+                        const Key a = sk[i], b = sk[o];
+                        if (up ? key_lt(b, a) : key_lt(a, b)) {
+                            sk[i] = b;
+                            sk[o] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint32_t i0 = 2 * t, i1 = 2 * t + 1;
+        const Key k0 = sk[i0 < kFuseCap ? i0 : 0], k1 = sk[i1 < kFuseCap ? i1 : 0];
+        const uint32_t f0 = i0 < nw && (i0 == 0 || !key_eq(k0, sk[i0 - 1])) ? 1u : 0u;
+        const uint32_t f1 = i1 < nw && !key_eq(k1, k0) ? 1u : 0u;
+        uint32_t nq;
+        const uint32_t qpos = fuse_block_scan(f0 + f1, tmp, nq);  // also orders the reads above before the writes
+        if (f0) sk[qpos] = k0;
+        if (f1) sk[qpos + f0] = k1;
+        __syncthreads();
+        Key *Qg = (k & 1) ? qbuf1 : qbuf0;  // this level's split octants, the next level's prevQ
+        for (uint32_t i = t; i < nq; i += kFuseB) {
+            Qg[i] = sk[i];
+            cm[i] = 0;
+        }
+        __syncthreads();
+        // children of Q that are split octants one level finer
+        for (uint32_t i = t; i < nprev; i += kFuseB) {
+            const Key c = PQ[i], par = key_parent(c);
+            const size_t j = lower_bound_lt(sk, 0, nq, par);
+            if (j < nq && key_eq(sk[j], par)) atomicOr(&cm[j], 1u << key_digit(c, key_level(c)));
+        }
+        __syncthreads();
+        // leaves of level l: the children of Q that are not split
+        const uint32_t c0 = i0 < nq ? 8u - __popc(cm[i0] & 255u) : 0u, c1 = i1 < nq ? 8u - __popc(cm[i1] & 255u) : 0u;
+        uint32_t nlv;
+        uint32_t lpos = used + fuse_block_scan(c0 + c1, tmp, nlv);
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t i = e ? i1 : i0;
+            if (i >= nq) continue;
+            uint32_t x, y, z, lq;
+            morton_decode(sk[i], md, x, y, z, lq);
+            const uint32_t len = 1u << (md - lq - 1), msk = cm[i];
+            for (uint32_t c = 0; c < 8; ++c) {
+                if (msk & (1u << c)) continue;
+                leaves[lpos++] = dev_encode(s_rc, x + (c & 1) * len, y + ((c >> 1) & 1) * len,
+                                            z + ((c >> 2) & 1) * len, lq + 1, md);
+            }
+        }
+        used += nlv;
+        levels += 1;
+        seeds += nw;
+        if (l < 2) {
+            l = 0;
+            break;
+        }
+        // seeds of level l-1: input parents of that level, then neighbour parents of Q
+        const uint32_t nin = loff[l] - loff[l - 1];
+        const Key *src = inpar + loff[l - 1];
+        Key *Wn = (k & 1) ? wbuf1 : wbuf0;
+        const uint32_t lvl = l - 1;
+        int64_t cx0[2], cx1[2], cy0[2], cy1[2], cz0[2], cz1[2];
+        uint32_t valid[2] = {0u, 0u};
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t i = e ? i1 : i0;
+            if (i >= nq) continue;
+            uint32_t x, y, z, lq;
+            morton_decode(sk[i], md, x, y, z, lq);
+            const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
+            cx0[e] = ((int64_t)x / plen) * plen;
+            cy0[e] = ((int64_t)y / plen) * plen;
+            cz0[e] = ((int64_t)z / plen) * plen;
+            cx1[e] = ((int64_t)x % plen) ? cx0[e] + plen : cx0[e] - plen;
+            cy1[e] = ((int64_t)y % plen) ? cy0[e] + plen : cy0[e] - plen;
+            cz1[e] = ((int64_t)z % plen) ? cz0[e] + plen : cz0[e] - plen;
+            const uint32_t okx = 1u | ((cx1[e] >= 0 && cx1[e] < dom) ? 2u : 0u);
+            const uint32_t oky = 1u | ((cy1[e] >= 0 && cy1[e] < dom) ? 2u : 0u);
+            const uint32_t okz = 1u | ((cz1[e] >= 0 && cz1[e] < dom) ? 2u : 0u);
+            for (int b = 0; b < 8; ++b)
+                if (((okx >> (b & 1)) & 1u) && ((oky >> ((b >> 1) & 1)) & 1u) && ((okz >> ((b >> 2) & 1)) & 1u))
+                    valid[e] |= 1u << b;
+        }
+        uint32_t nnb;
+        uint32_t npos = nin + fuse_block_scan(__popc(valid[0]) + __popc(valid[1]), tmp, nnb);
+        const uint32_t nwn = nin + nnb;  // <= largest level input count + 8 kFuseCap = the buffers' capacity
+        for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
+        for (uint32_t e = 0; e < 2; ++e)
+            for (int b = 0; b < 8; ++b)
+                if ((valid[e] >> b) & 1u) {
+                    const int64_t px = (b & 1) ? cx1[e] : cx0[e], py = (b & 2) ? cy1[e] : cy0[e],
+                                  pz = (b & 4) ? cz1[e] : cz0[e];
+                    Wn[npos++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
+                }
+        __syncthreads();
+        PQ = Qg;
+        nprev = nq;
+        l -= 1;
+        nw = nwn;
+        W = Wn;
+        from_inpar = false;
+        // levels without seeds: the next level starts from its input parents
+        // only, with no split octants below it
+        while (nw == 0 && l >= 1) {
+            nprev = 0;
+            if (l >= 2) {
+                nw = loff[l] - loff[l - 1];
+                W = inpar + loff[l - 1];
+                from_inpar = true;
+            }
+            l -= 1;
+        }
+        k += 1;
+        if (l == 0 || nw > kFuseCap) break;
+    }
+    if (t == 0) {
+        st->level = l;
+        st->nw = nw;
+        st->wsel = (k & 1) ? 0u : 1u;  // the buffer written last: (k - 1) & 1
+        st->from_inpar = from_inpar ? 1u : 0u;
+        st->nprev = nprev;
+        st->qsel = (k & 1) ? 0u : 1u;
+        st->used = used;
+        st->levels = levels;
+        st->seeds = seeds;
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host ----
@@ -611,7 +807,14 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     nw = hl[maxL];
     W.alloc(nw, s);
     copy_keys(W.p, inpar.p + loff[maxL], nw, s);
-    for (uint32_t l = maxL; l >= 1; --l) {
+    // buffers of the single-workgroup runs over small levels (k_bal_fused)
+    DevBuf<Key> fw0, fw1, fq0, fq1;
+    DevBuf<uint32_t> d_loff;
+    DevBuf<FuseState> fst;
+    uint32_t maxhl = 0;
+    for (int l = 0; l < 32; ++l) maxhl = std::max(maxhl, hl[l]);
+    uint32_t l = maxL;
+    while (l >= 1) {
         if (nw == 0) {
             prevQ.release();
             nprev = 0;
@@ -620,6 +823,44 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
                 W.alloc(nw, s);
                 copy_keys(W.p, inpar.p + loff[l - 1], nw, s);
             }
+            --l;
+            continue;
+        }
+        if (nw <= kFuseCap) {
+            // run levels l, l-1, ... in one workgroup while their seeds stay small
+            if (!fst.p) {
+                const size_t wcap = (size_t)maxhl + 8 * (size_t)kFuseCap;
+                fw0.alloc(wcap, s);
+                fw1.alloc(wcap, s);
+                fq0.alloc(kFuseCap, s);
+                fq1.alloc(kFuseCap, s);
+                d_loff.alloc(33, s);
+                HIP_OK(hipMemcpyAsync(d_loff.p, loff, 33 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+                fst.alloc(1, s);
+            }
+            const size_t need = used + (size_t)8 * kFuseCap * l;  // at most 8 leaves per split octant
+            if (leaves.n < need) {
+                DevBuf<Key> g(std::max(2 * leaves.n, need), s);
+                copy_keys(g.p, leaves.p, used, s);
+                leaves = std::move(g);
+            }
+            k_bal_fused<<<1, kFuseB, 0, s>>>(l, W.p, (uint32_t)nw, prevQ.p, (uint32_t)nprev, inpar.p, d_loff.p, md,
+                                             tab, leaves.p, (uint32_t)used, fw0.p, fw1.p, fq0.p, fq1.p, fst.p);
+            HIP_OK(hipGetLastError());
+            FuseState h;
+            HIP_OK(hipMemcpyAsync(&h, fst.p, sizeof h, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            used = h.used;
+            st.levels += h.levels;
+            st.seeds += h.seeds;
+            if (h.level == 0) break;
+            l = h.level;
+            nw = h.nw;
+            nprev = h.nprev;
+            prevQ.alloc(nprev, s);
+            copy_keys(prevQ.p, h.qsel ? fq1.p : fq0.p, nprev, s);
+            W.alloc(nw, s);
+            copy_keys(W.p, h.from_inpar ? inpar.p + loff[l] : (h.wsel ? fw1.p : fw0.p), nw, s);
             continue;
         }
         // parents live at level lv = l-1: only digits 1..lv vary
@@ -662,6 +903,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         nprev = hc[0];
         W = std::move(Wn);
         nw = nin_next + hc[2];
+        --l;
     }
     W.release();
     prevQ.release();
