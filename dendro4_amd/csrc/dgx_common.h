@@ -318,6 +318,9 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // keys without payload are sorted in one workgroup.
 void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128,
                      bool probe = true);
+// The same sort over the byte digits in which some key differs from the
+// others: vary_hi / vary_lo = OR over all keys of (key XOR any one key).
+void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo);
 // Exclusive scan of n uint32 values; returns the total (synchronises the stream).
 // *p = v in stream order (a kernel, so it is safe under graph capture and
 // needs no host staging).

@@ -50,23 +50,58 @@ namespace {
 
 constexpr int B = 256;
 
+// Level-maxDepth keys of the points.  Also ORs into vary[0] (hi) / vary[1]
+// (lo) every bit in which some key differs from point 0's key, so the sort
+// needs no separate pass to find its constant digits.
 __global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, double gx, double gy,
-                                                double gz, SfcPacked tab, Key *keys, uint32_t *bad) {
+                                                double gz, SfcPacked tab, Key *keys, uint32_t *bad,
+                                                unsigned long long *vary) {
     LOAD_TABLES(tab)
-    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    if (i >= n) return;
+    const size_t i = (size_t)blockIdx.x * B + threadIdx.x;
     const double scale = (double)(1u << md);
     // (unsigned)(x * 2^maxDepth / gLen), construct.cpp:933-941
-    double fx = xyz[3 * i] * scale / gx, fy = xyz[3 * i + 1] * scale / gy, fz = xyz[3 * i + 2] * scale / gz;
-    uint32_t px = 0, py = 0, pz = 0;
-    if (fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale) {
-        px = (uint32_t)fx;
-        py = (uint32_t)fy;
-        pz = (uint32_t)fz;
-    } else {
-        atomicOr(bad, 1u);
+    auto key_of = [&](size_t j, bool &inside) {
+        const double fx = xyz[3 * j] * scale / gx, fy = xyz[3 * j + 1] * scale / gy, fz = xyz[3 * j + 2] * scale / gz;
+        uint32_t px = 0, py = 0, pz = 0;
+        inside = fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale;
+        if (inside) {
+            px = (uint32_t)fx;
+            py = (uint32_t)fy;
+            pz = (uint32_t)fz;
+        }
+        return dev_encode(s_rc, px, py, pz, md, md);
+    };
+    unsigned long long dhi = 0, dlo = 0;
+    if (i < n) {
+        bool inside, inside0;
+        const Key k = key_of(i, inside), r = key_of(0, inside0);
+        if (!inside) atomicOr(bad, 1u);
+        keys[i] = k;
+        dhi = k.hi ^ r.hi;
+        dlo = k.lo ^ r.lo;
     }
-    keys[i] = dev_encode(s_rc, px, py, pz, md, md);
+    for (int o = 32; o > 0; o >>= 1) {
+        dhi |= __shfl_xor(dhi, o, 64);
+        dlo |= __shfl_xor(dlo, o, 64);
+    }
+    __shared__ unsigned long long wv[B / 64][2];
+    if ((threadIdx.x & 63) == 0) {
+        wv[threadIdx.x >> 6][0] = dhi;
+        wv[threadIdx.x >> 6][1] = dlo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < B / 64; ++w) {
+            dhi |= wv[w][0];
+            dlo |= wv[w][1];
+        }
+        // one atomic per workgroup, and only for bits not yet recorded: the
+        // mask fills after a few workgroups and the rest only read it
+        const unsigned long long ghi = __atomic_load_n(&vary[0], __ATOMIC_RELAXED);
+        const unsigned long long glo = __atomic_load_n(&vary[1], __ATOMIC_RELAXED);
+        if (dhi & ~ghi) atomicOr(&vary[0], dhi);
+        if (dlo & ~glo) atomicOr(&vary[1], dlo);
+    }
 }
 
 __global__ void k_p2o_lev(const Key *pk, size_t n, uint32_t m, uint32_t md, uint8_t *lev) {
@@ -656,13 +691,17 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
     }
     DevBuf<Key> pk(n, s);
     DevBuf<uint32_t> bad(1, s);
-    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
+    DevBuf<unsigned long long> vary(3, s);  // vary[0..1], then the out-of-domain flag in the low word of [2]
+    HIP_OK(hipMemsetAsync(vary.p, 0, 3 * sizeof(unsigned long long), s));
     const double gx = glens ? glens[0] : 1.0, gy = glens ? glens[1] : 1.0, gz = glens ? glens[2] : 1.0;
-    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p, bad.p);
+    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p,
+                                           reinterpret_cast<uint32_t *>(vary.p + 2), vary.p);
     HIP_OK(hipGetLastError());
-    radix_sort_keys(pk.p, nullptr, n, s, 0, 128);
-    uint32_t hbad = 0;
-    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+    unsigned long long hv[3];
+    HIP_OK(hipMemcpyAsync(hv, vary.p, sizeof hv, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    radix_sort_keys_mask(pk.p, nullptr, n, s, hv[0], hv[1]);
+    uint32_t hbad = (uint32_t)hv[2];
     // textbook leaves
     DevBuf<uint8_t> lev(n - m, s);
     k_p2o_lev<<<grid_for(n - m, B), B, 0, s>>>(pk.p, n, m, md, lev.p);

@@ -306,6 +306,10 @@ uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStre
     return h;
 }
 
+namespace {
+void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &passes, hipStream_t s);
+}  // namespace
+
 void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit, int end_bit, bool probe) {
     if (n < 2) return;
     if (n > 0xffffffffull) {
@@ -340,6 +344,31 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
         for (int p = 0; p < 16; ++p)
             if (!(8 * p + 8 <= begin_bit || 8 * p >= end_bit)) passes.push_back(8 * p);
     }
+    run_passes(keys, vals, n, passes, s);
+}
+
+void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo) {
+    if (n < 2) return;
+    if (n > 0xffffffffull) {
+        dgx_set_error("radix_sort_keys: %zu keys exceed the 32-bit index range", n);
+        throw DgxError();
+    }
+    if (!vals && n <= (size_t)SMALL_N) {
+        k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    std::vector<int> passes;
+    for (int p = 0; p < 16; ++p) {
+        const uint64_t w = p < 8 ? vary_lo : vary_hi;
+        if ((w >> (8 * (p & 7))) & 255u) passes.push_back(8 * p);
+    }
+    run_passes(keys, vals, n, passes, s);
+}
+
+namespace {
+
+void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &passes, hipStream_t s) {
     if (passes.empty()) return;
     size_t nblocks = (n + STILE - 1) / STILE;
     DevBuf<Key> ktmp(n, s);
@@ -364,3 +393,5 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
         if (vals) HIP_OK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     }
 }
+
+}  // namespace
