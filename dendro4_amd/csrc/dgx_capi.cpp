@@ -57,6 +57,7 @@ static hipStream_t S() { return g_ctx.stream; }
 
 #define API_BEGIN try {
 #define API_END                                                         \
+    sort_fault_check();                                                 \
     }                                                                   \
     catch (DgxError &) {                                                \
         return -1;                                                      \

@@ -199,14 +199,17 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
 
 // Histogram add of one count per valid lane into h[v] (LDS or global).  The
 // few most common values of the wave are counted with one atomic each
-// (ballot match), the rest with per-lane atomics, so clustered data do not
-// serialise 64 lanes on one address.  Every lane of the wave must call it.
+// (ballot match) while each still holds at least 8 lanes, the rest with
+// per-lane atomics: clustered data do not serialise 64 lanes on one address,
+// and scattered data pay no match rounds that save nothing.  Every lane of
+// the wave must call it.
 __device__ inline void wave_hist_add(uint32_t *h, uint32_t v, bool valid) {
     uint64_t act = __ballot(valid);
     for (int it = 0; it < 4 && act; ++it) {
         const int leader = __ffsll((unsigned long long)act) - 1;
         const uint32_t vl = __shfl(v, leader, 64);
         const uint64_t m = __ballot(valid && v == vl) & act;
+        if (__popcll(m) < 8) break;  // scattered values: one per-lane atomic is cheaper than more rounds
         if (__lane_id() == leader) atomicAdd(&h[vl], (uint32_t)__popcll(m));
         act &= ~m;
     }
@@ -311,16 +314,18 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // Primitives (dgx_sort.hip)
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of 128-bit keys with an optional 32-bit payload.
-// Bits [begin_bit, end_bit) of the 128-bit value (hi:lo) are sorted.  With
-// probe, one histogram pass (and one host read) finds the 8-bit digit
-// positions in which every key agrees and skips them; without, every position
-// overlapping the range is sorted and nothing is read back.  At most 2048
-// keys without payload are sorted in one workgroup.
-void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128,
-                     bool probe = true);
+// Bits [begin_bit, end_bit) of the 128-bit value (hi:lo) are sorted, one pass
+// per 8-bit digit position that overlaps the range; positions in which every
+// key agrees are detected and skipped on the device.  Nothing is read back:
+// the sort is stream-ordered.  At most 2048 keys without payload are sorted in
+// one workgroup.
+void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
 // The same sort over the byte digits in which some key differs from the
 // others: vary_hi / vary_lo = OR over all keys of (key XOR any one key).
 void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo);
+// Throws DgxError if a sort kernel reported a look-back wait that exceeded its
+// bound since the last check (the flag is cleared).
+void sort_fault_check();
 // Exclusive scan of n uint32 values; returns the total (synchronises the stream).
 // *p = v in stream order (a kernel, so it is safe under graph capture and
 // needs no host staging).

@@ -905,7 +905,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         // parents live at level lv = l-1: only digits 1..lv vary
         const uint32_t lv = l - 1;
         const int begin_bit = lv <= 21 ? 127 - 3 * (int)lv : 63 - 3 * ((int)lv - 21);
-        radix_sort_keys(W.p, nullptr, nw, s, begin_bit, 127, nw >= (size_t(1) << 18));
+        radix_sort_keys(W.p, nullptr, nw, s, begin_bit, 127);
         DevBuf<uint32_t> f(nw, s), fp(nw, s);
         k_unique_flag<<<grid_for(nw, B), B, 0, s>>>(W.p, nw, f.p);
         scan_exclusive_u32_async(f.p, fp.p, nw, d_nq, s);

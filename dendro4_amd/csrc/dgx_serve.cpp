@@ -236,6 +236,7 @@ struct dgx_server_s {
             b.release();
             HIP_OK(hipStreamSynchronize(s));
             d.r.s_hierarchy = now() - tb;
+            sort_fault_check();
         }
         std::vector<double>().swap(rq.xyz);
         const double tc = now();

@@ -1,14 +1,18 @@
 // dgx_sort.hip -- device-wide primitives: exclusive scan and stable LSD radix
 // sort of 128-bit SFC keys for gfx950.
 //
-// Radix sort: 8-bit digits, 256-thread blocks (4 wave64s).  One up-front pass
-// builds the global 256-bin histogram of every 8-bit digit position; passes
-// whose digit is constant over all keys are skipped.  Each remaining pass is
-// block histogram -> digit-major exclusive scan -> stable scatter, where the
-// scatter ranks keys inside a wave with 8 ballots (one per digit bit) and
-// orders the four waves of a block through LDS counters.
+// Radix sort (one-sweep): 8-bit digits, tiles of 2048 keys on 256-thread
+// blocks (4 wave64s).  One up-front pass builds the global 256-bin histogram
+// of every candidate digit position, and one workgroup turns them into global
+// digit offsets and skips positions whose digit is the same in every key, all
+// on the device.  Each remaining pass is ONE kernel: a tile counts its digits,
+// publishes the counts, ranks its keys stably (8 ballots per wave, LDS
+// counters across the four waves), obtains the counts of the tiles before it
+// by decoupled look-back, and writes its keys run by run.
 #include "dgx_common.h"
 
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -162,21 +166,31 @@ __global__ void __launch_bounds__(1024) k_sort_small(Key *keys, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += 1024) keys[i] = sk[i];
 }
 
-// Global histograms of all 16 digit positions (16 x 256 counters).
-__global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, uint32_t *ghist) {
+// Global histograms of the digit positions in pmask (bit p: the byte at bit
+// 8p), into 16 x 256 counters.  Four keys per thread are loaded ahead.
+constexpr int GH_U = 4;
+__global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, uint32_t *ghist, uint32_t pmask) {
     __shared__ uint32_t h[16 * 256];
     for (int i = threadIdx.x; i < 16 * 256; i += SB) h[i] = 0;
     __syncthreads();
-    size_t stride = (size_t)gridDim.x * SB;
+    const size_t stride = (size_t)gridDim.x * SB * GH_U;
     const size_t nround = (n + stride - 1) / stride * stride;  // whole waves stay in the loop
-    for (size_t i = (size_t)blockIdx.x * SB + threadIdx.x; i < nround; i += stride) {
-        const bool valid = i < n;
-        Key k{0ull, 0ull};
-        if (valid) k = keys[i];
+    for (size_t i0 = (size_t)blockIdx.x * SB * GH_U + threadIdx.x; i0 < nround; i0 += stride) {
+        Key k[GH_U];
 #pragma unroll
-        for (int p = 0; p < 8; ++p) {
-            wave_hist_add(&h[p * 256], (uint32_t)((k.lo >> (8 * p)) & 255u), valid);
-            wave_hist_add(&h[(p + 8) * 256], (uint32_t)((k.hi >> (8 * p)) & 255u), valid);
+        for (int u = 0; u < GH_U; ++u) {
+            const size_t i = i0 + (size_t)u * SB;
+            k[u] = i < n ? keys[i] : Key{0ull, 0ull};
+        }
+#pragma unroll
+        for (int u = 0; u < GH_U; ++u) {
+            const bool valid = i0 + (size_t)u * SB < n;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                if (pmask & (1u << p)) wave_hist_add(&h[p * 256], (uint32_t)((k[u].lo >> (8 * p)) & 255u), valid);
+                if (pmask & (1u << (p + 8)))
+                    wave_hist_add(&h[(p + 8) * 256], (uint32_t)((k[u].hi >> (8 * p)) & 255u), valid);
+            }
         }
     }
     __syncthreads();
@@ -184,87 +198,185 @@ __global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, u
         if (h[i]) atomicAdd(&ghist[i], h[i]);
 }
 
-__global__ void __launch_bounds__(SB) k_block_hist(const Key *keys, size_t n, int bit, uint32_t *bhist,
-                                                   uint32_t nblocks) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    size_t base = (size_t)blockIdx.x * STILE;
-#pragma unroll
-    for (int i = 0; i < SITEMS; ++i) {
-        size_t idx = base + (size_t)i * SB + threadIdx.x;
-        const bool valid = idx < n;
-        wave_hist_add(h, valid ? digit_of(keys[idx], bit) : 0u, valid);
-    }
-    __syncthreads();
-    bhist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+// Look-back status of one (tile, digit) of one pass: bits 0-31 the count,
+// bits 32-61 the pass's epoch (its index + 1; the words are zeroed once per
+// sort), bits 62-63 what the count is: the tile's own count (AGG) or the
+// count of that digit in all tiles up to and including it (INCL).
+constexpr uint64_t ST_AGG = 1ull << 62, ST_INCL = 2ull << 62;
+constexpr uint32_t SPIN_LIMIT = 1u << 20;  // polls of one status word before giving up
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Stable scatter of one tile (STILE keys).  Keys are ranked inside the tile
-// (8 ballots per wave, LDS counters across the 4 waves, rounds in order),
-// staged in LDS in digit order, and then written run by run: consecutive
-// threads store consecutive addresses of one digit's output run.  The tile's
-// count of digit d is the difference of consecutive entries of the scanned
-// digit-major histogram.
-template <bool HAS_VALS>
-__global__ void __launch_bounds__(SB) k_scatter(const Key *kin, const uint32_t *vin, Key *kout, uint32_t *vout,
-                                                size_t n, int bit, const uint32_t *boff, uint32_t nblocks) {
-    __shared__ uint32_t run_off[256];   // output position of the tile's first key of digit d
-    __shared__ uint32_t tile_off[256];  // start of digit d in the staged tile
-    __shared__ uint32_t seen[256];      // keys of digit d staged so far
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t scan_tmp[SB / 64 + 1];
-    __shared__ Key sk[STILE];
-    __shared__ uint32_t sv[HAS_VALS ? STILE : 1];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const size_t hi_idx = (size_t)tid * nblocks + blockIdx.x;
-    const uint32_t cur = boff[hi_idx];
-    const uint32_t nxt = hi_idx + 1 < (size_t)256 * nblocks ? boff[hi_idx + 1] : (uint32_t)n;
-    run_off[tid] = cur;
-    uint32_t total;
-    tile_off[tid] = block_exclusive_scan(nxt - cur, scan_tmp, total);
-    seen[tid] = 0;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t base = (size_t)blockIdx.x * STILE;
-    for (int r = 0; r < SITEMS; ++r) {
-        size_t idx = base + (size_t)r * SB + tid;
-        bool valid = idx < n;
-        Key k{0ull, 0ull};
-        uint32_t v = 0;
-        if (valid) {
-            k = kin[idx];
-            if (HAS_VALS) v = vin[idx];
+// Candidate passes of one sort and what the plan kernel derives from the
+// global histograms: whether the pass's digit varies at all, which buffer
+// holds its input (0 = caller's, 1 = temporary), and the exclusive global
+// offset of every digit value.
+struct SortPlan {
+    uint32_t active[16];
+    uint32_t src[17];  // src[npass]: the buffer holding the sorted result
+    uint32_t gofs[16][256];
+};
+
+// One workgroup: per candidate pass j (digit position pos[j] = (packed >> 4j) & 15),
+// a digit position is skipped when one digit value holds all n keys.
+__global__ void __launch_bounds__(256) k_sort_plan(const uint32_t *ghist, uint32_t n, uint64_t packed, int npass,
+                                                   SortPlan *plan) {
+    __shared__ uint32_t scan_tmp[256 / 64 + 1];
+    const int tid = threadIdx.x;
+    uint32_t src = 0;
+    for (int j = 0; j < npass; ++j) {
+        const uint32_t p = (uint32_t)(packed >> (4 * j)) & 15u;
+        const uint32_t c = ghist[p * 256 + tid];
+        const bool trivial = __syncthreads_or(c == n) != 0;
+        uint32_t total;
+        plan->gofs[j][tid] = block_exclusive_scan(c, scan_tmp, total);
+        if (tid == 0) {
+            plan->active[j] = trivial ? 0u : 1u;
+            plan->src[j] = src;
         }
-        uint32_t d = valid ? digit_of(k, bit) : 0u;
+        if (!trivial) src ^= 1u;
+    }
+    if (tid == 0) plan->src[npass] = src;
+}
+
+// One pass of the one-sweep LSD sort over the 8-bit digit at `bit`, on tiles
+// of NT x SITEMS keys.  Tiles are taken in the order of a global counter, so a
+// tile only ever waits for tiles that are already running.  A tile: loads its
+// keys; ranks them stably (per round of NT keys: 8 ballots give each lane its
+// peers of equal digit within the wave, one lane per group records the group
+// size, LDS counters order the waves and the rounds), which also yields the
+// tile's digit counts; publishes the counts (AGG); stages the keys in digit
+// order in LDS; sums the counts of the tiles before it by walking back until
+// a tile that already published its inclusive count (INCL); publishes its
+// own inclusive count; and writes the staged keys run by run to global digit
+// offset + earlier tiles' count + position in the run.  The first 256
+// threads own one digit each for the counters and the look-back.
+template <bool HAS_VALS, int NT>
+__global__ void __launch_bounds__(NT) k_onesweep(Key *ka, Key *kb, uint32_t *va, uint32_t *vb, uint32_t n, int bit,
+                                                 int j, const SortPlan *plan, uint64_t *status, uint32_t *tile_ctr,
+                                                 uint32_t *fault) {
+    constexpr int NW = NT / 64, TILE = NT * SITEMS;
+    __shared__ uint32_t run_off[256];   // output position of staged index 0 of digit d's run, minus its start
+    __shared__ uint32_t tile_off[256];  // start of digit d in the staged tile
+    __shared__ uint32_t seen[256];      // keys of digit d ranked so far
+    __shared__ uint32_t wcnt[NW][256];  // keys of digit d in wave w of the current round
+    __shared__ uint32_t scan_tmp[NW + 1];
+    __shared__ uint32_t s_tile;
+    __shared__ Key sk[TILE];
+    __shared__ uint32_t sv[HAS_VALS ? TILE : 1];
+    if (!plan->active[j]) return;  // the digit is the same in every key
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const bool from_b = plan->src[j] != 0;
+    const Key *kin = from_b ? kb : ka;
+    Key *kout = from_b ? ka : kb;
+    const uint32_t *vin = from_b ? vb : va;
+    uint32_t *vout = from_b ? va : vb;
+    const uint64_t epoch = (uint64_t)(j + 1) << 32;
+    if (tid == 0) s_tile = atomicAdd(&tile_ctr[j], 1u);
+    if (tid < 256) seen[tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)TILE;
+    Key k[SITEMS];
+    uint32_t v[SITEMS], d[SITEMS], prank[SITEMS];
 #pragma unroll
-        for (int w = 0; w < 4; ++w) wcnt[w][tid] = 0;
+    for (int r = 0; r < SITEMS; ++r) {
+        const uint32_t idx = base + (uint32_t)(r * NT + tid);
+        const bool valid = idx < n;
+        k[r] = valid ? kin[idx] : Key{0ull, 0ull};
+        v[r] = (HAS_VALS && valid) ? vin[idx] : 0u;
+        d[r] = digit_of(k[r], bit);
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int r = 0; r < SITEMS; ++r) {
+        const bool valid = base + (uint32_t)(r * NT + tid) < n;
+        if (tid < 256)
+            for (int w = 0; w < NW; ++w) wcnt[w][tid] = 0;
         __syncthreads();
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            uint64_t bal = __ballot((d >> b) & 1u);
-            peers &= ((d >> b) & 1u) ? bal : ~bal;
+            const uint64_t bal = __ballot((d[r] >> b) & 1u);
+            peers &= ((d[r] >> b) & 1u) ? bal : ~bal;
         }
-        uint32_t rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[wid][d] = __popcll(peers);
+        const uint32_t rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) wcnt[wid][d[r]] = __popcll(peers);
         __syncthreads();
-        if (valid) {
-            uint32_t pos = tile_off[d] + seen[d] + rank;
-            for (int w = 0; w < wid; ++w) pos += wcnt[w][d];
-            sk[pos] = k;
-            if (HAS_VALS) sv[pos] = v;
+        uint32_t pos = seen[d[r]] + rank;
+        for (int w = 0; w < wid; ++w) pos += wcnt[w][d[r]];
+        prank[r] = pos;
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t a = 0;
+            for (int w = 0; w < NW; ++w) a += wcnt[w][tid];
+            seen[tid] += a;
         }
-        __syncthreads();
-        seen[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
     }
     __syncthreads();
-    const uint32_t tn = (uint32_t)min((size_t)STILE, n - base);
-    for (uint32_t i = tid; i < tn; i += SB) {
-        const Key k = sk[i];
-        const uint32_t d = digit_of(k, bit);
-        const uint32_t pos = run_off[d] + (i - tile_off[d]);
-        kout[pos] = k;
-        if (HAS_VALS) vout[pos] = sv[i];
+    const uint32_t cnt = tid < 256 ? seen[tid] : 0u;
+    uint64_t *mine = status + (size_t)tile * 256 + (tid & 255);
+    if (tid < 256) status_store(mine, (tile == 0 ? ST_INCL : ST_AGG) | epoch | cnt);
+    uint32_t total;
+    const uint32_t toff = block_exclusive_scan(cnt, scan_tmp, total);
+    if (tid < 256) tile_off[tid] = toff;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SITEMS; ++r)
+        if (base + (uint32_t)(r * NT + tid) < n) {
+            const uint32_t pos = tile_off[d[r]] + prank[r];
+            sk[pos] = k[r];
+            if (HAS_VALS) sv[pos] = v[r];
+        }
+    // count of digit tid in the tiles before this one
+    if (tid < 256) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+            uint32_t t = tile - 1;
+            for (;;) {
+                uint64_t w = status_load(status + (size_t)t * 256 + tid);
+                uint32_t spins = 0;
+                while ((w & ~(3ull << 62) & ~0xffffffffull) != epoch) {
+                    if (++spins > SPIN_LIMIT) {
+                        __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                    w = status_load(status + (size_t)t * 256 + tid);
+                }
+                excl += (uint32_t)w;
+                if ((w & ST_INCL) || t == 0 || spins > SPIN_LIMIT) break;
+                --t;
+            }
+            status_store(mine, ST_INCL | epoch | (excl + cnt));
+        }
+        run_off[tid] = plan->gofs[j][tid] + excl - toff;
+    }
+    __syncthreads();
+    const uint32_t tn = min((uint32_t)TILE, n - base);
+    for (uint32_t i = tid; i < tn; i += NT) {
+        const Key key = sk[i];
+        const uint32_t pos = run_off[digit_of(key, bit)] + i;
+        if (pos < n) {  // always true unless a look-back wait gave up (then the fault word is set)
+            kout[pos] = key;
+            if (HAS_VALS) vout[pos] = sv[i];
+        }
+    }
+}
+
+// Copies the result back into the caller's buffers when the last active pass
+// wrote the temporary ones.
+__global__ void k_sort_finish(Key *ka, const Key *kb, uint32_t *va, const uint32_t *vb, size_t n, int npass,
+                              const SortPlan *plan) {
+    if (!plan->src[npass]) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        ka[i] = kb[i];
+        if (va) va[i] = vb[i];
     }
 }
 
@@ -308,51 +420,53 @@ uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStre
 
 namespace {
 void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &passes, hipStream_t s);
+
+// Pinned host word that the sort kernels set when a look-back wait exceeds its
+// bound (it never should: every wait is for a tile that is already running).
+uint32_t *g_fault = nullptr;
+std::once_flag g_fault_once;
+uint32_t *fault_word() {
+    std::call_once(g_fault_once, [] {
+        void *p = nullptr;
+        HIP_OK(hipHostMalloc(&p, 64, hipHostMallocMapped));
+        std::memset(p, 0, 64);
+        g_fault = (uint32_t *)p;
+    });
+    return g_fault;
+}
 }  // namespace
 
-void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit, int end_bit, bool probe) {
-    if (n < 2) return;
-    if (n > 0xffffffffull) {
+void sort_fault_check() {
+    if (g_fault && __atomic_exchange_n(g_fault, 0u, __ATOMIC_ACQ_REL)) {
+        dgx_set_error("radix sort: a look-back wait exceeded its bound; results of that sort are invalid");
+        throw DgxError();
+    }
+}
+
+static void check_sort_size(size_t n) {
+    if (n > 0xffffffffull - 1024 * SITEMS) {  // 32-bit indices, up to the largest tile past the end
         dgx_set_error("radix_sort_keys: %zu keys exceed the 32-bit index range", n);
         throw DgxError();
     }
+}
+
+void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit, int end_bit) {
+    if (n < 2) return;
+    check_sort_size(n);
     if (!vals && n <= (size_t)SMALL_N) {
         k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
         HIP_OK(hipGetLastError());
         return;
     }
     std::vector<int> passes;
-    if (probe) {
-        // skip digit positions that are constant over all keys (one host read)
-        DevBuf<uint32_t> ghist(16 * 256, s);
-        HIP_OK(hipMemsetAsync(ghist.p, 0, 16 * 256 * sizeof(uint32_t), s));
-        unsigned gb = (unsigned)std::min<size_t>(grid_for(n, SB), 2048);
-        k_global_hist<<<gb, SB, 0, s>>>(keys, n, ghist.p);
-        HIP_OK(hipGetLastError());
-        std::vector<uint32_t> h(16 * 256);
-        HIP_OK(hipMemcpyAsync(h.data(), ghist.p, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        for (int p = 0; p < 16; ++p) {
-            int bit = 8 * p;
-            if (bit + 8 <= begin_bit || bit >= end_bit) continue;
-            bool trivial = false;
-            for (int d = 0; d < 256; ++d)
-                if (h[p * 256 + d] == n) trivial = true;
-            if (!trivial) passes.push_back(bit);
-        }
-    } else {
-        for (int p = 0; p < 16; ++p)
-            if (!(8 * p + 8 <= begin_bit || 8 * p >= end_bit)) passes.push_back(8 * p);
-    }
+    for (int p = 0; p < 16; ++p)
+        if (!(8 * p + 8 <= begin_bit || 8 * p >= end_bit)) passes.push_back(8 * p);
     run_passes(keys, vals, n, passes, s);
 }
 
 void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo) {
     if (n < 2) return;
-    if (n > 0xffffffffull) {
-        dgx_set_error("radix_sort_keys: %zu keys exceed the 32-bit index range", n);
-        throw DgxError();
-    }
+    check_sort_size(n);
     if (!vals && n <= (size_t)SMALL_N) {
         k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
         HIP_OK(hipGetLastError());
@@ -368,30 +482,64 @@ void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, ui
 
 namespace {
 
+// Threads per tile: larger tiles mean fewer look-back steps and fewer
+// barriers per key, smaller ones more workgroups for small sorts.  A payload
+// needs 4 more bytes of LDS per key, so payload sorts keep 256.
+int sort_threads(size_t n, bool has_vals) {
+    if (has_vals) return 256;
+    if (n >= (size_t(1) << 20)) return 1024;
+    if (n >= (size_t(1) << 18)) return 512;
+    return 256;
+}
+
+// Workspace of one sort, zeroed by one fill: 16 x 256 global counters, 16
+// tile counters, then the plan and the look-back status words.
 void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &passes, hipStream_t s) {
     if (passes.empty()) return;
-    size_t nblocks = (n + STILE - 1) / STILE;
+    const int np = (int)passes.size();
+    const int nt = sort_threads(n, vals != nullptr);
+    const size_t ntiles = (n + (size_t)nt * SITEMS - 1) / ((size_t)nt * SITEMS);
+    const size_t off_plan = 16 * 256 * 4 + 16 * 4;
+    const size_t off_status = (off_plan + sizeof(SortPlan) + 255) / 256 * 256;
+    const size_t ws_bytes = off_status + ntiles * 256 * sizeof(uint64_t);
+    DevBuf<uint8_t> ws(ws_bytes, s);
+    uint32_t *ghist = (uint32_t *)ws.p, *ctr = ghist + 16 * 256;
+    SortPlan *plan = (SortPlan *)(ws.p + off_plan);
+    uint64_t *status = (uint64_t *)(ws.p + off_status);
+    HIP_OK(hipMemsetAsync(ws.p, 0, ws_bytes, s));
+    uint32_t pmask = 0;
+    uint64_t packed = 0;
+    for (int j = 0; j < np; ++j) {
+        pmask |= 1u << (passes[j] / 8);
+        packed |= (uint64_t)(passes[j] / 8) << (4 * j);
+    }
+    const unsigned gb = (unsigned)std::min<size_t>(grid_for(n, SB * GH_U), 1024);
+    k_global_hist<<<gb, SB, 0, s>>>(keys, n, ghist, pmask);
+    k_sort_plan<<<1, 256, 0, s>>>(ghist, (uint32_t)n, packed, np, plan);
+    HIP_OK(hipGetLastError());
     DevBuf<Key> ktmp(n, s);
     DevBuf<uint32_t> vtmp(vals ? n : 0, s);
-    DevBuf<uint32_t> bhist(256 * nblocks, s);
-    Key *kin = keys, *kout = ktmp.p;
-    uint32_t *vin = vals, *vout = vtmp.p;
-    for (int bit : passes) {
-        k_block_hist<<<(unsigned)nblocks, SB, 0, s>>>(kin, n, bit, bhist.p, (uint32_t)nblocks);
-        scan_exclusive_u32_async(bhist.p, bhist.p, 256 * nblocks, nullptr, s);
+    uint32_t *fault = fault_word();
+    const unsigned grid = (unsigned)ntiles;
+    for (int j = 0; j < np; ++j) {
+        auto *kp = keys, *kt = ktmp.p;
         if (vals)
-            k_scatter<true><<<(unsigned)nblocks, SB, 0, s>>>(kin, vin, kout, vout, n, bit, bhist.p, (uint32_t)nblocks);
+            k_onesweep<true, 256><<<grid, 256, 0, s>>>(kp, kt, vals, vtmp.p, (uint32_t)n, passes[j], j, plan, status,
+                                                       ctr, fault);
+        else if (nt == 1024)
+            k_onesweep<false, 1024><<<grid, 1024, 0, s>>>(kp, kt, nullptr, nullptr, (uint32_t)n, passes[j], j, plan,
+                                                          status, ctr, fault);
+        else if (nt == 512)
+            k_onesweep<false, 512><<<grid, 512, 0, s>>>(kp, kt, nullptr, nullptr, (uint32_t)n, passes[j], j, plan,
+                                                        status, ctr, fault);
         else
-            k_scatter<false><<<(unsigned)nblocks, SB, 0, s>>>(kin, nullptr, kout, nullptr, n, bit, bhist.p,
-                                                             (uint32_t)nblocks);
+            k_onesweep<false, 256><<<grid, 256, 0, s>>>(kp, kt, nullptr, nullptr, (uint32_t)n, passes[j], j, plan,
+                                                        status, ctr, fault);
         HIP_OK(hipGetLastError());
-        std::swap(kin, kout);
-        std::swap(vin, vout);
     }
-    if (kin != keys) {
-        HIP_OK(hipMemcpyAsync(keys, kin, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
-        if (vals) HIP_OK(hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    }
+    k_sort_finish<<<(unsigned)std::min<size_t>(grid_for(n, 256), 2048), 256, 0, s>>>(keys, ktmp.p, vals, vtmp.p, n,
+                                                                                     np, plan);
+    HIP_OK(hipGetLastError());
 }
 
 }  // namespace

@@ -178,6 +178,38 @@ def test_sample_sort(dgx, oracle):
     assert np.array_equal(dgx.sample_sort(t[perm]), t)
 
 
+def random_octants(rng, m, lmin, lmax, md=30):
+    lev = rng.integers(lmin, lmax + 1, m).astype(np.uint32)
+    xyz = rng.integers(0, 1 << md, (m, 3), dtype=np.uint64).astype(np.uint32)
+    xyz &= ~((np.uint32(1) << (md - lev)) - np.uint32(1))[:, None]
+    tn = np.zeros((m, 7), np.uint32)
+    tn[:, :3], tn[:, 3], tn[:, 4], tn[:, 5], tn[:, 6] = xyz, lev, 1, 3, md
+    return tn
+
+
+@pytest.mark.parametrize("case", ["random", "duplicates", "one_subtree", "coarse"])
+def test_sort_many_tiles(dgx, oracle, case):
+    """The device radix sort over hundreds of tiles against the oracle's sort:
+    octants of all levels (every digit position varies), each octant repeated
+    many times (long runs of equal keys across tiles), octants inside one
+    level-7 subtree (the leading digit positions are constant and skipped), and
+    coarse octants (the trailing positions are constant)."""
+    rng = np.random.default_rng(7)
+    if case == "random":
+        tn = random_octants(rng, 700001, 0, 30)
+    elif case == "duplicates":
+        tn = np.repeat(random_octants(rng, 40000, 3, 25), 23, axis=0)
+    elif case == "one_subtree":
+        tn = random_octants(rng, 500000, 8, 30)
+        tn[:, :3] = (tn[:, :3] & ((1 << 23) - 1)) | (np.uint32(5) << 23)
+    else:
+        tn = random_octants(rng, 300000, 0, 7)
+    tn = tn[rng.permutation(len(tn))]
+    want = oracle.sort(tn)
+    print(case, len(tn), flush=True)
+    assert np.array_equal(dgx.sample_sort(tn), want)
+
+
 def test_uniform_256k_full_parity(dgx, oracle, stencils):
     """Config 1 at its full size: 262,144 uniform points, maxNumPts=1."""
     pts = dgx.workload_points("uniform-256K")
