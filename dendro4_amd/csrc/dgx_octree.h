@@ -25,6 +25,31 @@ __device__ inline Key dev_encode(const uint8_t *rc, uint32_t x, uint32_t y, uint
     k.lo |= lev;
     return k;
 }
+// dev_encode that also returns the curve's rotation state below the octant,
+// so that its children's keys need one table read each (child_key).
+__device__ inline Key dev_encode_rot(const uint8_t *rc, uint32_t x, uint32_t y, uint32_t z, uint32_t lev, uint32_t md,
+                                     uint32_t &rot) {
+    Key k{0ull, 0ull};
+    rot = 0;
+    for (uint32_t t = 1; t <= lev; ++t) {
+        uint32_t b = md - t;
+        uint32_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+        uint32_t v = rc[rot * 8 + m];
+        rot = v >> 3;
+        if (t <= 21) k.hi |= (uint64_t)(v & 7u) << (63 - 3 * t);
+        else k.lo |= (uint64_t)(v & 7u) << (63 - 3 * (t - 21));
+    }
+    k.lo |= lev;
+    return k;
+}
+// Key of Morton child m (bit 0: +x, 1: +y, 2: +z) of the octant with key p
+// and rotation state rot (from dev_encode_rot); equals dev_encode of the child.
+__device__ inline Key child_key(const uint8_t *rc, Key p, uint32_t rot, uint32_t m) {
+    const uint32_t t = key_level(p) + 1;
+    key_set_digit(p, t, rc[rot * 8 + m] & 7u);
+    p.lo = (p.lo & ~31ull) | t;
+    return p;
+}
 __device__ inline void dev_decode(const uint8_t *mc, const Key &k, uint32_t md, uint32_t &x, uint32_t &y, uint32_t &z,
                                   uint32_t &lev) {
     lev = key_level(k);

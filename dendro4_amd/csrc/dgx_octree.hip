@@ -52,7 +52,8 @@ constexpr int B = 256;
 
 // Level-maxDepth keys of the points.  Also ORs into vary[0] (hi) / vary[1]
 // (lo) every bit in which some key differs from point 0's key, so the sort
-// needs no separate pass to find its constant digits.
+// needs no separate pass to find its constant digits.  Each block encodes
+// point 0's key once (thread 0), not once per thread.
 __global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, double gx, double gy,
                                                 double gz, SfcPacked tab, Key *keys, uint32_t *bad,
                                                 unsigned long long *vary) {
@@ -71,14 +72,28 @@ __global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uin
         }
         return dev_encode(s_rc, px, py, pz, md, md);
     };
-    unsigned long long dhi = 0, dlo = 0;
+    // every varying bit is a bit in which some key differs from the block's
+    // first key, or in which that key differs from point 0's key
+    __shared__ Key first;
+    Key k{0ull, 0ull};
     if (i < n) {
-        bool inside, inside0;
-        const Key k = key_of(i, inside), r = key_of(0, inside0);
+        bool inside;
+        k = key_of(i, inside);
         if (!inside) atomicOr(bad, 1u);
         keys[i] = k;
-        dhi = k.hi ^ r.hi;
-        dlo = k.lo ^ r.lo;
+    }
+    if (threadIdx.x == 0) first = k;
+    __syncthreads();
+    unsigned long long dhi = 0, dlo = 0;
+    if (i < n) {
+        dhi = k.hi ^ first.hi;
+        dlo = k.lo ^ first.lo;
+    }
+    if (threadIdx.x == 0 && i < n) {
+        bool inside0;
+        const Key r = key_of(0, inside0);
+        dhi |= k.hi ^ r.hi;
+        dlo |= k.lo ^ r.lo;
     }
     for (int o = 32; o > 0; o >>= 1) {
         dhi |= __shfl_xor(dhi, o, 64);
@@ -335,32 +350,50 @@ __global__ void k_p2o_replay(const Key *P, size_t n, uint32_t m, uint32_t md, co
 }
 
 // ------------------------------------------------------------ balance ----
+// Histogram of the input levels, 8 keys per thread: few blocks, so few global
+// atomics on the 32 counters.
+constexpr int LH_ITEMS = 8;
 __global__ void __launch_bounds__(B) k_level_hist(const Key *k, size_t n, uint32_t *hist) {
     __shared__ uint32_t h[32];
     if (threadIdx.x < 32) h[threadIdx.x] = 0;
     __syncthreads();
-    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    wave_hist_add(h, i < n ? key_level(k[i]) : 0u, i < n);
+    const size_t base = (size_t)blockIdx.x * B * LH_ITEMS + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < LH_ITEMS; ++r) {
+        const size_t i = base + (size_t)r * B;
+        wave_hist_add(h, i < n ? key_level(k[i]) : 0u, i < n);
+    }
     __syncthreads();
     if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 // Parents (as Morton keys) of the input octants, bucketed by the input level.
-// Each block reserves its share of every level bucket with one atomic per level.
+// Each block (8 keys per thread) reserves its share of every level bucket
+// with one atomic per level; the order inside a bucket is free.
 __global__ void __launch_bounds__(B) k_bucket_parents(const Key *k, size_t n, uint32_t md, SfcPacked tab,
                                                       uint32_t *cursor /*32, pre-offset*/, Key *out) {
     LOAD_TABLES(tab)
     __shared__ uint32_t cnt[32], base[32];
     if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
     __syncthreads();
-    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    uint32_t x = 0, y = 0, z = 0, l = 0;
-    if (i < n) dev_decode(s_mc, k[i], md, x, y, z, l);
-    const uint32_t loc = wave_bucket_slot(cnt, l, i < n);
+    const size_t i0 = (size_t)blockIdx.x * B * LH_ITEMS + threadIdx.x;
+    uint32_t loc[LH_ITEMS], lv[LH_ITEMS];
+    Key par[LH_ITEMS];
+#pragma unroll
+    for (int r = 0; r < LH_ITEMS; ++r) {
+        const size_t i = i0 + (size_t)r * B;
+        uint32_t x = 0, y = 0, z = 0, l = 0;
+        if (i < n) dev_decode(s_mc, k[i], md, x, y, z, l);
+        loc[r] = wave_bucket_slot(cnt, l, i < n);
+        lv[r] = l;
+        par[r] = key_parent(morton_encode(x, y, z, l, md));
+    }
     __syncthreads();
     if (threadIdx.x < 32) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]) : 0u;
     __syncthreads();
-    if (i < n) out[base[l] + loc] = key_parent(morton_encode(x, y, z, l, md));
+#pragma unroll
+    for (int r = 0; r < LH_ITEMS; ++r)
+        if (i0 + (size_t)r * B < n) out[base[lv[r]] + loc[r]] = par[r];
 }
 
 __global__ void k_unique_flag(const Key *k, size_t n, uint32_t *flag) {
@@ -443,12 +476,13 @@ __global__ void __launch_bounds__(B) k_leaf_emit(const Key *Q, const uint32_t *d
     LOAD_TABLES(tab)
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
     if (i >= *d_nq) return;
-    uint32_t x, y, z, l;
+    uint32_t x, y, z, l, rot;
     morton_decode(Q[i], md, x, y, z, l);
-    uint32_t len = 1u << (md - l - 1), o = off[i], msk = cmask[i];
+    const Key pk = dev_encode_rot(s_rc, x, y, z, l, md, rot);
+    uint32_t o = off[i], msk = cmask[i];
     for (uint32_t c = 0; c < 8; ++c) {
         if (msk & (1u << c)) continue;
-        out[o++] = dev_encode(s_rc, x + (c & 1) * len, y + ((c >> 1) & 1) * len, z + ((c >> 2) & 1) * len, l + 1, md);
+        out[o++] = child_key(s_rc, pk, rot, c);
     }
 }
 
@@ -584,13 +618,13 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         for (uint32_t e = 0; e < 2; ++e) {
             const uint32_t i = e ? i1 : i0;
             if (i >= nq) continue;
-            uint32_t x, y, z, lq;
+            uint32_t x, y, z, lq, rot;
             morton_decode(sk[i], md, x, y, z, lq);
-            const uint32_t len = 1u << (md - lq - 1), msk = cm[i];
+            const Key pk = dev_encode_rot(s_rc, x, y, z, lq, md, rot);
+            const uint32_t msk = cm[i];
             for (uint32_t c = 0; c < 8; ++c) {
                 if (msk & (1u << c)) continue;
-                leaves[lpos++] = dev_encode(s_rc, x + (c & 1) * len, y + ((c >> 1) & 1) * len,
-                                            z + ((c >> 2) & 1) * len, lq + 1, md);
+                leaves[lpos++] = child_key(s_rc, pk, rot, c);
             }
         }
         used += nlv;
@@ -809,7 +843,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     // level histogram of the inputs (also gives the level range)
     DevBuf<uint32_t> lh(32, s), cursor(32, s);
     HIP_OK(hipMemsetAsync(lh.p, 0, 128, s));
-    k_level_hist<<<grid_for(n, B), B, 0, s>>>(in, n, lh.p);
+    k_level_hist<<<grid_for(n, B * LH_ITEMS), B, 0, s>>>(in, n, lh.p);
     uint32_t hl[32];
     HIP_OK(hipMemcpyAsync(hl, lh.p, 128, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -827,7 +861,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     for (int l = 0; l < 32; ++l) loff[l + 1] = loff[l] + hl[l];
     HIP_OK(hipMemcpyAsync(cursor.p, loff, 128, hipMemcpyHostToDevice, s));
     DevBuf<Key> inpar(n, s);
-    k_bucket_parents<<<grid_for(n, B), B, 0, s>>>(in, n, md, tab, cursor.p, inpar.p);
+    k_bucket_parents<<<grid_for(n, B * LH_ITEMS), B, 0, s>>>(in, n, md, tab, cursor.p, inpar.p);
     HIP_OK(hipGetLastError());
     const uint32_t maxL = lmax;
     // Per level: sort + unique the seeds W (parents at level l-1), emit the
