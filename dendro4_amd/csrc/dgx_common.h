@@ -197,6 +197,30 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
     return base + ex;
 }
 
+// Reserve n slots on *counter for this thread with one atomic per workgroup;
+// returns the first slot.  Every thread of the workgroup must call it; lds
+// holds blockDim.x / 64 + 1 entries.
+__device__ inline uint32_t block_reserve(uint32_t *counter, uint32_t n, uint32_t *lds) {
+    uint32_t wtot;
+    const uint32_t ex = wave_excl_sum(n, wtot);
+    const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (__lane_id() == 0) lds[wid] = wtot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            const uint32_t t = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[nw] = acc ? atomicAdd(counter, acc) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = lds[nw] + lds[wid] + ex;
+    __syncthreads();  // lds may be reused by the caller
+    return r;
+}
+
 // Histogram add of one count per valid lane into h[v] (LDS or global).  The
 // few most common values of the wave are counted with one atomic each
 // (ballot match) while each still holds at least 8 lanes, the rest with

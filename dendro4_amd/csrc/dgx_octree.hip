@@ -416,39 +416,62 @@ __global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32
 // parents take at most two values, so at most 8 distinct parents per octant.
 // They go after the next level's nseed input parents, which the same launch
 // copies to out[0..nseed).
-__global__ void k_bal_nbr_parents(const Key *Q, const uint32_t *d_nq, uint32_t lvl, uint32_t md, Key *out,
-                                  uint32_t *counter, const Key *seed, size_t nseed) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nseed) out[i] = seed[i];
-    out += nseed;
-    // per axis: the octant's own parent coordinate and the neighbouring one
-    int64_t cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
-    uint32_t valid = 0;  // bit (bz*4 + by*2 + bx): candidate parent inside the domain
-    if (i < *d_nq) {
-        uint32_t x, y, z, l;
-        morton_decode(Q[i], md, x, y, z, l);
-        const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
-        cx0 = ((int64_t)x / plen) * plen;
-        cy0 = ((int64_t)y / plen) * plen;
-        cz0 = ((int64_t)z / plen) * plen;
-        cx1 = ((int64_t)x % plen) ? cx0 + plen : cx0 - plen;
-        cy1 = ((int64_t)y % plen) ? cy0 + plen : cy0 - plen;
-        cz1 = ((int64_t)z % plen) ? cz0 + plen : cz0 - plen;
-        const uint32_t okx = 1u | ((cx1 >= 0 && cx1 < dom) ? 2u : 0u);
-        const uint32_t oky = 1u | ((cy1 >= 0 && cy1 < dom) ? 2u : 0u);
-        const uint32_t okz = 1u | ((cz1 >= 0 && cz1 < dom) ? 2u : 0u);
+// Neighbour parents of the split octants Q (level lvl): for each octant, per
+// axis its own parent coordinate and the neighbouring one on the side of the
+// octant inside its parent; the up-to-8 combinations inside the domain are
+// parents at level lvl-1 of the octant's 26 neighbours.
+__device__ __forceinline__ uint32_t nbr_parent_axes(const Key &q, uint32_t md, uint32_t lvl, uint32_t c0[3],
+                                                    uint32_t c1[3]) {
+    uint32_t p[3], l;
+    morton_decode(q, md, p[0], p[1], p[2], l);
+    const uint32_t sh = md - lvl + 1;  // log2 of the parent length
+    const uint64_t plen = 1ull << sh, dom = 1ull << md;
+    uint32_t ok[3];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-            if (((okx >> (t & 1)) & 1u) && ((oky >> ((t >> 1) & 1)) & 1u) && ((okz >> ((t >> 2) & 1)) & 1u))
-                valid |= 1u << t;
+    for (int a = 0; a < 3; ++a) {
+        const uint64_t lo = (uint64_t)(p[a] >> sh) << sh;
+        const bool upper = (p[a] & (uint32_t)(plen - 1)) != 0;  // the octant is in the upper half of its parent
+        c0[a] = (uint32_t)lo;
+        const uint64_t nb = upper ? lo + plen : lo - plen;  // wraps below 0: rejected by the domain test
+        c1[a] = (uint32_t)nb;
+        ok[a] = 1u | ((upper ? nb < dom : lo >= plen) ? 2u : 0u);
     }
-    uint32_t base = wave_reserve(counter, (uint32_t)__popc(valid));  // every lane of the wave reaches this call
+    uint32_t valid = 0;
 #pragma unroll
     for (int t = 0; t < 8; ++t)
-        if ((valid >> t) & 1u) {
-            const int64_t px = (t & 1) ? cx1 : cx0, py = (t & 2) ? cy1 : cy0, pz = (t & 4) ? cz1 : cz0;
-            out[base++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
-        }
+        if (((ok[0] >> (t & 1)) & 1u) && ((ok[1] >> ((t >> 1) & 1)) & 1u) && ((ok[2] >> ((t >> 2) & 1)) & 1u))
+            valid |= 1u << t;
+    return valid;
+}
+
+// The next level's seeds: the input parents (copied), then the neighbour
+// parents of the split octants, NB_ITEMS octants per thread and one atomic
+// per workgroup to reserve the output.
+constexpr int NB_ITEMS = 4;
+__global__ void __launch_bounds__(B) k_bal_nbr_parents(const Key *Q, const uint32_t *d_nq, uint32_t lvl, uint32_t md,
+                                                       Key *out, uint32_t *counter, const Key *seed, size_t nseed) {
+    __shared__ uint32_t red[B / 64 + 1];
+    const size_t i0 = (size_t)blockIdx.x * B * NB_ITEMS + threadIdx.x;
+    const uint32_t nq = *d_nq;
+    uint32_t c0[NB_ITEMS][3], c1[NB_ITEMS][3], valid[NB_ITEMS];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < NB_ITEMS; ++r) {
+        const size_t i = i0 + (size_t)r * B;
+        if (i < nseed) out[i] = seed[i];
+        valid[r] = 0;
+        if (i < nq) valid[r] = nbr_parent_axes(Q[i], md, lvl, c0[r], c1[r]);
+        cnt += (uint32_t)__popc(valid[r]);
+    }
+    out += nseed;
+    uint32_t base = block_reserve(counter, cnt, red);
+#pragma unroll
+    for (int r = 0; r < NB_ITEMS; ++r)
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            if ((valid[r] >> t) & 1u)
+                out[base++] = morton_encode((t & 1) ? c1[r][0] : c0[r][0], (t & 2) ? c1[r][1] : c0[r][1],
+                                            (t & 4) ? c1[r][2] : c0[r][2], lvl - 1, md);
 }
 
 // For every split octant at the finer level, mark its child bit in the parent (Qp sorted unique).
@@ -555,12 +578,55 @@ __device__ __forceinline__ uint32_t fuse_block_scan(uint32_t v, uint32_t *tmp, u
     return r;
 }
 
+// Orders this wave's LDS accesses before and after the call (LDS operations
+// of one wave complete in order); no other wave is waited for.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ bool fuse_less(const uint64_t &a, const uint64_t &b) { return a < b; }
+__device__ __forceinline__ bool fuse_less(const Key &a, const Key &b) { return key_lt(a, b); }
+
+// Bitonic sort of a[0..P) (P a power of two <= 2 kFuseB) by the kFuseB
+// threads.  In every stage thread t performs compare-exchange t: elements
+// i = t with a zero bit inserted at log2(j), and i + j.  For partner distances
+// j <= 64 the comparisons of one wave stay inside its own 128 elements, so
+// consecutive such stages are separated by a wave-level fence only.
+template <typename T>
+__device__ void fuse_bitonic(T *a, uint32_t P) {
+    const uint32_t t = threadIdx.x, ncmp = P >> 1;
+    bool prev_local = false;
+    for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            const bool local = j <= 64;
+            if (local && prev_local)
+                wave_lds_sync();
+            else
+                __syncthreads();
+            if (t < ncmp) {
+                const uint32_t lj = __ffs(j) - 1;
+                const uint32_t i = ((t >> lj) << (lj + 1)) | (t & (j - 1)), o = i + j;
+                const bool up = (i & kk) == 0;
+                const T x = a[i], y = a[o];
+                if (up ? fuse_less(y, x) : fuse_less(x, y)) {
+                    a[i] = y;
+                    a[o] = x;
+                }
+            }
+            prev_local = local;
+        }
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *win, uint32_t nw0, const Key *prevq,
                                                      uint32_t nprev0, const Key *inpar, const uint32_t *loff,
                                                      uint32_t md, SfcPacked tab, Key *leaves, uint32_t used0,
                                                      Key *wbuf0, Key *wbuf1, Key *qbuf0, Key *qbuf1, FuseState *st) {
     LOAD_TABLES(tab)
     __shared__ Key sk[kFuseCap];
+    __shared__ uint64_t sh[kFuseCap];  // the high words, when the low word is the level alone
     __shared__ uint32_t cm[kFuseCap];
     __shared__ uint32_t tmp[kFuseB / 64 + 1];
     const uint32_t t = threadIdx.x;
@@ -571,24 +637,24 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         // seeds of level l (1 <= nw <= kFuseCap): sort, then unique in place -> Q = sk[0..nq)
         uint32_t P = 1;
         while (P < nw) P <<= 1;
-        for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? W[i] : Key{~0ull, ~0ull};
-        __syncthreads();
-        for (uint32_t kk = 2; kk <= P; kk <<= 1) {
-            for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = t; i < P; i += kFuseB) {
-                    const uint32_t o = i ^ j;
-                    if (o > i) {
-                        const bool up = (i & kk) == 0;
-                        const Key a = sk[i], b = sk[o];
-                        if (up ? key_lt(b, a) : key_lt(a, b)) {
-                            sk[i] = b;
-                            sk[o] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        // seeds are parents at level l-1: at levels <= 21 every digit is in the
+        // high word and the low word is the level, so 8-byte values are sorted
+        const uint64_t lo_lv = l - 1;
+        bool narrow = true;
+        for (uint32_t i = t; i < P; i += kFuseB) {
+            const Key w = i < nw ? W[i] : Key{~0ull, lo_lv};
+            sh[i] = w.hi;
+            narrow = narrow && w.lo == lo_lv;
         }
+        narrow = __syncthreads_and(narrow) != 0;
+        if (narrow) {
+            fuse_bitonic(sh, P);
+            for (uint32_t i = t; i < nw; i += kFuseB) sk[i] = Key{sh[i], lo_lv};
+        } else {
+            for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? W[i] : Key{~0ull, ~0ull};
+            fuse_bitonic(sk, P);
+        }
+        __syncthreads();
         const uint32_t i0 = 2 * t, i1 = 2 * t + 1;
         const Key k0 = sk[i0 < kFuseCap ? i0 : 0], k1 = sk[i1 < kFuseCap ? i1 : 0];
         const uint32_t f0 = i0 < nw && (i0 == 0 || !key_eq(k0, sk[i0 - 1])) ? 1u : 0u;
@@ -639,26 +705,11 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         const Key *src = inpar + loff[l - 1];
         Key *Wn = (k & 1) ? wbuf1 : wbuf0;
         const uint32_t lvl = l - 1;
-        int64_t cx0[2], cx1[2], cy0[2], cy1[2], cz0[2], cz1[2];
+        uint32_t pc0[2][3], pc1[2][3];
         uint32_t valid[2] = {0u, 0u};
         for (uint32_t e = 0; e < 2; ++e) {
             const uint32_t i = e ? i1 : i0;
-            if (i >= nq) continue;
-            uint32_t x, y, z, lq;
-            morton_decode(sk[i], md, x, y, z, lq);
-            const int64_t len = 1ll << (md - lvl), plen = len << 1, dom = 1ll << md;
-            cx0[e] = ((int64_t)x / plen) * plen;
-            cy0[e] = ((int64_t)y / plen) * plen;
-            cz0[e] = ((int64_t)z / plen) * plen;
-            cx1[e] = ((int64_t)x % plen) ? cx0[e] + plen : cx0[e] - plen;
-            cy1[e] = ((int64_t)y % plen) ? cy0[e] + plen : cy0[e] - plen;
-            cz1[e] = ((int64_t)z % plen) ? cz0[e] + plen : cz0[e] - plen;
-            const uint32_t okx = 1u | ((cx1[e] >= 0 && cx1[e] < dom) ? 2u : 0u);
-            const uint32_t oky = 1u | ((cy1[e] >= 0 && cy1[e] < dom) ? 2u : 0u);
-            const uint32_t okz = 1u | ((cz1[e] >= 0 && cz1[e] < dom) ? 2u : 0u);
-            for (int b = 0; b < 8; ++b)
-                if (((okx >> (b & 1)) & 1u) && ((oky >> ((b >> 1) & 1)) & 1u) && ((okz >> ((b >> 2) & 1)) & 1u))
-                    valid[e] |= 1u << b;
+            if (i < nq) valid[e] = nbr_parent_axes(sk[i], md, lvl, pc0[e], pc1[e]);
         }
         uint32_t nnb;
         uint32_t npos = nin + fuse_block_scan(__popc(valid[0]) + __popc(valid[1]), tmp, nnb);
@@ -666,11 +717,9 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
         for (uint32_t e = 0; e < 2; ++e)
             for (int b = 0; b < 8; ++b)
-                if ((valid[e] >> b) & 1u) {
-                    const int64_t px = (b & 1) ? cx1[e] : cx0[e], py = (b & 2) ? cy1[e] : cy0[e],
-                                  pz = (b & 4) ? cz1[e] : cz0[e];
-                    Wn[npos++] = morton_encode((uint32_t)px, (uint32_t)py, (uint32_t)pz, lvl - 1, md);
-                }
+                if ((valid[e] >> b) & 1u)
+                    Wn[npos++] = morton_encode((b & 1) ? pc1[e][0] : pc0[e][0], (b & 2) ? pc1[e][1] : pc0[e][1],
+                                               (b & 4) ? pc1[e][2] : pc0[e][2], lvl - 1, md);
         __syncthreads();
         PQ = Qg;
         nprev = nq;
@@ -962,8 +1011,8 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         const size_t nin_next = l >= 2 ? hl[l - 1] : 0;
         if (l >= 2) {
             Wn.alloc(nin_next + 8 * nw, s);
-            k_bal_nbr_parents<<<grid_for(std::max(nw, nin_next), B), B, 0, s>>>(Q.p, d_nq, l - 1, md, Wn.p, d_nnb,
-                                                                             inpar.p + loff[l - 1], nin_next);
+            k_bal_nbr_parents<<<grid_for(std::max(nw, nin_next), B * NB_ITEMS), B, 0, s>>>(
+                Q.p, d_nq, l - 1, md, Wn.p, d_nnb, inpar.p + loff[l - 1], nin_next);
             HIP_OK(hipGetLastError());
         }
         uint32_t hc[3];
@@ -981,7 +1030,9 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     W.release();
     prevQ.release();
     const size_t total = used;
-    radix_sort_keys(leaves.p, nullptr, total, s, 0, 128);
+    // distinct leaves are never ancestor and descendant, so a digit at or above
+    // the coarser level tells them apart: the level byte needs no pass
+    radix_sort_keys(leaves.p, nullptr, total, s, 8, 128);
     HIP_OK(hipStreamSynchronize(s));
     out = std::move(leaves);
     out.n = total;

@@ -86,25 +86,40 @@ __global__ void __launch_bounds__(1024) k_scan_partials(uint32_t *partial, size_
     if (threadIdx.x == 0 && d_total) *d_total = carry;
 }
 
+// The tile is staged through LDS (padded against bank conflicts), so global
+// loads and stores are coalesced while each thread scans 16 consecutive items.
+__device__ __forceinline__ uint32_t scan_pad(uint32_t x) { return x + (x >> 5); }
 __global__ void __launch_bounds__(SCAN_B) k_scan_down(const uint32_t *in, uint32_t *out, size_t n,
                                                      const uint32_t *partial) {
     __shared__ uint32_t lds[SCAN_B / 64 + 1];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    __shared__ uint32_t t[SCAN_TILE + SCAN_TILE / 32];
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const uint32_t e = (uint32_t)r * SCAN_B + tid;
+        t[scan_pad(e)] = base + e < n ? in[base + e] : 0u;
+    }
+    __syncthreads();
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        size_t idx = base + i;
-        v[i] = idx < n ? in[idx] : 0u;
+        v[i] = t[scan_pad(tid * SCAN_ITEMS + i)];
         s += v[i];
     }
     uint32_t tot;
     uint32_t ex = block_exclusive_scan(s, lds, tot) + partial[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        size_t idx = base + i;
-        if (idx < n) out[idx] = ex;
+        t[scan_pad(tid * SCAN_ITEMS + i)] = ex;
         ex += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        const uint32_t e = (uint32_t)r * SCAN_B + tid;
+        if (base + e < n) out[base + e] = t[scan_pad(e)];
     }
 }
 
@@ -222,26 +237,41 @@ struct SortPlan {
     uint32_t gofs[16][256];
 };
 
-// One workgroup: per candidate pass j (digit position pos[j] = (packed >> 4j) & 15),
-// a digit position is skipped when one digit value holds all n keys.
-__global__ void __launch_bounds__(256) k_sort_plan(const uint32_t *ghist, uint32_t n, uint64_t packed, int npass,
-                                                   SortPlan *plan) {
-    __shared__ uint32_t scan_tmp[256 / 64 + 1];
-    const int tid = threadIdx.x;
-    uint32_t src = 0;
-    for (int j = 0; j < npass; ++j) {
-        const uint32_t p = (uint32_t)(packed >> (4 * j)) & 15u;
-        const uint32_t c = ghist[p * 256 + tid];
-        const bool trivial = __syncthreads_or(c == n) != 0;
-        uint32_t total;
-        plan->gofs[j][tid] = block_exclusive_scan(c, scan_tmp, total);
-        if (tid == 0) {
-            plan->active[j] = trivial ? 0u : 1u;
-            plan->src[j] = src;
+// One workgroup, one wave per candidate pass j (digit position
+// pos[j] = (packed >> 4j) & 15): a digit position is skipped when one digit
+// value holds all n keys; then thread 0 assigns the buffers in pass order.
+__global__ void __launch_bounds__(1024) k_sort_plan(const uint32_t *ghist, uint32_t n, uint64_t packed, int npass,
+                                                    SortPlan *plan) {
+    __shared__ uint32_t act[16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (w < npass) {
+        const uint32_t p = (uint32_t)(packed >> (4 * w)) & 15u;
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            c[q] = ghist[p * 256 + lane * 4 + q];
+            sum += c[q];
         }
-        if (!trivial) src ^= 1u;
+        const bool trivial = __ballot(c[0] == n || c[1] == n || c[2] == n || c[3] == n) != 0;
+        uint32_t total;
+        uint32_t ex = wave_excl_sum(sum, total);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            plan->gofs[w][lane * 4 + q] = ex;
+            ex += c[q];
+        }
+        if (lane == 0) act[w] = trivial ? 0u : 1u;
     }
-    if (tid == 0) plan->src[npass] = src;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t src = 0;
+        for (int j = 0; j < npass; ++j) {
+            plan->active[j] = act[j];
+            plan->src[j] = src;
+            src ^= act[j];
+        }
+        plan->src[npass] = src;
+    }
 }
 
 // One pass of the one-sweep LSD sort over the 8-bit digit at `bit`, on tiles
@@ -515,7 +545,7 @@ void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &pas
     }
     const unsigned gb = (unsigned)std::min<size_t>(grid_for(n, SB * GH_U), 1024);
     k_global_hist<<<gb, SB, 0, s>>>(keys, n, ghist, pmask);
-    k_sort_plan<<<1, 256, 0, s>>>(ghist, (uint32_t)n, packed, np, plan);
+    k_sort_plan<<<1, 1024, 0, s>>>(ghist, (uint32_t)n, packed, np, plan);
     HIP_OK(hipGetLastError());
     DevBuf<Key> ktmp(n, s);
     DevBuf<uint32_t> vtmp(vals ? n : 0, s);
@@ -537,7 +567,7 @@ void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &pas
                                                         status, ctr, fault);
         HIP_OK(hipGetLastError());
     }
-    k_sort_finish<<<(unsigned)std::min<size_t>(grid_for(n, 256), 2048), 256, 0, s>>>(keys, ktmp.p, vals, vtmp.p, n,
+    k_sort_finish<<<(unsigned)std::min<size_t>(grid_for(n, 256), 512), 256, 0, s>>>(keys, ktmp.p, vals, vtmp.p, n,
                                                                                      np, plan);
     HIP_OK(hipGetLastError());
 }
