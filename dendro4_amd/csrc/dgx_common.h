@@ -344,9 +344,6 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // the sort is stream-ordered.  At most 2048 keys without payload are sorted in
 // one workgroup.
 void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
-// The same sort over the byte digits in which some key differs from the
-// others: vary_hi / vary_lo = OR over all keys of (key XOR any one key).
-void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo);
 // Throws DgxError if a sort kernel reported a look-back wait that exceeded its
 // bound since the last check (the flag is cleared).
 void sort_fault_check();

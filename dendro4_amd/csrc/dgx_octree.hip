@@ -50,73 +50,26 @@ namespace {
 
 constexpr int B = 256;
 
-// Level-maxDepth keys of the points.  Also ORs into vary[0] (hi) / vary[1]
-// (lo) every bit in which some key differs from point 0's key, so the sort
-// needs no separate pass to find its constant digits.  Each block encodes
-// point 0's key once (thread 0), not once per thread.
+// Level-maxDepth keys of the points; *bad is set when a point lies outside
+// the domain.
 __global__ void __launch_bounds__(B) k_quantise(const double *xyz, size_t n, uint32_t md, double gx, double gy,
-                                                double gz, SfcPacked tab, Key *keys, uint32_t *bad,
-                                                unsigned long long *vary) {
+                                                double gz, SfcPacked tab, Key *keys, uint32_t *bad) {
     LOAD_TABLES(tab)
     const size_t i = (size_t)blockIdx.x * B + threadIdx.x;
+    if (i >= n) return;
     const double scale = (double)(1u << md);
     // (unsigned)(x * 2^maxDepth / gLen), construct.cpp:933-941
-    auto key_of = [&](size_t j, bool &inside) {
-        const double fx = xyz[3 * j] * scale / gx, fy = xyz[3 * j + 1] * scale / gy, fz = xyz[3 * j + 2] * scale / gz;
-        uint32_t px = 0, py = 0, pz = 0;
-        inside = fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale;
-        if (inside) {
-            px = (uint32_t)fx;
-            py = (uint32_t)fy;
-            pz = (uint32_t)fz;
-        }
-        return dev_encode(s_rc, px, py, pz, md, md);
-    };
-    // every varying bit is a bit in which some key differs from the block's
-    // first key, or in which that key differs from point 0's key
-    __shared__ Key first;
-    Key k{0ull, 0ull};
-    if (i < n) {
-        bool inside;
-        k = key_of(i, inside);
-        if (!inside) atomicOr(bad, 1u);
-        keys[i] = k;
+    const double fx = xyz[3 * i] * scale / gx, fy = xyz[3 * i + 1] * scale / gy, fz = xyz[3 * i + 2] * scale / gz;
+    uint32_t px = 0, py = 0, pz = 0;
+    const bool inside = fx >= 0.0 && fx < scale && fy >= 0.0 && fy < scale && fz >= 0.0 && fz < scale;
+    if (inside) {
+        px = (uint32_t)fx;
+        py = (uint32_t)fy;
+        pz = (uint32_t)fz;
+    } else {
+        atomicOr(bad, 1u);
     }
-    if (threadIdx.x == 0) first = k;
-    __syncthreads();
-    unsigned long long dhi = 0, dlo = 0;
-    if (i < n) {
-        dhi = k.hi ^ first.hi;
-        dlo = k.lo ^ first.lo;
-    }
-    if (threadIdx.x == 0 && i < n) {
-        bool inside0;
-        const Key r = key_of(0, inside0);
-        dhi |= k.hi ^ r.hi;
-        dlo |= k.lo ^ r.lo;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        dhi |= __shfl_xor(dhi, o, 64);
-        dlo |= __shfl_xor(dlo, o, 64);
-    }
-    __shared__ unsigned long long wv[B / 64][2];
-    if ((threadIdx.x & 63) == 0) {
-        wv[threadIdx.x >> 6][0] = dhi;
-        wv[threadIdx.x >> 6][1] = dlo;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < B / 64; ++w) {
-            dhi |= wv[w][0];
-            dlo |= wv[w][1];
-        }
-        // one atomic per workgroup, and only for bits not yet recorded: the
-        // mask fills after a few workgroups and the rest only read it
-        const unsigned long long ghi = __atomic_load_n(&vary[0], __ATOMIC_RELAXED);
-        const unsigned long long glo = __atomic_load_n(&vary[1], __ATOMIC_RELAXED);
-        if (dhi & ~ghi) atomicOr(&vary[0], dhi);
-        if (dlo & ~glo) atomicOr(&vary[1], dlo);
-    }
+    keys[i] = dev_encode(s_rc, px, py, pz, md, md);
 }
 
 __global__ void k_p2o_lev(const Key *pk, size_t n, uint32_t m, uint32_t md, uint8_t *lev) {
@@ -169,9 +122,17 @@ __device__ inline uint32_t gap_after(const Key &A) {
 // q in [0, Q]: count of octants emitted by slot q (the gap before leaf q plus the leaf; slot Q: the tail gap).
 // Consecutive non-empty leaves must be disjoint and increasing; otherwise the
 // slot counts 0 and err is raised (the host then refuses the result).
-__global__ void k_gap_count(const Key *leafkey, const uint32_t *NL, size_t Q, uint32_t *cnt, uint32_t *err) {
+// cnt[q] for q <= cap: octants emitted for leaf q (0 beyond the device count
+// *dQ of distinct leaves, so the scan may run over the capacity).
+__global__ void k_gap_count(const Key *leafkey, const uint32_t *NL, const uint32_t *dQ, size_t cap, uint32_t *cnt,
+                            uint32_t *err) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q > Q) return;
+    if (q > cap) return;
+    const size_t Q = *dQ;
+    if (q > Q) {
+        cnt[q] = 0;
+        return;
+    }
     if (q == Q) {
         cnt[q] = gap_after(leafkey[NL[Q - 1]]);
     } else {
@@ -197,9 +158,10 @@ __device__ inline Key with_digit(const Key &base, uint32_t t, uint32_t r) {
     return k;
 }
 
-__global__ void k_gap_emit(const Key *leafkey, const uint32_t *NL, size_t Q, size_t n, const uint32_t *off,
+__global__ void k_gap_emit(const Key *leafkey, const uint32_t *NL, const uint32_t *dQ, size_t n, const uint32_t *off,
                            Key *out, uint32_t *ptb, uint32_t *ptc) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t Q = *dQ;
     if (q > Q) return;
     uint32_t o = off[q];
     if (q == Q) {
@@ -773,18 +735,16 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
         return root;
     }
     DevBuf<Key> pk(n, s);
-    DevBuf<uint32_t> bad(1, s);
-    DevBuf<unsigned long long> vary(3, s);  // vary[0..1], then the out-of-domain flag in the low word of [2]
-    HIP_OK(hipMemsetAsync(vary.p, 0, 3 * sizeof(unsigned long long), s));
+    // device counters: [0] distinct leaf keys Q, [1] textbook leaves nT,
+    // [2] a point outside the domain, [3] overlapping leaves (internal error)
+    DevBuf<uint32_t> ctr(4, s);
+    HIP_OK(hipMemsetAsync(ctr.p, 0, 4 * sizeof(uint32_t), s));
     const double gx = glens ? glens[0] : 1.0, gy = glens ? glens[1] : 1.0, gz = glens ? glens[2] : 1.0;
-    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p,
-                                           reinterpret_cast<uint32_t *>(vary.p + 2), vary.p);
+    k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p, ctr.p + 2);
     HIP_OK(hipGetLastError());
-    unsigned long long hv[3];
-    HIP_OK(hipMemcpyAsync(hv, vary.p, sizeof hv, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    radix_sort_keys_mask(pk.p, nullptr, n, s, hv[0], hv[1]);
-    uint32_t hbad = (uint32_t)hv[2];
+    // every key is at level md and bits 5-32 of the low word are zero by the
+    // key layout: bits 32-127 decide the order (constant bytes are skipped)
+    radix_sort_keys(pk.p, nullptr, n, s, 32, 128);
     // textbook leaves
     DevBuf<uint8_t> lev(n - m, s);
     k_p2o_lev<<<grid_for(n - m, B), B, 0, s>>>(pk.p, n, m, md, lev.p);
@@ -793,29 +753,32 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
     lev.release();
     DevBuf<uint32_t> head(n, s), hpos(n, s);
     k_head<<<grid_for(n, B), B, 0, s>>>(leafkey.p, n, head.p);
-    uint64_t Q = scan_exclusive_u32(head.p, hpos.p, n, s);
-    if (hbad) {
-        dgx_set_error("points2Octree: coordinates must lie in [0,1)");
-        throw DgxError();
-    }
-    DevBuf<uint32_t> NL(Q, s);
+    scan_exclusive_u32_async(head.p, hpos.p, n, ctr.p, s);
+    // the distinct leaves stay counted on the device: the arrays below are
+    // sized by the capacity n and the kernels read Q
+    DevBuf<uint32_t> NL(n, s);
     k_compact_idx<<<grid_for(n, B), B, 0, s>>>(head.p, hpos.p, n, NL.p);
     head.release();
     hpos.release();
-    DevBuf<uint32_t> cnt(Q + 1, s), off(Q + 1, s);
-    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
-    k_gap_count<<<grid_for(Q + 1, B), B, 0, s>>>(leafkey.p, NL.p, Q, cnt.p, bad.p);
-    uint64_t nT = scan_exclusive_u32(cnt.p, off.p, Q + 1, s);
-    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+    DevBuf<uint32_t> cnt(n + 1, s), off(n + 1, s);
+    k_gap_count<<<grid_for(n + 1, B), B, 0, s>>>(leafkey.p, NL.p, ctr.p, n, cnt.p, ctr.p + 3);
+    scan_exclusive_u32_async(cnt.p, off.p, n + 1, ctr.p + 1, s);
+    uint32_t hc[4];
+    HIP_OK(hipMemcpyAsync(hc, ctr.p, sizeof hc, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (hbad) {
+    if (hc[2]) {
+        dgx_set_error("points2Octree: coordinates must lie in [0,1)");
+        throw DgxError();
+    }
+    if (hc[3]) {
         dgx_set_error("points2Octree: internal error, non-empty leaves overlap");
         throw DgxError();
     }
+    const uint64_t nT = hc[1];
     cnt.release();
     DevBuf<Key> T(nT, s);
     DevBuf<uint32_t> tb(nT, s), tc(nT, s);
-    k_gap_emit<<<grid_for(Q + 1, B), B, 0, s>>>(leafkey.p, NL.p, Q, n, off.p, T.p, tb.p, tc.p);
+    k_gap_emit<<<grid_for(n + 1, B), B, 0, s>>>(leafkey.p, NL.p, ctr.p, n, off.p, T.p, tb.p, tc.p);
     HIP_OK(hipGetLastError());
     leafkey.release();
     NL.release();

@@ -494,21 +494,6 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
     run_passes(keys, vals, n, passes, s);
 }
 
-void radix_sort_keys_mask(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t vary_hi, uint64_t vary_lo) {
-    if (n < 2) return;
-    check_sort_size(n);
-    if (!vals && n <= (size_t)SMALL_N) {
-        k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
-        HIP_OK(hipGetLastError());
-        return;
-    }
-    std::vector<int> passes;
-    for (int p = 0; p < 16; ++p) {
-        const uint64_t w = p < 8 ? vary_lo : vary_hi;
-        if ((w >> (8 * (p & 7))) & 255u) passes.push_back(8 * p);
-    }
-    run_passes(keys, vals, n, passes, s);
-}
 
 namespace {
 
