@@ -344,6 +344,9 @@ inline unsigned grid_for(size_t n, unsigned block) { return (unsigned)((n + bloc
 // the sort is stream-ordered.  At most 2048 keys without payload are sorted in
 // one workgroup.
 void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int begin_bit = 0, int end_bit = 128);
+// The same sort for keys that pack two bounded integers: lo <= lo_max and
+// hi <= hi_max; only the bytes those bounds can reach get a pass.
+void radix_sort_pairs(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t lo_max, uint64_t hi_max);
 // Throws DgxError if a sort kernel reported a look-back wait that exceeded its
 // bound since the last check (the flag is cleared).
 void sort_fault_check();

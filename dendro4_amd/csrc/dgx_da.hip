@@ -662,7 +662,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
     const uint32_t nblk = (uint32_t)((E + eb - 1) / eb);
     DevBuf<Key> pk(T, s);
     k_blk_pairs<<<grid_for(T, B), B, 0, s>>>(da.conn.p, E, eb, pk.p);
-    radix_sort_keys(pk.p, nullptr, T, s, 0, 128);
+    radix_sort_pairs(pk.p, nullptr, T, s, da.n_nodes, nblk);  // (block, node)
     DevBuf<uint32_t> head(T, s), pos(T, s), bstart(nblk, s);
     k_blk_head<<<grid_for(T, B), B, 0, s>>>(pk.p, T, head.p, bstart.p);
     uint64_t np = scan_exclusive_u32(head.p, pos.p, T, s);
@@ -697,7 +697,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
         DevBuf<Key> sk(ns, s);
         DevBuf<uint32_t> sv(ns, s), shead(ns, s), spos(ns, s);
         k_blk_shared_keys<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, pair_block.p, flag.p, fpos.p, np, sk.p, sv.p);
-        radix_sort_keys(sk.p, sv.p, ns, s, 0, 128);
+        radix_sort_pairs(sk.p, sv.p, ns, s, nblk, da.n_nodes);  // (node, block)
         k_blk_shared_slots<<<grid_for(ns, B), B, 0, s>>>(sk.p, sv.p, ns, da.bdst.p, da.bnode.p, shead.p);
         uint64_t nsn = scan_exclusive_u32(shead.p, spos.p, ns, s);
         da.snode.alloc(nsn, s);

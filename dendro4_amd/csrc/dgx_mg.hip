@@ -358,7 +358,7 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
         throw DgxError();
     }
     t.nnz = hcnt;
-    radix_sort_keys(keys.p, vals.p, hcnt, s, 0, 128);
+    radix_sort_pairs(keys.p, vals.p, hcnt, s, nf, t.nc);  // (coarse node, fine node)
     t.rcol.alloc(hcnt, s);
     t.rval.alloc(hcnt, s);
     t.rval32.alloc(hcnt, s);
@@ -453,6 +453,8 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     L0.da = da_build(L0.keys.p, n, md, curve, s);
     // the coarsest level must fit the dense FP64 coarse solve; its Cholesky and
     // inverse cost n^3, so coarsening goes on until it has at most 2048 nodes
+    // (a lower limit adds levels whose launches cost every V-cycle more than
+    // the smaller factorisation saves once)
     const uint32_t kMaxDense = 2048;
     for (int k = 1; k < 30; ++k) {
         // nlevels is honoured, but the hierarchy keeps coarsening while the

@@ -495,6 +495,20 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
 }
 
 
+void radix_sort_pairs(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t lo_max, uint64_t hi_max) {
+    if (n < 2) return;
+    check_sort_size(n);
+    if (!vals && n <= (size_t)SMALL_N) {
+        k_sort_small<<<1, 1024, 0, s>>>(keys, (uint32_t)n);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    std::vector<int> passes;
+    for (int p = 0; p < 8 && (lo_max >> (8 * p)); ++p) passes.push_back(8 * p);
+    for (int p = 0; p < 8 && (hi_max >> (8 * p)); ++p) passes.push_back(64 + 8 * p);
+    run_passes(keys, vals, n, passes, s);
+}
+
 namespace {
 
 // Threads per tile: larger tiles mean fewer look-back steps and fewer
