@@ -539,34 +539,42 @@ __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
 // in (ties rotate with the node index), colour c holding
 // exactly n/16 + (c < n mod 16) nodes, so that slot = 16 x (rank inside its
 // colour) + colour is again a permutation of 0..n-1.  The records and the
-// node list are then renumbered.  Deterministic; one wave per block.  Blocks
-// past this kernel's LDS tables (more than kColMaxN nodes or kColMaxE
-// elements) keep the sorted numbering.
-constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024;
-__global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_t *boff, uint32_t *bnode, size_t ne,
-                                                  uint32_t eb) {
+// node list are then renumbered.  Deterministic (the use counts are updated
+// with atomics); one workgroup per block.  Blocks past this kernel's LDS tables (more than kColMaxN nodes or
+// kColMaxE elements) keep the sorted numbering.
+// Two waves per block colour 8 nodes per step.  Measured on the lognormal-4M
+// DA (matvec back to back / multigrid set-up): one wave, 4 nodes per step,
+// 29.5 us / 30.0 ms; two waves 29.7 us / 29.0 ms; four waves 30.0 us / 28.3 ms.
+constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024, kColB = 128;
+__global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint32_t *boff, uint32_t *bnode,
+                                                     size_t ne, uint32_t eb) {
     __shared__ uint32_t deg[kColMaxN];             // degree, then fill position
     __shared__ uint16_t ptr[kColMaxN + 1];         // adjacency offsets
     __shared__ uint16_t adj[8 * kColMaxE];         // lane groups of each node's uses
-    __shared__ uint8_t cnt[(kColMaxE / 16) * 8 * 16];  // [lane group][colour] uses so far
+    // [lane group][colour] uses so far, one byte each (at most 128 uses per
+    // lane group), four to a word for the atomic updates
+    __shared__ uint32_t cnt32[(kColMaxE / 16) * 8 * 16 / 4];
+    uint8_t *cnt = reinterpret_cast<uint8_t *>(cnt32);
     __shared__ uint16_t newidx[kColMaxN];
     __shared__ uint32_t tmp[kColMaxN];
     __shared__ uint32_t used[16];
-    __shared__ uint32_t pick[4];
-    const uint32_t b = blockIdx.x, lane = threadIdx.x;
+    constexpr uint32_t kColG = kColB / 16;
+    __shared__ uint32_t pick[kColG];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = boff[b], n = boff[b + 1] - s0;
     const size_t e0 = (size_t)b * eb;
-    if (n > kColMaxN || eb > kColMaxE || e0 >= ne) return;  // uniform over the wave
+    if (n > kColMaxN || eb > kColMaxE || e0 >= ne) return;  // uniform over the workgroup
     const uint32_t nbe = (uint32_t)min((size_t)eb, ne - e0), nuse = 8 * nbe;
     const uint32_t ngroups = ((nbe + 15) / 16) * 8;
     uint16_t *rec = lconn + e0 * 8;
-    for (uint32_t i = lane; i < n; i += 64) deg[i] = 0;
-    for (uint32_t i = lane; i < ngroups * 16; i += 64) cnt[i] = 0;
-    if (lane < 16) used[lane] = 0;
+    for (uint32_t i = tid; i < n; i += kColB) deg[i] = 0;
+    for (uint32_t i = tid; i < ngroups * 4; i += kColB) cnt32[i] = 0;
+    if (tid < 16) used[tid] = 0;
     __syncthreads();
-    for (uint32_t t = lane; t < nuse; t += 64) atomicAdd(&deg[rec[t] >> 3], 1u);
+    for (uint32_t t = tid; t < nuse; t += kColB) atomicAdd(&deg[rec[t] >> 3], 1u);
     __syncthreads();
-    {  // exclusive scan of the degrees, one chunk per lane
+    if (tid < 64) {  // exclusive scan of the degrees, one chunk per lane of wave 0
+        const uint32_t lane = tid;
         const uint32_t ch = (n + 63) / 64, i0 = min(n, lane * ch), i1 = min(n, i0 + ch);
         uint32_t sum = 0;
         for (uint32_t i = i0; i < i1; ++i) sum += deg[i];
@@ -585,20 +593,20 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
         if (lane == 63) ptr[n] = (uint16_t)incl;
     }
     __syncthreads();
-    for (uint32_t t = lane; t < nuse; t += 64) {
+    for (uint32_t t = tid; t < nuse; t += kColB) {
         const uint32_t q = atomicAdd(&deg[rec[t] >> 3], 1u);
         adj[q] = (uint16_t)((t >> 7) * 8 + (t & 7));  // element slot t/8, lane group (t/8)/16, position t%8
     }
     __syncthreads();
     const uint32_t base = n >> 4, extra = n & 15u;
-    // Four nodes per step, one per 16-lane group, taken from the four quarters
-    // of the list (far apart, so they rarely share lane groups); lane j of a
-    // group scores candidate colour (j + l) mod 16.  Lane 0 then commits the
-    // four choices in order, moving a choice whose colour has filled up in
-    // this step to the next colour with room.
-    const uint32_t g = lane >> 4, j = lane & 15u, quarter = (n + 3) / 4;
-    for (uint32_t t = 0; t < quarter; ++t) {
-        const uint32_t l = g * quarter + t;
+    // kColG nodes per step, one per 16-lane group, taken from the kColG equal
+    // parts of the list (far apart, so they rarely share lane groups); lane j
+    // of a group scores candidate colour (j + l) mod 16.  Thread 0 then
+    // commits the choices in order, moving a choice whose colour has filled up
+    // in this step to the next colour with room.
+    const uint32_t g = tid >> 4, j = tid & 15u, part = (n + kColG - 1) / kColG;
+    for (uint32_t t = 0; t < part; ++t) {
+        const uint32_t l = g * part + t;
         const bool has = l < n;
         const uint32_t a = has ? ptr[l] : 0u, z = has ? ptr[l + 1] : 0u, cj = (j + l) & 15u;
         uint32_t pen = 0;
@@ -614,9 +622,9 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
         for (int o = 8; o > 0; o >>= 1) key = min(key, (uint32_t)__shfl_xor(key, o, 64));
         if (j == 0) pick[g] = key == 0xffffffffu ? 0xffffffffu : ((key & 15u) + l) & 15u;
         __syncthreads();
-        if (lane == 0) {
-            for (uint32_t h = 0; h < 4; ++h) {
-                const uint32_t lh = h * quarter + t;
+        if (tid == 0) {
+            for (uint32_t h = 0; h < kColG; ++h) {
+                const uint32_t lh = h * part + t;
                 if (lh >= n) continue;
                 uint32_t c = pick[h];
                 if (c == 0xffffffffu || used[c] >= base + (c < extra ? 1u : 0u)) {
@@ -635,13 +643,16 @@ __global__ void __launch_bounds__(64) k_blk_color(uint16_t *lconn, const uint32_
         }
         __syncthreads();
         if (has)
-            for (uint32_t q = a + j; q < z; q += 16) cnt[adj[q] * 16u + pick[g]] += 1;
+            for (uint32_t q = a + j; q < z; q += 16) {
+                const uint32_t idx = adj[q] * 16u + pick[g];
+                atomicAdd(&cnt32[idx >> 2], 1u << (8 * (idx & 3u)));
+            }
         __syncthreads();
     }
-    for (uint32_t t = lane; t < nuse; t += 64) rec[t] = (uint16_t)(newidx[rec[t] >> 3] << 3);
-    for (uint32_t i = lane; i < n; i += 64) tmp[i] = bnode[s0 + i];
+    for (uint32_t t = tid; t < nuse; t += kColB) rec[t] = (uint16_t)(newidx[rec[t] >> 3] << 3);
+    for (uint32_t i = tid; i < n; i += kColB) tmp[i] = bnode[s0 + i];
     __syncthreads();
-    for (uint32_t i = lane; i < n; i += 64) bnode[s0 + newidx[i]] = tmp[i];
+    for (uint32_t i = tid; i < n; i += kColB) bnode[s0 + newidx[i]] = tmp[i];
 }
 __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -684,7 +695,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
                                                 da.lconn.p);
         da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
         k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
-        k_blk_color<<<nblk, 64, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
+        k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
         HIP_OK(hipGetLastError());
     }
     // shared nodes and their node-major partial slots
