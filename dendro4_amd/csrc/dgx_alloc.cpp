@@ -165,6 +165,8 @@ hipStream_t dgx_alloc_free_stream(hipStream_t s) {
 }
 hipStream_t dgx_alloc_free_stream_get() { return t_free_override; }
 
+uint64_t dgx_alloc_get_tag() { return t_tag; }
+
 uint64_t dgx_alloc_set_tag(uint64_t tag) {
     uint64_t old = t_tag;
     t_tag = tag;

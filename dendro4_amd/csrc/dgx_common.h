@@ -295,6 +295,7 @@ hipStream_t dgx_alloc_free_stream_get();
 // under it (live and peak bytes), whichever thread frees them; tag 0 = none.
 // The counters of a tag disappear when its live bytes return to zero.
 uint64_t dgx_alloc_set_tag(uint64_t tag);  // returns the previous tag
+uint64_t dgx_alloc_get_tag();              // the calling thread's tag
 int64_t dgx_alloc_tag_live(uint64_t tag);
 int64_t dgx_alloc_tag_peak(uint64_t tag);
 

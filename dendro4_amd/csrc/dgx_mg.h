@@ -51,8 +51,12 @@ void mg_mark_idle(MgHierarchy &mg);
 // Changes the operator factors: recomputes the level diagonals and the
 // coarse inverse, drops captured graphs.
 void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s);
+// With overlap, each level's DA, its diagonal and the coarse inverse are built
+// on a second host thread and stream while the next levels are coarsened
+// (faster when nothing else drives the device; a caller that already runs
+// several set-ups at once passes false).
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
-                                      double lap_fac, double mass_fac, hipStream_t s);
+                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap = true);
 
 struct SolveOpts {
     int fp32_vcycle = 1;

@@ -23,6 +23,14 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include "../../include/dendro_mi355x.h"  // dgx_last_error
+
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <string>
+#include <thread>
+
 #include <chrono>
 #include <mutex>
 #include <cmath>
@@ -436,21 +444,163 @@ static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     }
 }
 
+namespace {
+
+// The stream on which a calling thread's set-up helper works: created once per
+// thread at default priority and kept for the process, because device buffers
+// built on it are freed on it when the hierarchy goes.
+hipStream_t helper_stream() {
+    thread_local hipStream_t h = nullptr;
+    if (!h) HIP_OK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+    return h;
+}
+
+// Runs set-up tasks in order on a second host thread and stream, so that the
+// DA of one level is built while the next level is coarsened and balanced.
+// Device memory is charged to the caller's allocation tag.  A failed task
+// keeps its message; the tasks after it are skipped, and wait() rethrows it on
+// the calling thread.  The destructor joins the thread on every exit path.
+// Without a thread (inline), submit runs the task at once on the caller's
+// stream and synchronises it.
+class SetupHelper {
+  public:
+    SetupHelper(hipStream_t hs, hipStream_t caller, bool threaded)
+        : hs_(threaded ? hs : caller), tag_(dgx_alloc_get_tag()), dev_(current_device()), inline_(!threaded) {
+        if (threaded) th_ = std::thread([this] { run(); });
+    }
+    ~SetupHelper() {
+        if (inline_) return;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    // returns the task's index
+    size_t submit(std::function<void(hipStream_t)> f) {
+        if (inline_) {
+            f(hs_);
+            HIP_OK(hipStreamSynchronize(hs_));
+            ++done_;
+            return queued_++;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        todo_.push_back(std::move(f));
+        const size_t k = queued_++;
+        cv_.notify_all();
+        return k;
+    }
+    // waits until tasks 0..k have run; rethrows a failure
+    void wait(size_t k) {
+        if (inline_) return;  // ran in submit; a failure was thrown there
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return done_ > k; });
+        if (failed_) {
+            dgx_set_error("%s", err_.c_str());
+            throw DgxError();
+        }
+    }
+    void wait_all() {
+        size_t q;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q = queued_;
+        }
+        if (q) wait(q - 1);
+    }
+
+  private:
+    static int current_device() {
+        int d = 0;
+        HIP_OK(hipGetDevice(&d));
+        return d;
+    }
+    void run() {
+        (void)hipSetDevice(dev_);  // the calling thread's device (the current device is per thread)
+        dgx_alloc_set_tag(tag_);
+        for (;;) {
+            std::function<void(hipStream_t)> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !todo_.empty(); });
+                if (todo_.empty()) break;
+                f = std::move(todo_.front());
+                todo_.pop_front();
+            }
+            bool skip;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                skip = failed_;
+            }
+            std::string msg;
+            bool bad = false;
+            if (!skip) {
+                try {
+                    f(hs_);
+                    HIP_OK(hipStreamSynchronize(hs_));
+                } catch (DgxError &) {
+                    bad = true;
+                    msg = dgx_last_error();
+                } catch (std::exception &e) {
+                    bad = true;
+                    msg = e.what();
+                }
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (bad && !failed_) {
+                    failed_ = true;
+                    err_ = msg;
+                }
+                ++done_;
+            }
+            cv_.notify_all();
+        }
+        (void)hipStreamSynchronize(hs_);
+        dgx_alloc_set_tag(0);
+    }
+    hipStream_t hs_;
+    uint64_t tag_;
+    int dev_;
+    bool inline_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void(hipStream_t)>> todo_;
+    size_t queued_ = 0, done_ = 0;
+    bool stop_ = false, failed_ = false;
+    std::string err_;
+    std::thread th_;  // started in the constructor body, once the members above exist
+};
+
+}  // namespace
+
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
-                                      double lap_fac, double mass_fac, hipStream_t s) {
+                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap) {
     std::unique_ptr<MgHierarchy> mg(new MgHierarchy());
     mg->md = md;
     mg->curve = curve;
     mg->lap_fac = lap_fac;
     mg->mass_fac = mass_fac;
     if (nlevels < 1) nlevels = 1;
-    mg->lev.reserve(32);
+    mg->lev.reserve(32);  // the helper holds pointers to levels: no reallocation (at most 30 levels)
     mg->lev.emplace_back();
     MgLevel &L0 = mg->lev.back();
     L0.keys.alloc(n, s);
     HIP_OK(hipMemcpyAsync(L0.keys.p, finest, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));  // the helper reads the keys on its own stream
     L0.n_oct = n;
-    L0.da = da_build(L0.keys.p, n, md, curve, s);
+    // Each level's DA and Jacobi diagonal are built by the helper while the
+    // calling thread coarsens and balances the next level; the stopping rule
+    // needs a level's node count only once the requested levels exist.
+    SetupHelper helper(overlap ? helper_stream() : s, s, overlap);
+    auto build_da = [&, md, curve](MgLevel *L) {
+        return helper.submit([L, md, curve, lap_fac, mass_fac](hipStream_t hs) {
+            L->da = da_build(L->keys.p, L->n_oct, md, curve, hs);
+            build_level_diag(*L, lap_fac, mass_fac, hs);
+        });
+    };
+    size_t last = build_da(&L0);
     // the coarsest level must fit the dense FP64 coarse solve; its Cholesky and
     // inverse cost n^3, so coarsening goes on until it has at most 2048 nodes
     // (a lower limit adds levels whose launches cost every V-cycle more than
@@ -459,21 +609,30 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     for (int k = 1; k < 30; ++k) {
         // nlevels is honoured, but the hierarchy keeps coarsening while the
         // coarsest level is too large for the dense FP64 coarse solve
-        if (k >= nlevels && mg->lev.back().da->n_nodes <= kMaxDense) break;
+        if (k >= nlevels) {
+            helper.wait(last);
+            if (mg->lev.back().da->n_nodes <= kMaxDense) break;
+        }
         MgLevel &prev = mg->lev.back();
         DevBuf<Key> c = coarsen_build(prev.keys.p, prev.n_oct, s);
-        DevBuf<Key> b = balance_build(c.p, c.n, md, curve, s, nullptr);
+        DevBuf<Key> b = balance_build(c.p, c.n, md, curve, s, nullptr);  // synchronises s
         if (b.n >= prev.n_oct || b.n <= 8) break;  // nothing left to coarsen
         mg->lev.emplace_back();
         MgLevel &L = mg->lev.back();
         L.n_oct = b.n;
         L.keys = std::move(b);
-        L.da = da_build(L.keys.p, L.n_oct, md, curve, s);
-        if (L.n_oct < 500 && L.da->n_nodes <= kMaxDense) break;  // blockDiag.cpp:1417-1420
+        last = build_da(&L);
+        if (L.n_oct < 500) {  // blockDiag.cpp:1417-1420
+            helper.wait(last);
+            if (L.da->n_nodes <= kMaxDense) break;
+        }
     }
-    for (auto &L : mg->lev) build_level_diag(L, lap_fac, mass_fac, s);
+    helper.wait_all();
+    // the coarse inverse on the helper, the transfers here
+    helper.submit([&mg](hipStream_t hs) { build_coarse_inverse(*mg, hs); });
     for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
-    build_coarse_inverse(*mg, s);
+    helper.wait_all();
+    HIP_OK(hipStreamSynchronize(s));
     return mg;
 }
 

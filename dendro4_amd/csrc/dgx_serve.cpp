@@ -231,7 +231,9 @@ struct dgx_server_s {
             HIP_OK(hipStreamSynchronize(s));
             const double tb = now();
             d.r.s_octree = tb - ta;
-            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s);
+            // several slots already overlap their set-ups; one slot overlaps
+            // the levels of its own set-up instead
+            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s, o.slots == 1);
             mg = own.get();
             b.release();
             HIP_OK(hipStreamSynchronize(s));
