@@ -4,10 +4,11 @@
 // each costs tens of microseconds of host time per pair, so freed blocks are
 // kept here and handed out again to the next request of the same size class
 // on the same stream.  Reuse is safe in stream order: a block freed on stream
-// s is only reused by work queued later on s.  Blocks beyond the cache cap go
-// back to the HIP pool on the stream they were freed on; when an allocation
-// fails, the blocks cached for the allocating stream are returned and the
-// allocation is retried.
+// s is only reused by work queued later on s.  When the cache is at its cap,
+// blocks cached for other streams go back to the HIP pool (on their streams)
+// to make room, and a block that still does not fit goes back on the stream it
+// was freed on; when an allocation fails, the blocks cached for the
+// allocating stream are returned and the allocation is retried.
 //
 // Accounting: every allocation carries the calling thread's current tag
 // (dgx_alloc_set_tag); live and peak bytes are kept per tag, whichever thread
@@ -42,6 +43,7 @@ std::multimap<size_t, Cached> g_cache;        // class bytes -> free block
 size_t g_cached_bytes = 0;
 constexpr size_t kCacheCap = size_t(8) << 30;  // bytes held back from the pool at most
 
+std::unordered_map<hipStream_t, int> g_capturing;   // streams inside a graph capture
 thread_local hipStream_t t_free_override = nullptr;  // see dgx_alloc_free_stream
 thread_local uint64_t t_tag = 0;                     // see dgx_alloc_set_tag
 
@@ -66,6 +68,25 @@ void trim_stream_locked(hipStream_t s) {
     for (auto it = g_cache.begin(); it != g_cache.end();) {
         if (it->second.s == s) {
             (void)hipFreeAsync(it->second.p, s);
+            g_cached_bytes -= it->first;
+            it = g_cache.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// The cap is full: return blocks cached for other streams to the pool (each
+// queued on its own stream) until `need` more bytes fit.  Blocks of a stream
+// that no longer allocates (a finished set-up, a closed request slot) would
+// otherwise hold the cache, and the streams still working would go through
+// the pool for every block.
+// Streams inside a graph capture are skipped: a free queued there would join
+// the capture.
+void make_room_locked(size_t need, hipStream_t keep) {
+    for (auto it = g_cache.begin(); it != g_cache.end() && g_cached_bytes + need > kCacheCap;) {
+        if (it->second.s != keep && !g_capturing.count(it->second.s)) {
+            (void)hipFreeAsync(it->second.p, it->second.s);
             g_cached_bytes -= it->first;
             it = g_cache.erase(it);
         } else {
@@ -129,6 +150,7 @@ void dgx_free(void *p, hipStream_t s) {
         }
     }
     g_live.erase(it);
+    if (g_cached_bytes + cls > kCacheCap) make_room_locked(cls, as);
     if (g_cached_bytes + cls > kCacheCap) {
         (void)hipFreeAsync(p, as);
         return;
@@ -166,6 +188,16 @@ hipStream_t dgx_alloc_free_stream(hipStream_t s) {
 hipStream_t dgx_alloc_free_stream_get() { return t_free_override; }
 
 uint64_t dgx_alloc_get_tag() { return t_tag; }
+
+void dgx_alloc_capture_begin(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ++g_capturing[s];
+}
+void dgx_alloc_capture_end(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_capturing.find(s);
+    if (it != g_capturing.end() && --it->second <= 0) g_capturing.erase(it);
+}
 
 uint64_t dgx_alloc_set_tag(uint64_t tag) {
     uint64_t old = t_tag;

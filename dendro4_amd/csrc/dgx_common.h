@@ -296,6 +296,10 @@ hipStream_t dgx_alloc_free_stream_get();
 // The counters of a tag disappear when its live bytes return to zero.
 uint64_t dgx_alloc_set_tag(uint64_t tag);  // returns the previous tag
 uint64_t dgx_alloc_get_tag();              // the calling thread's tag
+// Brackets a graph capture on s (call begin before hipStreamBeginCapture and
+// end after hipStreamEndCapture): the allocator then queues no frees on s.
+void dgx_alloc_capture_begin(hipStream_t s);
+void dgx_alloc_capture_end(hipStream_t s);
 int64_t dgx_alloc_tag_live(uint64_t tag);
 int64_t dgx_alloc_tag_peak(uint64_t tag);
 

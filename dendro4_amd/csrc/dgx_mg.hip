@@ -837,9 +837,22 @@ void run_graph(Graph &g, const SolveOpts &o, bool use_graph, hipStream_t s, F bo
     }
     if (!g.valid || !same_opts(g.key, o)) {
         g.reset();
-        hipGraph_t graph;
+        hipGraph_t graph = nullptr;
+        // the allocator queues no frees on s while it is captured
+        dgx_alloc_capture_begin(s);
+        struct Unmark {
+            hipStream_t s;
+            ~Unmark() { dgx_alloc_capture_end(s); }
+        } unmark{s};
         HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        body();
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(s, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
         HIP_OK(hipStreamEndCapture(s, &graph));
         HIP_OK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
         HIP_OK(hipGraphDestroy(graph));
