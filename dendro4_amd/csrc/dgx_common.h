@@ -187,6 +187,29 @@ __device__ inline uint32_t wave_excl_sum(uint32_t v, uint32_t &total) {
     total = __shfl(x, 63, 64);
     return x - v;
 }
+// Exclusive scan over the workgroup (a multiple of 64 threads); lds holds
+// blockDim.x / 64 + 1 entries; every thread must call it.
+__device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t *lds, uint32_t &total) {
+    uint32_t wt;
+    const uint32_t ex = wave_excl_sum(v, wt);
+    const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (__lane_id() == 0) lds[wid] = wt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            const uint32_t t = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[nw] = acc;
+    }
+    __syncthreads();
+    const uint32_t r = lds[wid] + ex;
+    total = lds[nw];
+    __syncthreads();
+    return r;
+}
 // Reserve n slots on *counter for this lane; returns the first slot.  Every
 // lane of the wave must call it (inactive lanes pass 0).
 __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {

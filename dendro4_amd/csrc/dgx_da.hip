@@ -481,6 +481,103 @@ __global__ void k_blk_lconn(const uint32_t *conn, const uint16_t *meta, size_t n
     }
     lconn[((size_t)b * eb + epos[e]) * 8 + (j ^ c)] = (uint16_t)((lo - boff[b]) * 8u);
 }
+// One workgroup per element block of at most kBlkNodesE elements: the
+// block's 8 x nb vertex node ids are sorted in LDS (bitonic; stages with
+// partner distance <= 64 stay inside one wave's elements and synchronise that
+// wave only), made unique, and written to ulist[b * 8 * eb ...] with their
+// count in ucount[b]; every vertex's local index (lower bound in the unique
+// list) goes to lconn in the child frame, as k_blk_lconn does.
+constexpr uint32_t kBlkNodesE = 512, kBlkNodesB = 256;
+__device__ __forceinline__ void blk_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__global__ void __launch_bounds__(kBlkNodesB) k_blk_nodes(const uint32_t *conn, const uint16_t *meta, size_t ne,
+                                                          uint32_t eb, const uint32_t *epos, uint32_t *ulist,
+                                                          uint32_t *ucount, uint32_t nblk, uint16_t *lconn) {
+    constexpr uint32_t CAP = 8 * kBlkNodesE, PER_T = CAP / kBlkNodesB;  // 16 entries per thread
+    __shared__ uint32_t v[CAP];
+    __shared__ uint32_t u[CAP];
+    __shared__ uint32_t scan_tmp[kBlkNodesB / 64 + 1];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    const size_t e0 = (size_t)b * eb;
+    const uint32_t nb = (uint32_t)min((size_t)eb, ne - e0), m = 8 * nb;
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    if (b == 0 && tid == 0) ucount[nblk] = 0;  // so that the exclusive scan ends with the total
+    for (uint32_t i = tid; i < P; i += kBlkNodesB) v[i] = i < m ? conn[(size_t)(i / nb) * ne + e0 + i % nb] : 0xffffffffu;
+    // bitonic sort of v[0..P); compare-exchange c = tid + 256 q touches, for
+    // partner distances <= 64, only elements of the 128-element groups
+    // 2 (tid / 64) + 8 q and its neighbour, which no other wave touches
+    bool prev_local = false;
+    for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            const bool local = j <= 64;
+            if (local && prev_local)
+                blk_wave_sync();
+            else
+                __syncthreads();
+            const uint32_t lj = __ffs(j) - 1;
+            for (uint32_t c = tid; c < (P >> 1); c += kBlkNodesB) {
+                const uint32_t i = ((c >> lj) << (lj + 1)) | (c & (j - 1)), o = i + j;
+                const uint32_t x = v[i], y = v[o];
+                if ((i & kk) == 0 ? y < x : x < y) {
+                    v[i] = y;
+                    v[o] = x;
+                }
+            }
+            prev_local = local;
+        }
+    }
+    __syncthreads();
+    // unique: each thread flags 16 consecutive entries
+    const uint32_t i0 = tid * PER_T;
+    uint32_t f = 0;
+#pragma unroll
+    for (int q = 0; q < (int)PER_T; ++q) {
+        const uint32_t i = i0 + q;
+        f += (i < m && (i == 0 || v[i] != v[i - 1])) ? 1u : 0u;
+    }
+    uint32_t nu;
+    uint32_t w = block_excl_scan_u32(f, scan_tmp, nu);
+#pragma unroll
+    for (int q = 0; q < (int)PER_T; ++q) {
+        const uint32_t i = i0 + q;
+        if (i < m && (i == 0 || v[i] != v[i - 1])) u[w++] = v[i];
+    }
+    __syncthreads();
+    if (tid == 0) ucount[b] = nu;
+    uint32_t *ul = ulist + (size_t)b * 8 * eb;
+    for (uint32_t i = tid; i < nu; i += kBlkNodesB) ul[i] = u[i];
+    // local index of every vertex, in the child frame of its element
+    for (uint32_t i = tid; i < m; i += kBlkNodesB) {
+        const uint32_t j = i / nb, k = i % nb;
+        const size_t e = e0 + k;
+        const uint32_t g = conn[(size_t)j * ne + e];
+        uint32_t lo = 0, hi = nu;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (u[mid] < g) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t c = (meta[e] >> 8) & 7u;
+        lconn[((size_t)b * eb + epos[e]) * 8 + (j ^ c)] = (uint16_t)(lo * 8u);
+    }
+}
+// Scatters the unique node lists to their block offsets: bnode, pair_block,
+// and the number of blocks touching each node.
+__global__ void __launch_bounds__(B) k_blk_scatter_nodes(const uint32_t *ulist, uint32_t eb, const uint32_t *boff,
+                                                         uint32_t *bnode, uint32_t *pair_block, uint32_t *ncount) {
+    const uint32_t b = blockIdx.x, s0 = boff[b], n = boff[b + 1] - s0;
+    const uint32_t *ul = ulist + (size_t)b * 8 * eb;
+    for (uint32_t i = threadIdx.x; i < n; i += B) {
+        const uint32_t g = ul[i];
+        bnode[s0 + i] = g;
+        pair_block[s0 + i] = b;
+        atomicAdd(&ncount[g], 1u);
+    }
+}
 __global__ void k_blk_cmeta(const uint16_t *meta, size_t ne, uint32_t eb, const uint32_t *epos, uint16_t *cmeta) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
@@ -489,37 +586,57 @@ __global__ void k_blk_cmeta(const uint16_t *meta, size_t ne, uint32_t eb, const 
         if ((mask >> (j ^ c)) & 1u) cm |= 1u << j;
     cmeta[(e / eb) * eb + epos[e]] = (uint16_t)(cm | ((mt >> 11) << 8));
 }
-__global__ void k_blk_shared_flag(const uint32_t *bnode, const uint32_t *ncount, uint64_t np, uint32_t *flag) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Shared nodes (touched by more than one block) and their node-major partial
+// slots, one per block touching the node, blocks in increasing order.
+// sh[g]: node g is shared; cnt2[g]: its slot count (0 if not shared).
+__global__ void k_shared_count(const uint32_t *ncount, uint32_t nn, uint32_t *sh, uint32_t *cnt2) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn) return;
+    const uint32_t c = ncount[g];
+    sh[g] = c > 1 ? 1u : 0u;
+    cnt2[g] = c > 1 ? c : 0u;
+}
+// sidx / base: exclusive scans of sh / cnt2 over the nodes
+__global__ void k_shared_index(const uint32_t *ncount, const uint32_t *sidx, const uint32_t *base, uint32_t nn,
+                               uint32_t *snode, uint32_t *soff) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nn || ncount[g] < 2) return;
+    snode[sidx[g]] = g;
+    soff[sidx[g]] = base[g];
+}
+// every shared pair claims a slot of its node (in any order) ...
+__global__ void k_shared_claim(const uint32_t *bnode, const uint32_t *pair_block, const uint32_t *ncount,
+                               const uint32_t *base, uint64_t np, uint32_t *cursor, uint32_t *tblk,
+                               uint32_t *tpair) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
-    flag[p] = ncount[bnode[p]] > 1 ? 1u : 0u;
+    const uint32_t g = bnode[p];
+    if (ncount[g] < 2) return;
+    const uint32_t q = base[g] + atomicAdd(&cursor[g], 1u);
+    tblk[q] = pair_block[p];
+    tpair[q] = (uint32_t)p;
 }
-// Node-major keys (node, block) of the shared pairs, payload = pair index.
-__global__ void k_blk_shared_keys(const uint32_t *bnode, const uint32_t *pair_block, const uint32_t *flag,
-                                  const uint32_t *pos, uint64_t np, Key *sk, uint32_t *sv) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np || !flag[p]) return;
-    uint32_t q = pos[p];
-    sk[q] = Key{(uint64_t)bnode[p], (uint64_t)pair_block[p]};
-    sv[q] = (uint32_t)p;
-}
-// Partial slots are node-major: the partials of one shared node are
-// contiguous, in block order, so the second pass reads them as one run.
-__global__ void k_blk_shared_slots(const Key *sk, const uint32_t *sv, uint64_t ns, uint32_t *bdst, uint32_t *bnode,
-                                   uint32_t *shead) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ns) return;
-    uint32_t p = sv[q];
-    bdst[p] = (uint32_t)q;
-    bnode[p] |= 0x80000000u;
-    shead[q] = (q == 0 || sk[q].hi != sk[q - 1].hi) ? 1u : 0u;
-}
-__global__ void k_blk_shared_nodes(const Key *sk, const uint32_t *shead, const uint32_t *spos, uint64_t ns,
-                                   uint32_t *snode, uint32_t *soff) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ns || !shead[q]) return;
-    snode[spos[q]] = (uint32_t)sk[q].hi;
-    soff[spos[q]] = (uint32_t)q;
+// ... and each shared node orders its slots by block (few per node)
+__global__ void k_shared_order(const uint32_t *soff, uint32_t nsn, uint32_t *tblk, uint32_t *tpair, uint32_t *bdst,
+                               uint32_t *bnode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsn) return;
+    const uint32_t a = soff[i], z = soff[i + 1];
+    for (uint32_t q = a + 1; q < z; ++q) {
+        const uint32_t kb = tblk[q], kp = tpair[q];
+        uint32_t r = q;
+        while (r > a && tblk[r - 1] > kb) {
+            tblk[r] = tblk[r - 1];
+            tpair[r] = tpair[r - 1];
+            --r;
+        }
+        tblk[r] = kb;
+        tpair[r] = kp;
+    }
+    for (uint32_t q = a; q < z; ++q) {
+        bdst[tpair[q]] = q;
+        bnode[tpair[q]] |= 0x80000000u;
+    }
 }
 // Bit 30 of a block node / shared node entry marks a Dirichlet boundary node.
 __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
@@ -671,51 +788,74 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
         throw DgxError();
     }
     const uint32_t nblk = (uint32_t)((E + eb - 1) / eb);
-    DevBuf<Key> pk(T, s);
-    k_blk_pairs<<<grid_for(T, B), B, 0, s>>>(da.conn.p, E, eb, pk.p);
-    radix_sort_pairs(pk.p, nullptr, T, s, da.n_nodes, nblk);  // (block, node)
-    DevBuf<uint32_t> head(T, s), pos(T, s), bstart(nblk, s);
-    k_blk_head<<<grid_for(T, B), B, 0, s>>>(pk.p, T, head.p, bstart.p);
-    uint64_t np = scan_exclusive_u32(head.p, pos.p, T, s);
-    da.bnode.alloc(np, s);
-    DevBuf<uint32_t> pair_block(np, s), ncount(da.n_nodes, s);
+    uint64_t np;
+    DevBuf<uint32_t> pair_block, ncount(da.n_nodes, s);
     HIP_OK(hipMemsetAsync(ncount.p, 0, (size_t)da.n_nodes * 4, s));
-    k_blk_compact<<<grid_for(T, B), B, 0, s>>>(pk.p, head.p, pos.p, T, da.bnode.p, pair_block.p, ncount.p);
+    DevBuf<uint32_t> epos(E, s);
+    k_blk_order<<<nblk, B, 0, s>>>(da.meta.p, E, eb, epos.p);
+    da.lconn.alloc((size_t)nblk * 8 * eb, s);
     da.bnode_off.alloc(nblk + 1, s);
-    k_blk_offsets<<<grid_for(nblk + 1, B), B, 0, s>>>(bstart.p, pos.p, nblk, np, da.bnode_off.p);
-    pk.release();
-    head.release();
-    pos.release();
-    bstart.release();
-    {
-        DevBuf<uint32_t> epos(E, s);
-        k_blk_order<<<nblk, B, 0, s>>>(da.meta.p, E, eb, epos.p);
-        da.lconn.alloc((size_t)nblk * 8 * eb, s);
+    if (eb <= kBlkNodesE) {
+        // per block in LDS: sorted unique nodes and local indices
+        DevBuf<uint32_t> ulist((size_t)nblk * 8 * eb, s), ucount(nblk + 1, s);
+        k_blk_nodes<<<nblk, kBlkNodesB, 0, s>>>(da.conn.p, da.meta.p, E, eb, epos.p, ulist.p, ucount.p, nblk,
+                                                da.lconn.p);
+        HIP_OK(hipGetLastError());
+        np = scan_exclusive_u32(ucount.p, da.bnode_off.p, nblk + 1, s);
+        da.bnode.alloc(np, s);
+        pair_block.alloc(np, s);
+        k_blk_scatter_nodes<<<nblk, B, 0, s>>>(ulist.p, eb, da.bnode_off.p, da.bnode.p, pair_block.p, ncount.p);
+        HIP_OK(hipGetLastError());
+    } else {
+        // larger blocks: one sort of all (block, node) pairs
+        DevBuf<Key> pk(T, s);
+        k_blk_pairs<<<grid_for(T, B), B, 0, s>>>(da.conn.p, E, eb, pk.p);
+        radix_sort_pairs(pk.p, nullptr, T, s, da.n_nodes, nblk);  // (block, node)
+        DevBuf<uint32_t> head(T, s), pos(T, s), bstart(nblk, s);
+        k_blk_head<<<grid_for(T, B), B, 0, s>>>(pk.p, T, head.p, bstart.p);
+        np = scan_exclusive_u32(head.p, pos.p, T, s);
+        da.bnode.alloc(np, s);
+        pair_block.alloc(np, s);
+        k_blk_compact<<<grid_for(T, B), B, 0, s>>>(pk.p, head.p, pos.p, T, da.bnode.p, pair_block.p, ncount.p);
+        k_blk_offsets<<<grid_for(nblk + 1, B), B, 0, s>>>(bstart.p, pos.p, nblk, np, da.bnode_off.p);
         k_blk_lconn<<<grid_for(T, B), B, 0, s>>>(da.conn.p, da.meta.p, E, eb, da.bnode_off.p, da.bnode.p, epos.p,
                                                 da.lconn.p);
-        da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
-        k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
-        k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
         HIP_OK(hipGetLastError());
     }
+    da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
+    k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
+    k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
+    HIP_OK(hipGetLastError());
+    epos.release();
     // shared nodes and their node-major partial slots
-    DevBuf<uint32_t> flag(np, s), fpos(np, s);
-    k_blk_shared_flag<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, ncount.p, np, flag.p);
-    uint64_t ns = scan_exclusive_u32(flag.p, fpos.p, np, s);
+    const uint32_t nn = da.n_nodes;
+    uint64_t ns = 0;
     da.bdst.alloc(np, s);
     HIP_OK(hipMemsetAsync(da.bdst.p, 0xff, np * 4, s));
-    if (ns) {
-        DevBuf<Key> sk(ns, s);
-        DevBuf<uint32_t> sv(ns, s), shead(ns, s), spos(ns, s);
-        k_blk_shared_keys<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, pair_block.p, flag.p, fpos.p, np, sk.p, sv.p);
-        radix_sort_pairs(sk.p, sv.p, ns, s, nblk, da.n_nodes);  // (node, block)
-        k_blk_shared_slots<<<grid_for(ns, B), B, 0, s>>>(sk.p, sv.p, ns, da.bdst.p, da.bnode.p, shead.p);
-        uint64_t nsn = scan_exclusive_u32(shead.p, spos.p, ns, s);
-        da.snode.alloc(nsn, s);
-        da.soff.alloc(nsn + 1, s);
-        k_blk_shared_nodes<<<grid_for(ns, B), B, 0, s>>>(sk.p, shead.p, spos.p, ns, da.snode.p, da.soff.p);
-        set_u32_async(da.soff.p + nsn, (uint32_t)ns, s);
-        da.n_shared = (uint32_t)nsn;
+    {
+        DevBuf<uint32_t> shf(nn, s), cnt2(nn, s), sidx(nn, s), base(nn, s), tot(2, s);
+        k_shared_count<<<grid_for(nn, B), B, 0, s>>>(ncount.p, nn, shf.p, cnt2.p);
+        scan_exclusive_u32_async(shf.p, sidx.p, nn, tot.p, s);
+        scan_exclusive_u32_async(cnt2.p, base.p, nn, tot.p + 1, s);
+        uint32_t ht[2];
+        HIP_OK(hipMemcpyAsync(ht, tot.p, sizeof ht, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        const uint64_t nsn = ht[0];
+        ns = ht[1];
+        if (ns) {
+            da.snode.alloc(nsn, s);
+            da.soff.alloc(nsn + 1, s);
+            k_shared_index<<<grid_for(nn, B), B, 0, s>>>(ncount.p, sidx.p, base.p, nn, da.snode.p, da.soff.p);
+            set_u32_async(da.soff.p + nsn, (uint32_t)ns, s);
+            DevBuf<uint32_t> cursor(nn, s), tblk(ns, s), tpair(ns, s);
+            HIP_OK(hipMemsetAsync(cursor.p, 0, (size_t)nn * 4, s));
+            k_shared_claim<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, pair_block.p, ncount.p, base.p, np, cursor.p,
+                                                         tblk.p, tpair.p);
+            k_shared_order<<<grid_for(nsn, B), B, 0, s>>>(da.soff.p, (uint32_t)nsn, tblk.p, tpair.p, da.bdst.p,
+                                                          da.bnode.p);
+            HIP_OK(hipGetLastError());
+            da.n_shared = (uint32_t)nsn;
+        }
     }
     k_blk_mark_bdy<<<grid_for(np, B), B, 0, s>>>(da.bnode.p, np, da.bdy_node.p);
     if (da.n_shared) k_blk_mark_bdy<<<grid_for(da.n_shared, B), B, 0, s>>>(da.snode.p, da.n_shared, da.bdy_node.p);
