@@ -43,9 +43,13 @@ struct DaDevice {
     ~DaDevice();
 };
 
-std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s);
+// colour: renumber each block's nodes against LDS bank conflicts (k_blk_color).
+// It makes every later matvec faster and costs about 40 matvecs once, so a
+// DA used for few matvecs (one served solve) is built without it.
+std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s,
+                                   bool colour = true);
 // Builds the matvec block view from conn (called by da_build).
-void da_build_blocks(DaDevice &da, hipStream_t s);
+void da_build_blocks(DaDevice &da, hipStream_t s, bool colour = true);
 
 // y = A x with A = lapFac * stiffness + massFac * mass (type-2 hanging stencils),
 // FP64 on compact node vectors (Jacobian1MatMult, odaJac.C:275-366).

@@ -285,7 +285,7 @@ __global__ void k_da_elem_view(const uint32_t *nlist, const uint8_t *mask, const
 
 DaDevice::~DaDevice() {}
 
-std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s) {
+std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int curve, hipStream_t s, bool colour) {
     if (n <= 8) {  // blockPartStage1 asserts nodes.size() > 2^dim (blockPart.cpp:583)
         dgx_set_error("ot::DA needs more than 8 octants (got %zu)", n);
         throw DgxError();
@@ -365,7 +365,7 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     k_da_elem_view<<<grid_for(n, B), B, 0, s>>>(da->nlist.p, da->mask.p, da->l2n.p, da->x.p, da->y.p, da->z.p,
                                                da->flags.p, n, md, da->conn.p, da->meta.p);
     HIP_OK(hipGetLastError());
-    da_build_blocks(*da, s);
+    da_build_blocks(*da, s, colour);
     HIP_OK(hipStreamSynchronize(s));
     return da;
 }
@@ -780,7 +780,7 @@ __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx
 
 }  // namespace
 
-void da_build_blocks(DaDevice &da, hipStream_t s) {
+void da_build_blocks(DaDevice &da, hipStream_t s, bool colour) {
     const size_t E = da.n_elem, T = 8 * E;
     const uint32_t eb = da.elem_block;
     if (eb == 0 || eb > 8192) {
@@ -824,7 +824,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s) {
     }
     da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
     k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
-    k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
+    if (colour) k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
     HIP_OK(hipGetLastError());
     epos.release();
     // shared nodes and their node-major partial slots

@@ -54,9 +54,10 @@ void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream
 // With overlap, each level's DA, its diagonal and the coarse inverse are built
 // on a second host thread and stream while the next levels are coarsened
 // (faster when nothing else drives the device; a caller that already runs
-// several set-ups at once passes false).
+// several set-ups at once passes false).  colour as for da_build.
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
-                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap = true);
+                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap = true,
+                                      bool colour = true);
 
 struct SolveOpts {
     int fp32_vcycle = 1;

@@ -576,7 +576,7 @@ class SetupHelper {
 }  // namespace
 
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
-                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap) {
+                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap, bool colour) {
     std::unique_ptr<MgHierarchy> mg(new MgHierarchy());
     mg->md = md;
     mg->curve = curve;
@@ -594,9 +594,9 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     // calling thread coarsens and balances the next level; the stopping rule
     // needs a level's node count only once the requested levels exist.
     SetupHelper helper(overlap ? helper_stream() : s, s, overlap);
-    auto build_da = [&, md, curve](MgLevel *L) {
-        return helper.submit([L, md, curve, lap_fac, mass_fac](hipStream_t hs) {
-            L->da = da_build(L->keys.p, L->n_oct, md, curve, hs);
+    auto build_da = [&, md, curve, colour](MgLevel *L) {
+        return helper.submit([L, md, curve, lap_fac, mass_fac, colour](hipStream_t hs) {
+            L->da = da_build(L->keys.p, L->n_oct, md, curve, hs, colour);
             build_level_diag(*L, lap_fac, mass_fac, hs);
         });
     };

@@ -232,8 +232,10 @@ struct dgx_server_s {
             const double tb = now();
             d.r.s_octree = tb - ta;
             // several slots already overlap their set-ups; one slot overlaps
-            // the levels of its own set-up instead
-            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s, o.slots == 1);
+            // the levels of its own set-up instead.  A request solves once, so
+            // its DAs are not coloured (the colouring costs more than the
+            // matvecs of one solve save).
+            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s, o.slots == 1, false);
             mg = own.get();
             b.release();
             HIP_OK(hipStreamSynchronize(s));
