@@ -198,8 +198,23 @@ __global__ void __launch_bounds__(B) k_restrict(const uint32_t *__restrict__ rof
     const uint32_t t = blockIdx.x * B + threadIdx.x, i = t >> 3, sub = t & 7u;
     T acc = T(0);
     if (i < nc) {
+        // four entries per lane per round: their columns and weights, then
+        // their gathers, are in flight together; the summation order is the
+        // entry order
         const uint32_t p1 = roff[i + 1];
-        for (uint32_t p = roff[i] + sub; p < p1; p += 8) acc += rval[p] * rf[rcol[p]];
+        for (uint32_t pb = roff[i] + sub; pb < p1; pb += 32) {
+            uint32_t c[4];
+            T v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t p = pb + 8 * k;
+                c[k] = p < p1 ? rcol[p] : 0u;
+                v[k] = p < p1 ? rval[p] : T(0);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (pb + 8 * k < p1) acc += v[k] * rf[c[k]];
+        }
     }
     acc += __shfl_xor(acc, 4, 64);
     acc += __shfl_xor(acc, 2, 64);
@@ -213,8 +228,19 @@ __global__ void __launch_bounds__(B) k_prolong_add(const uint32_t *__restrict__ 
                                                    T *__restrict__ xf) {
     const uint32_t n = blockIdx.x * B + threadIdx.x;
     if (n >= nf) return;
+    // a row has at most 8 entries: load them all, then gather all
+    const uint32_t p0 = poff[n], p1 = poff[n + 1];
+    uint32_t c[8];
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        c[k] = p0 + k < p1 ? pcol[p0 + k] : 0u;
+        v[k] = p0 + k < p1 ? pval[p0 + k] : T(0);
+    }
     T acc = T(0);
-    for (uint32_t p = poff[n], p1 = poff[n + 1]; p < p1; ++p) acc += pval[p] * xc[pcol[p]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (p0 + k < p1) acc += v[k] * xc[c[k]];
     xf[n] += acc;
 }
 __global__ void k_ell_count(const uint32_t *ecol, uint32_t nf, uint32_t *cnt) {
@@ -257,10 +283,24 @@ __global__ void k_f32_to_f64(const float *a, uint32_t n, double *b) {
     if (i < n) b[i] = (double)a[i];
 }
 // Two-stage deterministic dot products: per-block partial sums, then one block.
+// Per block: the sum over its grid-stride share, DOT_U pairs per thread
+// loaded ahead of their products (a fixed order, so the result is
+// deterministic).
+constexpr int DOT_U = 4;
 __global__ void k_dot_partial(const double *a, const double *b, uint32_t n, double *part) {
     __shared__ double sh[B / 64];
     double acc = 0.0;
-    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) acc += a[i] * b[i];
+    for (uint32_t i0 = blockIdx.x * B * DOT_U + threadIdx.x; i0 < n; i0 += gridDim.x * B * DOT_U) {
+        double va[DOT_U], vb[DOT_U];
+#pragma unroll
+        for (int u = 0; u < DOT_U; ++u) {
+            const uint32_t i = i0 + u * B;
+            va[u] = i < n ? a[i] : 0.0;
+            vb[u] = i < n ? b[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < DOT_U; ++u) acc += va[u] * vb[u];
+    }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -304,7 +344,7 @@ __global__ void k_mask_bdy(const uint8_t *bdy, uint32_t n, double *v) {
     if (i < n && bdy[i]) v[i] = 0.0;
 }
 
-constexpr uint32_t DOT_BLOCKS = 512;
+constexpr uint32_t DOT_BLOCKS = 1024;
 
 }  // namespace
 
