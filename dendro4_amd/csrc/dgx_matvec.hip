@@ -501,16 +501,24 @@ __global__ void __launch_bounds__(B) k_matvec_shared(const uint32_t *__restrict_
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i >= ns) return;
     const uint32_t g = snode[i], id = g & ID_MASK;
-    T v;
-    if (DIR && (g & BDY_BIT)) {
-        v = x[id];
-    } else {
-        v = T(0);
-        for (uint32_t p = soff[i]; p < soff[i + 1]; ++p) v += partial[p];
-    }
+    // the first 8 partials and the writeout's operands are loaded together;
+    // the sum runs in slot (block) order
+    const uint32_t p0 = soff[i], p1 = soff[i + 1];
+    T pv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pv[k] = p0 + k < p1 ? partial[p0 + k] : T(0);
+    const T xi = (DIR || EPI == EPI_JACOBI) ? x[id] : T(0);
+    const T bi = EPI != EPI_Y ? bvec[id] : T(0);
+    const T di = EPI == EPI_JACOBI ? dinv[id] : T(0);
+    T v = T(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (p0 + k < p1) v += pv[k];
+    for (uint32_t p = p0 + 8; p < p1; ++p) v += partial[p];
+    if (DIR && (g & BDY_BIT)) v = xi;
     if (EPI == EPI_Y) y[id] = v;
-    else if (EPI == EPI_RESID) y[id] = bvec[id] - v;
-    else y[id] = x[id] + omega * dinv[id] * (bvec[id] - v);
+    else if (EPI == EPI_RESID) y[id] = bi - v;
+    else y[id] = xi + omega * di * (bi - v);
 }
 // Stand-alone writeouts for the blocks the pipelined kernel does not take.
 template <typename T>
