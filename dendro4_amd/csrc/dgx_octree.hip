@@ -502,7 +502,9 @@ __global__ void __launch_bounds__(B) k_import_tn(const uint32_t *tn, size_t n, u
 // parents of that level, then the neighbour parents).  Leaves and neighbour
 // parents may come out in any order: seeds are sorted per level and the
 // leaves once at the end, exactly as in the multi-kernel step.
-constexpr uint32_t kFuseCap = 2048, kFuseB = 1024;
+// 4096 seeds (4 per thread) ran the same build in the same time: the levels
+// it adds cost the single workgroup what their kernels and host reads cost.
+constexpr uint32_t kFuseCap = 2048, kFuseB = 1024, kFuseF = kFuseCap / kFuseB;  // seeds per thread
 struct FuseState {
     uint32_t level;      // next level to process (0: all done)
     uint32_t nw;         // its seeds: in wbuf[wsel], or the input parents of that level when from_inpar
@@ -550,11 +552,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ bool fuse_less(const uint64_t &a, const uint64_t &b) { return a < b; }
 __device__ __forceinline__ bool fuse_less(const Key &a, const Key &b) { return key_lt(a, b); }
 
-// Bitonic sort of a[0..P) (P a power of two <= 2 kFuseB) by the kFuseB
-// threads.  In every stage thread t performs compare-exchange t: elements
-// i = t with a zero bit inserted at log2(j), and i + j.  For partner distances
-// j <= 64 the comparisons of one wave stay inside its own 128 elements, so
-// consecutive such stages are separated by a wave-level fence only.
+// Bitonic sort of a[0..P) (P a power of two <= kFuseCap) by the kFuseB
+// threads.  In every stage thread t performs compare-exchanges c = t + q kFuseB:
+// elements i = c with a zero bit inserted at log2(j), and i + j.  For partner
+// distances j <= 64 the exchanges of wave w touch only elements
+// [128 w + 2 kFuseB q, +128), which no other wave touches, so consecutive such
+// stages are separated by a wave-level fence only.
 template <typename T>
 __device__ void fuse_bitonic(T *a, uint32_t P) {
     const uint32_t t = threadIdx.x, ncmp = P >> 1;
@@ -566,9 +569,9 @@ __device__ void fuse_bitonic(T *a, uint32_t P) {
                 wave_lds_sync();
             else
                 __syncthreads();
-            if (t < ncmp) {
-                const uint32_t lj = __ffs(j) - 1;
-                const uint32_t i = ((t >> lj) << (lj + 1)) | (t & (j - 1)), o = i + j;
+            const uint32_t lj = __ffs(j) - 1;
+            for (uint32_t c = t; c < ncmp; c += kFuseB) {
+                const uint32_t i = ((c >> lj) << (lj + 1)) | (c & (j - 1)), o = i + j;
                 const bool up = (i & kk) == 0;
                 const T x = a[i], y = a[o];
                 if (up ? fuse_less(y, x) : fuse_less(x, y)) {
@@ -617,14 +620,22 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
             fuse_bitonic(sk, P);
         }
         __syncthreads();
-        const uint32_t i0 = 2 * t, i1 = 2 * t + 1;
-        const Key k0 = sk[i0 < kFuseCap ? i0 : 0], k1 = sk[i1 < kFuseCap ? i1 : 0];
-        const uint32_t f0 = i0 < nw && (i0 == 0 || !key_eq(k0, sk[i0 - 1])) ? 1u : 0u;
-        const uint32_t f1 = i1 < nw && !key_eq(k1, k0) ? 1u : 0u;
+        // thread t owns entries kFuseF t .. kFuseF t + kFuseF - 1
+        const uint32_t ib = kFuseF * t;
+        Key kq[kFuseF];
+        uint32_t fq[kFuseF], fsum = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < kFuseF; ++e) {
+            const uint32_t i = ib + e;
+            kq[e] = sk[i];
+            fq[e] = i < nw && (i == 0 || !key_eq(kq[e], e ? kq[e - 1] : sk[i - 1])) ? 1u : 0u;
+            fsum += fq[e];
+        }
         uint32_t nq;
-        const uint32_t qpos = fuse_block_scan(f0 + f1, tmp, nq);  // also orders the reads above before the writes
-        if (f0) sk[qpos] = k0;
-        if (f1) sk[qpos + f0] = k1;
+        uint32_t qpos = fuse_block_scan(fsum, tmp, nq);  // also orders the reads above before the writes
+#pragma unroll
+        for (uint32_t e = 0; e < kFuseF; ++e)
+            if (fq[e]) sk[qpos++] = kq[e];
         __syncthreads();
         Key *Qg = (k & 1) ? qbuf1 : qbuf0;  // this level's split octants, the next level's prevQ
         for (uint32_t i = t; i < nq; i += kFuseB) {
@@ -640,11 +651,13 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         }
         __syncthreads();
         // leaves of level l: the children of Q that are not split
-        const uint32_t c0 = i0 < nq ? 8u - __popc(cm[i0] & 255u) : 0u, c1 = i1 < nq ? 8u - __popc(cm[i1] & 255u) : 0u;
+        uint32_t csum = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < kFuseF; ++e) csum += ib + e < nq ? 8u - __popc(cm[ib + e] & 255u) : 0u;
         uint32_t nlv;
-        uint32_t lpos = used + fuse_block_scan(c0 + c1, tmp, nlv);
-        for (uint32_t e = 0; e < 2; ++e) {
-            const uint32_t i = e ? i1 : i0;
+        uint32_t lpos = used + fuse_block_scan(csum, tmp, nlv);
+        for (uint32_t e = 0; e < kFuseF; ++e) {
+            const uint32_t i = ib + e;
             if (i >= nq) continue;
             uint32_t x, y, z, lq, rot;
             morton_decode(sk[i], md, x, y, z, lq);
@@ -667,17 +680,19 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         const Key *src = inpar + loff[l - 1];
         Key *Wn = (k & 1) ? wbuf1 : wbuf0;
         const uint32_t lvl = l - 1;
-        uint32_t pc0[2][3], pc1[2][3];
-        uint32_t valid[2] = {0u, 0u};
-        for (uint32_t e = 0; e < 2; ++e) {
-            const uint32_t i = e ? i1 : i0;
-            if (i < nq) valid[e] = nbr_parent_axes(sk[i], md, lvl, pc0[e], pc1[e]);
+        uint32_t pc0[kFuseF][3], pc1[kFuseF][3];
+        uint32_t valid[kFuseF], vsum = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < kFuseF; ++e) {
+            const uint32_t i = ib + e;
+            valid[e] = i < nq ? nbr_parent_axes(sk[i], md, lvl, pc0[e], pc1[e]) : 0u;
+            vsum += (uint32_t)__popc(valid[e]);
         }
         uint32_t nnb;
-        uint32_t npos = nin + fuse_block_scan(__popc(valid[0]) + __popc(valid[1]), tmp, nnb);
+        uint32_t npos = nin + fuse_block_scan(vsum, tmp, nnb);
         const uint32_t nwn = nin + nnb;  // <= largest level input count + 8 kFuseCap = the buffers' capacity
         for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
-        for (uint32_t e = 0; e < 2; ++e)
+        for (uint32_t e = 0; e < kFuseF; ++e)
             for (int b = 0; b < 8; ++b)
                 if ((valid[e] >> b) & 1u)
                     Wn[npos++] = morton_encode((b & 1) ? pc1[e][0] : pc0[e][0], (b & 2) ? pc1[e][1] : pc0[e][1],
