@@ -59,7 +59,7 @@ def build_library(verbose=True, jobs=None):
         msg = "\n".join(f"--- {s} ---\n{log}" for s, log in errors)
         raise RuntimeError("hipcc failed:\n" + msg)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lrocsolver", "-lrocblas"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

@@ -20,8 +20,6 @@
 //     coarse solve stay FP64.
 #include "dgx_mg.h"
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "../../include/dendro_mi355x.h"  // dgx_last_error
 
@@ -177,11 +175,6 @@ __global__ void k_dinv(const double *diag, const uint8_t *bdy, uint32_t n, doubl
 
 // Gauss-Jordan inversion of an SPD matrix, one pivot per launch (no pivoting).
 // Column-major lower triangle (row >= col) -> full symmetric matrix.
-__global__ void k_mirror_lower(double *A, uint32_t n) {
-    const uint32_t r = blockIdx.x * B + threadIdx.x, c = blockIdx.y;
-    if (r >= n || r <= c) return;
-    A[(size_t)r * n + c] = A[(size_t)c * n + r];
-}
 
 // -------------------------------------------------------- vector kernels ----
 template <typename T>
@@ -431,20 +424,117 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
     HIP_OK(hipGetLastError());
 }
 
-// Coarsest level: the dense Dirichlet matrix (SPD) is factored and inverted
-// in FP64 by rocSOLVER (Cholesky, potrf + potri), then mirrored into a full
-// symmetric inverse applied as a GEMV per V-cycle.  The reference factors its
-// coarsest matrix with PETSc LU (blockDiag.cpp:780-1047).
-static std::mutex g_blas_mu;
-static rocblas_handle blas_handle() {
-    static rocblas_handle h = nullptr;
-    if (!h) {
-        if (rocblas_create_handle(&h) != rocblas_status_success) {
-            dgx_set_error("multigrid: rocblas_create_handle failed");
-            throw DgxError();
+// Coarsest level: the dense Dirichlet matrix (SPD) is inverted in place in
+// FP64 by blocked Gauss-Jordan elimination (no pivoting: every pivot block is
+// a Schur complement of an SPD matrix, hence SPD), and the inverse is applied
+// as a GEMV per V-cycle.  The reference factors its coarsest matrix with
+// PETSc LU (blockDiag.cpp:780-1047).  Per block step of kGJ rows/columns K:
+//   Dinv = A[K,K]^-1 (one workgroup, in LDS), T = Dinv A[K,:], C = A[:,K];
+//   A[i,j] -= C[i,:] T[:,j] (i, j outside K), A[K,j] = T[:,j],
+//   A[i,K] = -C[i,:] Dinv, A[K,K] = Dinv.
+constexpr uint32_t kGJ = 32;
+
+// Dinv = inverse of the kGJ x kGJ block at (k0, k0) (the part inside n), by
+// Gauss-Jordan in LDS; rows/columns past n are an identity.  *bad is set for
+// a pivot that is not positive and finite.
+__global__ void __launch_bounds__(kGJ * kGJ) k_gj_diag(const double *A, uint32_t n, uint32_t k0, double *dinv,
+                                                      uint32_t *bad) {
+    __shared__ double D[kGJ][kGJ + 1];
+    const uint32_t r = threadIdx.x / kGJ, c = threadIdx.x % kGJ;
+    const bool in = k0 + r < n && k0 + c < n;
+    D[r][c] = in ? A[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    for (uint32_t p = 0; p < kGJ; ++p) {
+        __syncthreads();
+        const double piv = D[p][p], f = D[r][p], dpc = D[p][c];
+        __syncthreads();
+        if (threadIdx.x == 0 && !(piv > 0.0 && piv < INFINITY)) atomicOr(bad, 1u);
+        double v;
+        if (r == p) v = (c == p) ? 1.0 / piv : dpc / piv;
+        else v = (c == p) ? -f / piv : D[r][c] - f * (dpc / piv);
+        D[r][c] = v;
+    }
+    __syncthreads();
+    dinv[r * kGJ + c] = D[r][c];
+}
+// T[q][j] = sum_c Dinv[q][c] A[k0 + c][j] for every column j, and
+// C[i][c] = A[i][k0 + c] for every row i (the old panels, read before the
+// update kernel overwrites them).
+__global__ void __launch_bounds__(B) k_gj_panels(const double *A, uint32_t n, uint32_t k0, uint32_t kb,
+                                                 const double *dinv, double *T, double *C) {
+    __shared__ double Ds[kGJ * kGJ];
+    for (uint32_t t = threadIdx.x; t < kGJ * kGJ; t += B) Ds[t] = dinv[t];
+    __syncthreads();
+    const uint32_t j = blockIdx.x * B + threadIdx.x;
+    if (j >= n) return;
+    double a[kGJ];
+#pragma unroll
+    for (uint32_t c = 0; c < kGJ; ++c) a[c] = c < kb ? A[(size_t)(k0 + c) * n + j] : 0.0;
+#pragma unroll 4
+    for (uint32_t q = 0; q < kb; ++q) {
+        double t = 0.0;
+#pragma unroll
+        for (uint32_t c = 0; c < kGJ; ++c) t += Ds[q * kGJ + c] * a[c];
+        T[(size_t)q * n + j] = t;
+    }
+    const uint32_t i = j;  // one thread per row as well
+#pragma unroll
+    for (uint32_t c = 0; c < kGJ; ++c)
+        if (c < kb) C[(size_t)i * kGJ + c] = A[(size_t)i * n + k0 + c];
+}
+// The rank-kb update and the block's new rows, columns and pivot block.  A
+// workgroup owns a 64 x 64 tile; each thread 4 x 4 entries.
+constexpr uint32_t kGJT = 64;
+__global__ void __launch_bounds__(B) k_gj_update(double *A, uint32_t n, uint32_t k0, uint32_t kb, const double *dinv,
+                                                 const double *T, const double *C) {
+    __shared__ double Cs[kGJT][kGJ + 1];
+    __shared__ double Ts[kGJ][kGJT + 1];
+    const uint32_t i0 = blockIdx.y * kGJT, j0 = blockIdx.x * kGJT;
+    for (uint32_t t = threadIdx.x; t < kGJT * kGJ; t += B) {
+        const uint32_t rr = t / kGJ, cc = t % kGJ, i = i0 + rr;
+        Cs[rr][cc] = (i < n && cc < kb) ? C[(size_t)i * kGJ + cc] : 0.0;
+        const uint32_t q = t / kGJT, jj = t % kGJT, j = j0 + jj;
+        Ts[q][jj] = (j < n && q < kb) ? T[(size_t)q * n + j] : 0.0;
+    }
+    __syncthreads();
+    const uint32_t tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+    double acc[4][4] = {};
+#pragma unroll 8
+    for (uint32_t c = 0; c < kGJ; ++c) {
+        double cv[4], tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cv[u] = Cs[tr + u][c];
+            tv[u] = Ts[c][tc + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += cv[u] * tv[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const uint32_t i = i0 + tr + u;
+        if (i >= n) continue;
+        const bool iK = i >= k0 && i < k0 + kb;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t j = j0 + tc + v;
+            if (j >= n) continue;
+            const bool jK = j >= k0 && j < k0 + kb;
+            double *a = &A[(size_t)i * n + j];
+            if (iK && jK) {
+                *a = dinv[(i - k0) * kGJ + (j - k0)];
+            } else if (iK) {
+                *a = Ts[i - k0][tc + v];
+            } else if (jK) {
+                double s = 0.0;
+                for (uint32_t c = 0; c < kb; ++c) s += Cs[tr + u][c] * dinv[c * kGJ + (j - k0)];
+                *a = -s;
+            } else {
+                *a -= acc[u][v];
+            }
         }
     }
-    return h;
 }
 
 static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
@@ -459,28 +549,23 @@ static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     mg.nc = n;
     mg.cinv.alloc((size_t)n * n, s);
     da_assemble_dense_dirichlet(d, mg.lap_fac, mg.mass_fac, mg.cinv.p, s);
-    DevBuf<rocblas_int> info(2, s);
-    {
-        std::lock_guard<std::mutex> lk(g_blas_mu);
-        rocblas_handle h = blas_handle();
-        if (rocblas_set_stream(h, s) != rocblas_status_success ||
-            rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, mg.cinv.p, (rocblas_int)n, info.p) !=
-                rocblas_status_success ||
-            rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, mg.cinv.p, (rocblas_int)n, info.p + 1) !=
-                rocblas_status_success) {
-            dgx_set_error("multigrid: rocSOLVER Cholesky of the %u x %u coarse matrix failed", n, n);
-            throw DgxError();
-        }
-        k_mirror_lower<<<dim3(grid_for(n, B), n), B, 0, s>>>(mg.cinv.p, n);
-        HIP_OK(hipGetLastError());
-        rocblas_int hinfo[2] = {0, 0};
-        HIP_OK(hipMemcpyAsync(hinfo, info.p, sizeof hinfo, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (hinfo[0] || hinfo[1]) {
-            dgx_set_error("multigrid: coarse matrix is not positive definite (potrf %d, potri %d)", (int)hinfo[0],
-                          (int)hinfo[1]);
-            throw DgxError();
-        }
+    DevBuf<double> dinv(kGJ * kGJ, s), T((size_t)kGJ * n, s), C((size_t)n * kGJ, s);
+    DevBuf<uint32_t> bad(1, s);
+    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
+    const dim3 tiles(grid_for(n, kGJT), grid_for(n, kGJT));
+    for (uint32_t k0 = 0; k0 < n; k0 += kGJ) {
+        const uint32_t kb = std::min(kGJ, n - k0);
+        k_gj_diag<<<1, kGJ * kGJ, 0, s>>>(mg.cinv.p, n, k0, dinv.p, bad.p);
+        k_gj_panels<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
+        k_gj_update<<<tiles, B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
+    }
+    HIP_OK(hipGetLastError());
+    uint32_t hbad = 0;
+    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (hbad) {
+        dgx_set_error("multigrid: coarse matrix is not positive definite (a Gauss-Jordan pivot is not positive)");
+        throw DgxError();
     }
 }
 
