@@ -262,8 +262,21 @@ template <typename T>
 __global__ void k_dense_gemv(const double *A, uint32_t n, const T *x, T *y) {
     const uint32_t row = blockIdx.x * (B / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= n) return;
+    // eight entries per lane are loaded ahead of their products; each lane
+    // still sums its entries in column order
     double acc = 0.0;
-    for (uint32_t j = lane; j < n; j += 64) acc += A[(size_t)row * n + j] * (double)x[j];
+    const double *Ar = A + (size_t)row * n;
+    for (uint32_t j0 = lane; j0 < n; j0 += 64 * 8) {
+        double a[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t j = j0 + 64 * u;
+            a[u] = j < n ? Ar[j] : 0.0;
+            xv[u] = j < n ? (double)x[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
+    }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (lane == 0) y[row] = (T)acc;
 }
@@ -278,7 +291,9 @@ __global__ void k_f32_to_f64(const float *a, uint32_t n, double *b) {
 // Two-stage deterministic dot products: per-block partial sums, then one block.
 // Per block: the sum over its grid-stride share, DOT_U pairs per thread
 // loaded ahead of their products (a fixed order, so the result is
-// deterministic).
+// deterministic).  A one-kernel form, in which the block that finishes last
+// sums the partials, was slower: each block's release and acquire at device
+// scope write back and invalidate its L2.
 constexpr int DOT_U = 4;
 __global__ void k_dot_partial(const double *a, const double *b, uint32_t n, double *part) {
     __shared__ double sh[B / 64];
