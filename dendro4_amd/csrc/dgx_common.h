@@ -210,6 +210,42 @@ __device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t *lds, uint32
     __syncthreads();
     return r;
 }
+// Decoupled look-back over tiles for one 32-bit value per tile, for
+// single-pass scans.  status[t] holds bits 0-31 a count, bits 32-61 the call's
+// epoch (status words are zeroed once and every call uses a new epoch), bits
+// 62-63 whether the count is the tile's own (1) or inclusive of all tiles
+// before it (2).  Called by one thread per tile, with tiles numbered in the
+// order they started (so every wait is for a running tile); returns the sum
+// of the earlier tiles' values.  A wait past its bound sets *fault.
+constexpr uint64_t LB_OWN = 1ull << 62, LB_INCL = 2ull << 62;
+__device__ inline uint32_t lookback_prefix(uint64_t *status, uint32_t tile, uint32_t value, uint32_t epoch,
+                                           uint32_t *fault) {
+    const uint64_t ep = (uint64_t)epoch << 32;
+    uint64_t *mine = status + tile;
+    if (tile == 0) {
+        __hip_atomic_store(mine, LB_INCL | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    __hip_atomic_store(mine, LB_OWN | ep | value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0, t = tile - 1;
+    for (;;) {
+        uint64_t w = __hip_atomic_load(status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while ((w & ~(3ull << 62) & ~0xffffffffull) != ep) {
+            if (++spins > (1u << 20)) {
+                __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            w = __hip_atomic_load(status + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        excl += (uint32_t)w;
+        if ((w & LB_INCL) || t == 0 || spins > (1u << 20)) break;
+        --t;
+    }
+    __hip_atomic_store(mine, LB_INCL | ep | (excl + value), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
 // Reserve n slots on *counter for this lane; returns the first slot.  Every
 // lane of the wave must call it (inactive lanes pass 0).
 __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
@@ -375,6 +411,8 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
 // The same sort for keys that pack two bounded integers: lo <= lo_max and
 // hi <= hi_max; only the bytes those bounds can reach get a pass.
 void radix_sort_pairs(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t lo_max, uint64_t hi_max);
+// The pinned host word the look-back kernels set on a wait past its bound.
+uint32_t *dgx_fault_word();
 // Throws DgxError if a sort kernel reported a look-back wait that exceeded its
 // bound since the last check (the flag is cleared).
 void sort_fault_check();

@@ -358,22 +358,6 @@ __global__ void __launch_bounds__(B) k_bucket_parents(const Key *k, size_t n, ui
         if (i0 + (size_t)r * B < n) out[base[lv[r]] + loc[r]] = par[r];
 }
 
-__global__ void k_unique_flag(const Key *k, size_t n, uint32_t *flag) {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    flag[j] = (j == 0 || !key_eq(k[j], k[j - 1])) ? 1u : 0u;
-}
-// Compaction of the flagged keys; also clears zmask[0..n) and *zcount for the
-// kernels that follow (one launch instead of two fills).
-__global__ void k_compact_keys(const Key *in, const uint32_t *flag, const uint32_t *pos, size_t n, Key *out,
-                               uint32_t *zmask, uint32_t *zcount) {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) *zcount = 0;
-    if (j >= n) return;
-    zmask[j] = 0;
-    if (flag[j]) out[pos[j]] = in[j];
-}
-
 // Parents of the 26 neighbours of every split octant: per axis the neighbours'
 // parents take at most two values, so at most 8 distinct parents per octant.
 // They go after the next level's nseed input parents, which the same launch
@@ -450,25 +434,92 @@ __global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, c
 }
 
 // cnt[i] for i < cap: leaves under split octant i (0 beyond the device count).
-__global__ void k_leaf_count(const uint32_t *cmask, const uint32_t *d_nq, size_t cap, uint32_t *cnt) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cap) return;
-    cnt[i] = i < *d_nq ? 8 - __popc(cmask[i] & 255u) : 0u;
+// Single-pass forms of the balance level's two compactions (flag, scan,
+// write at offset): tiles of FS_TILE entries, FS_ITEMS consecutive entries
+// per thread, tile ids from ctr[epoch], offsets by decoupled look-back.
+constexpr uint32_t FS_B = 256, FS_ITEMS = 8, FS_TILE = FS_B * FS_ITEMS;
+
+// Q = the distinct keys of the sorted W[0..n) in order, *d_nq their count;
+// also clears cmask[0..*d_nq) and *zcount for the kernels that follow.
+__global__ void __launch_bounds__(FS_B) k_unique_compact(const Key *W, size_t n, Key *Q, uint32_t *cmask,
+                                                         uint32_t *d_nq, uint32_t *zcount, uint64_t *status,
+                                                         uint32_t *ctr, uint32_t epoch, uint32_t *fault) {
+    __shared__ uint32_t s_tile, s_excl, tmp[FS_B / 64 + 1];
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[epoch], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t i0 = (size_t)tile * FS_TILE + (size_t)threadIdx.x * FS_ITEMS;
+    Key kv[FS_ITEMS];
+    Key prev = (i0 > 0 && i0 - 1 < n) ? W[i0 - 1] : Key{~0ull, ~0ull};
+    uint32_t fl = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < FS_ITEMS; ++e) {
+        const size_t i = i0 + e;
+        if (i < n) {
+            kv[e] = W[i];
+            if (i == 0 || !key_eq(kv[e], prev)) {
+                fl |= 1u << e;
+                ++cnt;
+            }
+            prev = kv[e];
+        }
+    }
+    uint32_t total;
+    const uint32_t tex = block_excl_scan_u32(cnt, tmp, total);
+    if (threadIdx.x == 0) s_excl = lookback_prefix(status, tile, total, epoch, fault);
+    __syncthreads();
+    uint32_t pos = s_excl + tex;
+#pragma unroll
+    for (uint32_t e = 0; e < FS_ITEMS; ++e)
+        if ((fl >> e) & 1u) {
+            Q[pos] = kv[e];
+            cmask[pos] = 0;
+            ++pos;
+        }
+    if (threadIdx.x == 0) {
+        if (tile + 1 == gridDim.x) *d_nq = s_excl + total;
+        if (tile == 0) *zcount = 0;
+    }
 }
 
-__global__ void __launch_bounds__(B) k_leaf_emit(const Key *Q, const uint32_t *d_nq, const uint32_t *cmask,
-                                                 const uint32_t *off, uint32_t md, SfcPacked tab, Key *out) {
+// The leaves of one level: the children of the split octants Q[0..*d_nq) that
+// are not split themselves (cmask), in Q order and child order, written to
+// out from offset 0; *d_nlv their count.  The same tiles as k_unique_compact
+// on 1024 threads, 2 octants each: emitting an octant's leaves is a chain of
+// table reads, so fewer octants per thread keep the chains short.
+constexpr uint32_t LE_B = 1024, LE_ITEMS = FS_TILE / LE_B;
+__global__ void __launch_bounds__(LE_B) k_leaf_emit_scan(const Key *Q, const uint32_t *d_nq, const uint32_t *cmask,
+                                                         uint32_t md, SfcPacked tab, Key *out, uint32_t *d_nlv,
+                                                         uint64_t *status, uint32_t *ctr, uint32_t epoch,
+                                                         uint32_t *fault) {
     LOAD_TABLES(tab)
-    size_t i = (size_t)blockIdx.x * B + threadIdx.x;
-    if (i >= *d_nq) return;
-    uint32_t x, y, z, l, rot;
-    morton_decode(Q[i], md, x, y, z, l);
-    const Key pk = dev_encode_rot(s_rc, x, y, z, l, md, rot);
-    uint32_t o = off[i], msk = cmask[i];
-    for (uint32_t c = 0; c < 8; ++c) {
-        if (msk & (1u << c)) continue;
-        out[o++] = child_key(s_rc, pk, rot, c);
+    __shared__ uint32_t s_tile, s_excl, tmp[LE_B / 64 + 1];
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[epoch], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile, nq = *d_nq;
+    const size_t i0 = (size_t)tile * FS_TILE + (size_t)threadIdx.x * LE_ITEMS;
+    uint32_t msk[LE_ITEMS], cnt = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < LE_ITEMS; ++e) {
+        const size_t i = i0 + e;
+        msk[e] = i < nq ? cmask[i] : 0xffu;
+        cnt += 8u - (uint32_t)__popc(msk[e] & 255u);
     }
+    uint32_t total;
+    const uint32_t tex = block_excl_scan_u32(cnt, tmp, total);
+    if (threadIdx.x == 0) s_excl = lookback_prefix(status, tile, total, epoch, fault);
+    __syncthreads();
+    uint32_t pos = s_excl + tex;
+    for (uint32_t e = 0; e < LE_ITEMS; ++e) {
+        const size_t i = i0 + e;
+        if (i >= nq) break;
+        uint32_t x, y, z, l, rot;
+        morton_decode(Q[i], md, x, y, z, l);
+        const Key pk = dev_encode_rot(s_rc, x, y, z, l, md, rot);
+        for (uint32_t c = 0; c < 8; ++c)
+            if (!((msk[e] >> c) & 1u)) out[pos++] = child_key(s_rc, pk, rot, c);
+    }
+    if (threadIdx.x == 0 && tile + 1 == gridDim.x) *d_nlv = s_excl + total;
 }
 
 __global__ void __launch_bounds__(B) k_export_tn(const Key *k, size_t n, uint32_t md, SfcPacked tab, uint32_t *tn) {
@@ -897,6 +948,13 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     // neighbour parents); the host reads them once per level.
     DevBuf<uint32_t> cnt(4, s);
     uint32_t *d_nq = cnt.p, *d_nlv = cnt.p + 1, *d_nnb = cnt.p + 2;
+    // single-pass compactions: look-back words and one tile counter per call
+    // (epoch), zeroed together and reused until the counters run out
+    DevBuf<uint64_t> lb_status;
+    DevBuf<uint32_t> lb_ctr(128, s);
+    HIP_OK(hipMemsetAsync(lb_ctr.p, 0, lb_ctr.n * sizeof(uint32_t), s));
+    uint32_t lb_epoch = 0;
+    uint32_t *fault = dgx_fault_word();
     DevBuf<Key> prevQ;  // split octants one level finer (sorted unique Morton)
     size_t nprev = 0;
     DevBuf<Key> W;      // seeds of the current level: input parents, then neighbour parents
@@ -967,22 +1025,29 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         const uint32_t lv = l - 1;
         const int begin_bit = lv <= 21 ? 127 - 3 * (int)lv : 63 - 3 * ((int)lv - 21);
         radix_sort_keys(W.p, nullptr, nw, s, begin_bit, 127);
-        DevBuf<uint32_t> f(nw, s), fp(nw, s);
-        k_unique_flag<<<grid_for(nw, B), B, 0, s>>>(W.p, nw, f.p);
-        scan_exclusive_u32_async(f.p, fp.p, nw, d_nq, s);
+        const unsigned fs_grid = (unsigned)((nw + FS_TILE - 1) / FS_TILE);
+        if (fs_grid > lb_status.n) {  // look-back words for the largest level so far, zeroed
+            lb_status.alloc(std::max<size_t>(fs_grid, 2 * lb_status.n), s);
+            HIP_OK(hipMemsetAsync(lb_status.p, 0, lb_status.n * sizeof(uint64_t), s));
+        }
+        if (lb_epoch + 2 >= lb_ctr.n) {  // a zeroed tile counter per call
+            HIP_OK(hipMemsetAsync(lb_ctr.p, 0, lb_ctr.n * sizeof(uint32_t), s));
+            HIP_OK(hipMemsetAsync(lb_status.p, 0, lb_status.n * sizeof(uint64_t), s));
+            lb_epoch = 0;
+        }
         DevBuf<Key> Q(nw, s);  // capacity; *d_nq used
-        DevBuf<uint32_t> cmask(nw, s), lc(nw, s), lo(nw, s);
-        k_compact_keys<<<grid_for(nw, B), B, 0, s>>>(W.p, f.p, fp.p, nw, Q.p, cmask.p, d_nnb);
+        DevBuf<uint32_t> cmask(nw, s);
+        k_unique_compact<<<fs_grid, FS_B, 0, s>>>(W.p, nw, Q.p, cmask.p, d_nq, d_nnb, lb_status.p, lb_ctr.p,
+                                                  ++lb_epoch, fault);
         // leaves at level l: children of Q not split themselves
         if (nprev) k_mark_split_children<<<grid_for(nprev, B), B, 0, s>>>(prevQ.p, nprev, Q.p, d_nq, cmask.p);
-        k_leaf_count<<<grid_for(nw, B), B, 0, s>>>(cmask.p, d_nq, nw, lc.p);
-        scan_exclusive_u32_async(lc.p, lo.p, nw, d_nlv, s);
         if (leaves.n < used + 8 * nw) {  // grow: at most 8 leaves per split octant
             DevBuf<Key> g(std::max(2 * leaves.n, used + 8 * nw), s);
             copy_keys(g.p, leaves.p, used, s);
             leaves = std::move(g);
         }
-        k_leaf_emit<<<grid_for(nw, B), B, 0, s>>>(Q.p, d_nq, cmask.p, lo.p, md, tab, leaves.p + used);
+        k_leaf_emit_scan<<<fs_grid, LE_B, 0, s>>>(Q.p, d_nq, cmask.p, md, tab, leaves.p + used, d_nlv, lb_status.p,
+                                                  lb_ctr.p, ++lb_epoch, fault);
         HIP_OK(hipGetLastError());
         // neighbours of the split octants seed the next coarser level
         DevBuf<Key> Wn;

@@ -466,9 +466,11 @@ uint32_t *fault_word() {
 }
 }  // namespace
 
+uint32_t *dgx_fault_word() { return fault_word(); }
+
 void sort_fault_check() {
     if (g_fault && __atomic_exchange_n(g_fault, 0u, __ATOMIC_ACQ_REL)) {
-        dgx_set_error("radix sort: a look-back wait exceeded its bound; results of that sort are invalid");
+        dgx_set_error("a look-back wait (radix sort or single-pass scan) exceeded its bound; that result is invalid");
         throw DgxError();
     }
 }
