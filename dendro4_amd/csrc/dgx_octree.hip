@@ -88,6 +88,69 @@ __global__ void k_p2o_leaf(const Key *pk, size_t n, uint32_t m, const uint8_t *l
     leafkey[j] = key_truncate(pk[j], L + 1);
 }
 
+// Keys sorted on their high words alone: each run of equal high words is put
+// in order of bits 8-63 of the low word (stably), which completes a stable
+// sort on bits 8-127.  The wave that holds a run's first key sorts the run:
+// lane l ranks keys l, l+64, ... against the whole run, then stores them at
+// their ranks.  Only low words move, and only that wave reads them, so the
+// high words other waves compare stay put.  A run longer than kHiRunCap is
+// left alone and sets *longrun.
+constexpr uint32_t kHiRunCap = 256, kHiRunQ = kHiRunCap / 64;
+__global__ void __launch_bounds__(B) k_sort_hi_runs(Key *k, size_t n, uint32_t *longrun) {
+    const size_t j = (size_t)blockIdx.x * B + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    // one load per key; the neighbours' high words come from the adjacent
+    // lanes, and from memory at the ends of the wave
+    const uint64_t h = j < n ? k[j].hi : 0ull;
+    uint64_t hp = __shfl_up(h, 1), hn = __shfl_down(h, 1);
+    if (lane == 0 && j > 0 && j <= n) hp = k[j - 1].hi;
+    if (lane == 63 && j + 1 < n) hn = k[j + 1].hi;
+    const bool start = j + 1 < n && hn == h && !(j > 0 && hp == h);
+    uint64_t starts = __ballot(start);
+    const size_t wbase = j - lane;
+    while (starts) {
+        const uint32_t sl = (uint32_t)__ffsll((unsigned long long)starts) - 1;
+        starts &= starts - 1;
+        const size_t j0 = wbase + sl;
+        const uint64_t h0 = __shfl(h, (int)sl);
+        // run end: the first key past j0 with another high word
+        size_t e = n;
+        for (size_t c = j0 + 1; c < n; c += 64) {
+            const size_t idx = c + lane;
+            const uint64_t other = __ballot(idx < n && k[idx].hi != h0);
+            if (other) {
+                e = c + (size_t)__ffsll((unsigned long long)other) - 1;
+                break;
+            }
+            if (c + 64 - j0 > kHiRunCap) break;
+        }
+        const size_t len = e - j0;
+        if (len > kHiRunCap) {
+            if (lane == 0) atomicOr(longrun, 1u);
+            continue;
+        }
+        uint64_t v[kHiRunQ];
+        uint32_t r[kHiRunQ];
+#pragma unroll
+        for (uint32_t q = 0; q < kHiRunQ; ++q) {
+            const uint32_t t = lane + 64 * q;
+            v[q] = t < len ? k[j0 + t].lo : 0ull;
+            r[q] = 0;
+        }
+        for (uint32_t t = 0; t < len; ++t) {
+            const uint64_t w = k[j0 + t].lo >> 8;
+#pragma unroll
+            for (uint32_t q = 0; q < kHiRunQ; ++q) {
+                const uint64_t x = v[q] >> 8;
+                r[q] += (w < x || (w == x && t < lane + 64 * q)) ? 1u : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kHiRunQ; ++q)
+            if (lane + 64 * q < len) k[j0 + r[q]].lo = v[q];
+    }
+}
+
 __global__ void k_head(const Key *k, size_t n, uint32_t *head) {
     size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -802,39 +865,53 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
     }
     DevBuf<Key> pk(n, s);
     // device counters: [0] distinct leaf keys Q, [1] textbook leaves nT,
-    // [2] a point outside the domain, [3] overlapping leaves (internal error)
-    DevBuf<uint32_t> ctr(4, s);
-    HIP_OK(hipMemsetAsync(ctr.p, 0, 4 * sizeof(uint32_t), s));
+    // [2] a point outside the domain, [3] overlapping leaves (internal error),
+    // [4] a run of equal high words too long for k_sort_hi_runs
+    DevBuf<uint32_t> ctr(5, s);
+    HIP_OK(hipMemsetAsync(ctr.p, 0, 5 * sizeof(uint32_t), s));
     const double gx = glens ? glens[0] : 1.0, gy = glens ? glens[1] : 1.0, gz = glens ? glens[2] : 1.0;
     k_quantise<<<grid_for(n, B), B, 0, s>>>(d_xyz, n, md, gx, gy, gz, tab, pk.p, ctr.p + 2);
     HIP_OK(hipGetLastError());
-    // every key is at level md and bits 5-32 of the low word are zero by the
-    // key layout: bits 32-127 decide the order (constant bytes are skipped)
-    radix_sort_keys(pk.p, nullptr, n, s, 32, 128);
-    // textbook leaves
-    DevBuf<uint8_t> lev(n - m, s);
-    k_p2o_lev<<<grid_for(n - m, B), B, 0, s>>>(pk.p, n, m, md, lev.p);
-    DevBuf<Key> leafkey(n, s);
-    k_p2o_leaf<<<grid_for(n, B), B, 0, s>>>(pk.p, n, m, lev.p, leafkey.p);
-    lev.release();
-    DevBuf<uint32_t> head(n, s), hpos(n, s);
-    k_head<<<grid_for(n, B), B, 0, s>>>(leafkey.p, n, head.p);
-    scan_exclusive_u32_async(head.p, hpos.p, n, ctr.p, s);
-    // the distinct leaves stay counted on the device: the arrays below are
-    // sized by the capacity n and the kernels read Q
-    DevBuf<uint32_t> NL(n, s);
-    k_compact_idx<<<grid_for(n, B), B, 0, s>>>(head.p, hpos.p, n, NL.p);
-    head.release();
-    hpos.release();
-    DevBuf<uint32_t> cnt(n + 1, s), off(n + 1, s);
-    k_gap_count<<<grid_for(n + 1, B), B, 0, s>>>(leafkey.p, NL.p, ctr.p, n, cnt.p, ctr.p + 3);
-    scan_exclusive_u32_async(cnt.p, off.p, n + 1, ctr.p + 1, s);
-    uint32_t hc[4];
-    HIP_OK(hipMemcpyAsync(hc, ctr.p, sizeof hc, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    if (hc[2]) {
-        dgx_set_error("points2Octree: coordinates must lie in [0,1)");
-        throw DgxError();
+    // Every key is at level md and bits 5-32 of the low word are zero by the
+    // key layout: bits 32-127 decide the order.  The low word holds levels 22
+    // and finer, which only order points that share their level-21 cell.  So
+    // the keys are sorted on the high word alone (8 digits instead of 12) and
+    // the short runs of equal high words on their low words; only a run longer
+    // than kHiRunCap has the keys sorted on bits 32-127 and the steps redone.
+    DevBuf<uint8_t> lev;
+    DevBuf<Key> leafkey;
+    DevBuf<uint32_t> NL, cnt, off;
+    uint32_t hc[5];
+    for (int begin_bit = 64;; begin_bit = 32) {
+        radix_sort_keys(pk.p, nullptr, n, s, begin_bit, 128);
+        if (begin_bit == 64) k_sort_hi_runs<<<grid_for(n, B), B, 0, s>>>(pk.p, n, ctr.p + 4);
+        // textbook leaves
+        lev.alloc(n - m, s);
+        k_p2o_lev<<<grid_for(n - m, B), B, 0, s>>>(pk.p, n, m, md, lev.p);
+        leafkey.alloc(n, s);
+        k_p2o_leaf<<<grid_for(n, B), B, 0, s>>>(pk.p, n, m, lev.p, leafkey.p);
+        lev.release();
+        DevBuf<uint32_t> head(n, s), hpos(n, s);
+        k_head<<<grid_for(n, B), B, 0, s>>>(leafkey.p, n, head.p);
+        scan_exclusive_u32_async(head.p, hpos.p, n, ctr.p, s);
+        // the distinct leaves stay counted on the device: the arrays below are
+        // sized by the capacity n and the kernels read Q
+        NL.alloc(n, s);
+        k_compact_idx<<<grid_for(n, B), B, 0, s>>>(head.p, hpos.p, n, NL.p);
+        head.release();
+        hpos.release();
+        cnt.alloc(n + 1, s);
+        off.alloc(n + 1, s);
+        k_gap_count<<<grid_for(n + 1, B), B, 0, s>>>(leafkey.p, NL.p, ctr.p, n, cnt.p, ctr.p + 3);
+        scan_exclusive_u32_async(cnt.p, off.p, n + 1, ctr.p + 1, s);
+        HIP_OK(hipMemcpyAsync(hc, ctr.p, sizeof hc, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (hc[2]) {
+            dgx_set_error("points2Octree: coordinates must lie in [0,1)");
+            throw DgxError();
+        }
+        if (!hc[4] || begin_bit == 32) break;
+        HIP_OK(hipMemsetAsync(ctr.p + 3, 0, 2 * sizeof(uint32_t), s));
     }
     if (hc[3]) {
         dgx_set_error("points2Octree: internal error, non-empty leaves overlap");
@@ -1073,10 +1150,23 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     W.release();
     prevQ.release();
     const size_t total = used;
-    // distinct leaves are never ancestor and descendant, so a digit at or above
-    // the coarser level tells them apart: the level byte needs no pass
-    radix_sort_keys(leaves.p, nullptr, total, s, 8, 128);
+    // Distinct leaves are never ancestor and descendant, so a digit at or above
+    // the coarser level tells them apart: the level byte needs no pass.  Only
+    // leaves finer than level 21 that share a level-21 cell have equal high
+    // words, so the high-word sort and k_sort_hi_runs order them, and bits
+    // 8-127 are sorted in full only when such a cell holds too many leaves.
+    DevBuf<uint32_t> longrun(1, s);
+    HIP_OK(hipMemsetAsync(longrun.p, 0, sizeof(uint32_t), s));
+    radix_sort_keys(leaves.p, nullptr, total, s, 64, 128);
+    if (total) k_sort_hi_runs<<<grid_for(total, B), B, 0, s>>>(leaves.p, total, longrun.p);
+    HIP_OK(hipGetLastError());
+    uint32_t hlong = 0;
+    HIP_OK(hipMemcpyAsync(&hlong, longrun.p, sizeof hlong, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    if (hlong) {
+        radix_sort_keys(leaves.p, nullptr, total, s, 8, 128);
+        HIP_OK(hipStreamSynchronize(s));
+    }
     out = std::move(leaves);
     out.n = total;
     st.leaves = total;

@@ -171,6 +171,30 @@ def test_lognormal_corner_cluster(dgx, oracle, stencils):
     print("stats", ou.stats(), flush=True)
 
 
+@pytest.mark.parametrize("layout,curve,m", [("long", 0, 1), ("long", 1, 1), ("long", 0, 3), ("short", 0, 1),
+                                             ("short", 1, 2)])
+def test_points_sharing_level21_cells(dgx, oracle, layout, curve, m):
+    """Points that share level-21 cells (2^-21 wide), in shuffled order: their
+    keys agree in the high word and differ only in levels 22-30.  "short": 400
+    cells with 4 points each, which points2Octree orders run by run after the
+    high-word sort; "long": 3000 points in a 3e-7 cube, runs too long for that,
+    so the keys are sorted again on the full key."""
+    rng = np.random.default_rng(21)
+    uni = rng.random((3000, 3))
+    if layout == "long":
+        c = rng.random(3) * 0.9
+        tight = c + rng.random((3000, 3)) * 3e-7
+    else:
+        corner = np.floor(rng.random((400, 3)) * 2.0 ** 21) / 2.0 ** 21
+        tight = np.repeat(corner, 4, axis=0) + rng.random((1600, 3)) * 4e-7
+    dup = np.repeat(tight[:5], 3, axis=0)
+    pts = np.vstack([uni, tight, dup])
+    pts = pts[rng.permutation(len(pts))]
+    cells = np.floor(pts * 2.0 ** 21)
+    assert len(np.unique(cells, axis=0)) < len(pts) - 15
+    check_pipeline(dgx, oracle, pts, 30, m, curve=curve)
+
+
 def test_sample_sort(dgx, oracle):
     pts = dgx.gen_points(0, 5, 4000)
     t = oracle.points2octree(pts, 30, 1)
