@@ -255,6 +255,10 @@ struct dgx_server_s {
         so.fp32_vcycle = o.fp32_vcycle;
         so.rtol = o.rtol;
         so.maxit = o.maxit;
+        // A new hierarchy is solved once: capturing and instantiating its CG
+        // graphs costs more than their replay saves over a few iterations.
+        // A resident one is solved again, and its graphs are captured then.
+        so.use_graph = res ? 1 : 0;
         SolveStats st;
         mg_solve(*mg, bvec.p, x.p, so, &st, s);  // allocates the hierarchy's work vectors once
         d.r.iterations = st.iterations;
