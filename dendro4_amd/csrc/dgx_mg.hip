@@ -450,51 +450,70 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
 constexpr uint32_t kGJ = 32;
 
 // Dinv = inverse of the kGJ x kGJ block at (k0, k0) (the part inside n), by
-// Gauss-Jordan in LDS; rows/columns past n are an identity.  *bad is set for
-// a pivot that is not positive and finite.
-__global__ void __launch_bounds__(kGJ * kGJ) k_gj_diag(const double *A, uint32_t n, uint32_t k0, double *dinv,
-                                                      uint32_t *bad) {
+// Gauss-Jordan in LDS; rows/columns past n are an identity.  Thread t owns
+// column t % kGJ of rows t / kGJ + kGJR u.  *bad is set for a pivot that is
+// not positive and finite.
+constexpr uint32_t kGJD = 1024, kGJR = kGJD / kGJ, kGJU = kGJ / kGJR;
+__global__ void __launch_bounds__(kGJD) k_gj_diag(const double *A, uint32_t n, uint32_t k0, double *dinv,
+                                                 uint32_t *bad) {
     __shared__ double D[kGJ][kGJ + 1];
-    const uint32_t r = threadIdx.x / kGJ, c = threadIdx.x % kGJ;
-    const bool in = k0 + r < n && k0 + c < n;
-    D[r][c] = in ? A[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    const uint32_t c = threadIdx.x % kGJ, r0 = threadIdx.x / kGJ;
+#pragma unroll
+    for (uint32_t u = 0; u < kGJU; ++u) {
+        const uint32_t r = r0 + kGJR * u;
+        const bool in = k0 + r < n && k0 + c < n;
+        D[r][c] = in ? A[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    }
     for (uint32_t p = 0; p < kGJ; ++p) {
         __syncthreads();
-        const double piv = D[p][p], f = D[r][p], dpc = D[p][c];
+        const double piv = D[p][p], dpc = D[p][c];
+        double f[kGJU];
+#pragma unroll
+        for (uint32_t u = 0; u < kGJU; ++u) f[u] = D[r0 + kGJR * u][p];
         __syncthreads();
         if (threadIdx.x == 0 && !(piv > 0.0 && piv < INFINITY)) atomicOr(bad, 1u);
-        double v;
-        if (r == p) v = (c == p) ? 1.0 / piv : dpc / piv;
-        else v = (c == p) ? -f / piv : D[r][c] - f * (dpc / piv);
-        D[r][c] = v;
+#pragma unroll
+        for (uint32_t u = 0; u < kGJU; ++u) {
+            const uint32_t r = r0 + kGJR * u;
+            double v;
+            if (r == p) v = (c == p) ? 1.0 / piv : dpc / piv;
+            else v = (c == p) ? -f[u] / piv : D[r][c] - f[u] * (dpc / piv);
+            D[r][c] = v;
+        }
     }
     __syncthreads();
-    dinv[r * kGJ + c] = D[r][c];
+#pragma unroll
+    for (uint32_t u = 0; u < kGJU; ++u) {
+        const uint32_t r = r0 + kGJR * u;
+        dinv[r * kGJ + c] = D[r][c];
+    }
 }
 // T[q][j] = sum_c Dinv[q][c] A[k0 + c][j] for every column j, and
 // C[i][c] = A[i][k0 + c] for every row i (the old panels, read before the
-// update kernel overwrites them).
+// update kernel overwrites them).  A workgroup takes kGPC columns (and the
+// same range of rows for C): it stages the panel rows of its columns in LDS,
+// and thread group g computes rows q = g, g + kGPG, ... of T.
+constexpr uint32_t kGPC = 32, kGPG = B / kGPC;
 __global__ void __launch_bounds__(B) k_gj_panels(const double *A, uint32_t n, uint32_t k0, uint32_t kb,
                                                  const double *dinv, double *T, double *C) {
     __shared__ double Ds[kGJ * kGJ];
+    __shared__ double As[kGJ][kGPC + 1];
     for (uint32_t t = threadIdx.x; t < kGJ * kGJ; t += B) Ds[t] = dinv[t];
+    const uint32_t jj = threadIdx.x % kGPC, g = threadIdx.x / kGPC;
+    const uint32_t j = blockIdx.x * kGPC + jj;
+    for (uint32_t c = g; c < kGJ; c += kGPG) As[c][jj] = (c < kb && j < n) ? A[(size_t)(k0 + c) * n + j] : 0.0;
     __syncthreads();
-    const uint32_t j = blockIdx.x * B + threadIdx.x;
-    if (j >= n) return;
-    double a[kGJ];
-#pragma unroll
-    for (uint32_t c = 0; c < kGJ; ++c) a[c] = c < kb ? A[(size_t)(k0 + c) * n + j] : 0.0;
-#pragma unroll 4
-    for (uint32_t q = 0; q < kb; ++q) {
-        double t = 0.0;
-#pragma unroll
-        for (uint32_t c = 0; c < kGJ; ++c) t += Ds[q * kGJ + c] * a[c];
-        T[(size_t)q * n + j] = t;
+    if (j < n)
+        for (uint32_t q = g; q < kb; q += kGPG) {
+            double t = 0.0;
+#pragma unroll 8
+            for (uint32_t c = 0; c < kGJ; ++c) t += Ds[q * kGJ + c] * As[c][jj];
+            T[(size_t)q * n + j] = t;
+        }
+    for (uint32_t t = threadIdx.x; t < kGPC * kGJ; t += B) {
+        const uint32_t i = blockIdx.x * kGPC + t / kGJ, c = t % kGJ;
+        if (i < n && c < kb) C[(size_t)i * kGJ + c] = A[(size_t)i * n + k0 + c];
     }
-    const uint32_t i = j;  // one thread per row as well
-#pragma unroll
-    for (uint32_t c = 0; c < kGJ; ++c)
-        if (c < kb) C[(size_t)i * kGJ + c] = A[(size_t)i * n + k0 + c];
 }
 // The rank-kb update and the block's new rows, columns and pivot block.  A
 // workgroup owns a 64 x 64 tile; each thread 4 x 4 entries.
@@ -503,6 +522,7 @@ __global__ void __launch_bounds__(B) k_gj_update(double *A, uint32_t n, uint32_t
                                                  const double *T, const double *C) {
     __shared__ double Cs[kGJT][kGJ + 1];
     __shared__ double Ts[kGJ][kGJT + 1];
+    __shared__ double Ds[kGJ][kGJ + 1];
     const uint32_t i0 = blockIdx.y * kGJT, j0 = blockIdx.x * kGJT;
     for (uint32_t t = threadIdx.x; t < kGJT * kGJ; t += B) {
         const uint32_t rr = t / kGJ, cc = t % kGJ, i = i0 + rr;
@@ -510,6 +530,10 @@ __global__ void __launch_bounds__(B) k_gj_update(double *A, uint32_t n, uint32_t
         const uint32_t q = t / kGJT, jj = t % kGJT, j = j0 + jj;
         Ts[q][jj] = (j < n && q < kb) ? T[(size_t)q * n + j] : 0.0;
     }
+    // tiles that hold columns of the block also need Dinv (for A[i,K])
+    const bool holds_k = j0 < k0 + kb && k0 < j0 + kGJT;
+    if (holds_k)
+        for (uint32_t t = threadIdx.x; t < kGJ * kGJ; t += B) Ds[t / kGJ][t % kGJ] = dinv[t];
     __syncthreads();
     const uint32_t tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
     double acc[4][4] = {};
@@ -543,7 +567,7 @@ __global__ void __launch_bounds__(B) k_gj_update(double *A, uint32_t n, uint32_t
                 *a = Ts[i - k0][tc + v];
             } else if (jK) {
                 double s = 0.0;
-                for (uint32_t c = 0; c < kb; ++c) s += Cs[tr + u][c] * dinv[c * kGJ + (j - k0)];
+                for (uint32_t c = 0; c < kb; ++c) s += Cs[tr + u][c] * Ds[c][j - k0];
                 *a = -s;
             } else {
                 *a -= acc[u][v];
@@ -570,8 +594,8 @@ static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     const dim3 tiles(grid_for(n, kGJT), grid_for(n, kGJT));
     for (uint32_t k0 = 0; k0 < n; k0 += kGJ) {
         const uint32_t kb = std::min(kGJ, n - k0);
-        k_gj_diag<<<1, kGJ * kGJ, 0, s>>>(mg.cinv.p, n, k0, dinv.p, bad.p);
-        k_gj_panels<<<grid_for(n, B), B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
+        k_gj_diag<<<1, kGJD, 0, s>>>(mg.cinv.p, n, k0, dinv.p, bad.p);
+        k_gj_panels<<<grid_for(n, kGPC), B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
         k_gj_update<<<tiles, B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
     }
     HIP_OK(hipGetLastError());
