@@ -500,7 +500,7 @@ __global__ void k_mark_split_children(const Key *Qc, size_t nc, const Key *Qp, c
 // Single-pass forms of the balance level's two compactions (flag, scan,
 // write at offset): tiles of FS_TILE entries, FS_ITEMS consecutive entries
 // per thread, tile ids from ctr[epoch], offsets by decoupled look-back.
-constexpr uint32_t FS_B = 256, FS_ITEMS = 8, FS_TILE = FS_B * FS_ITEMS;
+constexpr uint32_t FS_B = 256, FS_ITEMS = 16, FS_TILE = FS_B * FS_ITEMS;
 
 // Q = the distinct keys of the sorted W[0..n) in order, *d_nq their count;
 // also clears cmask[0..*d_nq) and *zcount for the kernels that follow.
@@ -547,10 +547,10 @@ __global__ void __launch_bounds__(FS_B) k_unique_compact(const Key *W, size_t n,
 
 // The leaves of one level: the children of the split octants Q[0..*d_nq) that
 // are not split themselves (cmask), in Q order and child order, written to
-// out from offset 0; *d_nlv their count.  The same tiles as k_unique_compact
-// on 1024 threads, 2 octants each: emitting an octant's leaves is a chain of
-// table reads, so fewer octants per thread keep the chains short.
-constexpr uint32_t LE_B = 1024, LE_ITEMS = FS_TILE / LE_B;
+// out from offset 0; *d_nlv their count.  Tiles of 2048 octants on 1024
+// threads, 2 octants each: emitting an octant's leaves is a chain of table
+// reads, so fewer octants per thread keep the chains short.
+constexpr uint32_t LE_B = 1024, LE_ITEMS = 2, LE_TILE = LE_B * LE_ITEMS;
 __global__ void __launch_bounds__(LE_B) k_leaf_emit_scan(const Key *Q, const uint32_t *d_nq, const uint32_t *cmask,
                                                          uint32_t md, SfcPacked tab, Key *out, uint32_t *d_nlv,
                                                          uint64_t *status, uint32_t *ctr, uint32_t epoch,
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(LE_B) k_leaf_emit_scan(const Key *Q, const uin
     if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[epoch], 1u);
     __syncthreads();
     const uint32_t tile = s_tile, nq = *d_nq;
-    const size_t i0 = (size_t)tile * FS_TILE + (size_t)threadIdx.x * LE_ITEMS;
+    const size_t i0 = (size_t)tile * LE_TILE + (size_t)threadIdx.x * LE_ITEMS;
     uint32_t msk[LE_ITEMS], cnt = 0;
 #pragma unroll
     for (uint32_t e = 0; e < LE_ITEMS; ++e) {
@@ -1103,8 +1103,9 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
         const int begin_bit = lv <= 21 ? 127 - 3 * (int)lv : 63 - 3 * ((int)lv - 21);
         radix_sort_keys(W.p, nullptr, nw, s, begin_bit, 127);
         const unsigned fs_grid = (unsigned)((nw + FS_TILE - 1) / FS_TILE);
-        if (fs_grid > lb_status.n) {  // look-back words for the largest level so far, zeroed
-            lb_status.alloc(std::max<size_t>(fs_grid, 2 * lb_status.n), s);
+        const unsigned le_grid = (unsigned)((nw + LE_TILE - 1) / LE_TILE);  // at least fs_grid
+        if (le_grid > lb_status.n) {  // look-back words for the largest level so far, zeroed
+            lb_status.alloc(std::max<size_t>(le_grid, 2 * lb_status.n), s);
             HIP_OK(hipMemsetAsync(lb_status.p, 0, lb_status.n * sizeof(uint64_t), s));
         }
         if (lb_epoch + 2 >= lb_ctr.n) {  // a zeroed tile counter per call
@@ -1123,7 +1124,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
             copy_keys(g.p, leaves.p, used, s);
             leaves = std::move(g);
         }
-        k_leaf_emit_scan<<<fs_grid, LE_B, 0, s>>>(Q.p, d_nq, cmask.p, md, tab, leaves.p + used, d_nlv, lb_status.p,
+        k_leaf_emit_scan<<<le_grid, LE_B, 0, s>>>(Q.p, d_nq, cmask.p, md, tab, leaves.p + used, d_nlv, lb_status.p,
                                                   lb_ctr.p, ++lb_epoch, fault);
         HIP_OK(hipGetLastError());
         // neighbours of the split octants seed the next coarser level
