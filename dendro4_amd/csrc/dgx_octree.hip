@@ -708,10 +708,15 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
     __shared__ uint64_t sh[kFuseCap];  // the high words, when the low word is the level alone
     __shared__ uint32_t cm[kFuseCap];
     __shared__ uint32_t tmp[kFuseB / 64 + 1];
+    // while the run stays in this kernel, the next level's seeds (wl) and this
+    // level's split octants (pq) stay in LDS; global memory gets them only
+    // when the host takes over
+    __shared__ Key wl[kFuseCap];
+    __shared__ Key pq[kFuseCap];
     const uint32_t t = threadIdx.x;
     uint32_t l = l0, nw = nw0, nprev = nprev0, used = used0, levels = 0, seeds = 0, k = 0;
     const Key *W = win, *PQ = prevq;
-    bool from_inpar = false;
+    bool from_inpar = false, w_lds = false, pq_lds = false;
     for (;;) {
         // seeds of level l (1 <= nw <= kFuseCap): sort, then unique in place -> Q = sk[0..nq)
         uint32_t P = 1;
@@ -721,7 +726,7 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         const uint64_t lo_lv = l - 1;
         bool narrow = true;
         for (uint32_t i = t; i < P; i += kFuseB) {
-            const Key w = i < nw ? W[i] : Key{~0ull, lo_lv};
+            const Key w = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, lo_lv};
             sh[i] = w.hi;
             narrow = narrow && w.lo == lo_lv;
         }
@@ -730,7 +735,7 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
             fuse_bitonic(sh, P);
             for (uint32_t i = t; i < nw; i += kFuseB) sk[i] = Key{sh[i], lo_lv};
         } else {
-            for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? W[i] : Key{~0ull, ~0ull};
+            for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, ~0ull};
             fuse_bitonic(sk, P);
         }
         __syncthreads();
@@ -751,15 +756,12 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         for (uint32_t e = 0; e < kFuseF; ++e)
             if (fq[e]) sk[qpos++] = kq[e];
         __syncthreads();
-        Key *Qg = (k & 1) ? qbuf1 : qbuf0;  // this level's split octants, the next level's prevQ
-        for (uint32_t i = t; i < nq; i += kFuseB) {
-            Qg[i] = sk[i];
-            cm[i] = 0;
-        }
+        Key *Qg = (k & 1) ? qbuf1 : qbuf0;  // this level's split octants, for the host
+        for (uint32_t i = t; i < nq; i += kFuseB) cm[i] = 0;
         __syncthreads();
         // children of Q that are split octants one level finer
         for (uint32_t i = t; i < nprev; i += kFuseB) {
-            const Key c = PQ[i], par = key_parent(c);
+            const Key c = pq_lds ? pq[i] : PQ[i], par = key_parent(c);
             const size_t j = lower_bound_lt(sk, 0, nq, par);
             if (j < nq && key_eq(sk[j], par)) atomicOr(&cm[j], 1u << key_digit(c, key_level(c)));
         }
@@ -805,18 +807,32 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         uint32_t nnb;
         uint32_t npos = nin + fuse_block_scan(vsum, tmp, nnb);
         const uint32_t nwn = nin + nnb;  // <= largest level input count + 8 kFuseCap = the buffers' capacity
-        for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
+        // level l-1 runs here too when it has seeds and they fit (a level
+        // without seeds starts from input parents, below, and needs no Q)
+        const bool stay = nwn > 0 && nwn <= kFuseCap;
+        if (stay) {
+            for (uint32_t i = t; i < nin; i += kFuseB) wl[i] = src[i];
+            for (uint32_t i = t; i < nq; i += kFuseB) pq[i] = sk[i];
+        } else {
+            for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
+            for (uint32_t i = t; i < nq; i += kFuseB) Qg[i] = sk[i];
+        }
         for (uint32_t e = 0; e < kFuseF; ++e)
             for (int b = 0; b < 8; ++b)
-                if ((valid[e] >> b) & 1u)
-                    Wn[npos++] = morton_encode((b & 1) ? pc1[e][0] : pc0[e][0], (b & 2) ? pc1[e][1] : pc0[e][1],
-                                               (b & 4) ? pc1[e][2] : pc0[e][2], lvl - 1, md);
+                if ((valid[e] >> b) & 1u) {
+                    const Key v = morton_encode((b & 1) ? pc1[e][0] : pc0[e][0], (b & 2) ? pc1[e][1] : pc0[e][1],
+                                                (b & 4) ? pc1[e][2] : pc0[e][2], lvl - 1, md);
+                    if (stay) wl[npos++] = v;
+                    else Wn[npos++] = v;
+                }
         __syncthreads();
         PQ = Qg;
+        pq_lds = stay;
         nprev = nq;
         l -= 1;
         nw = nwn;
         W = Wn;
+        w_lds = stay;
         from_inpar = false;
         // levels without seeds: the next level starts from its input parents
         // only, with no split octants below it
@@ -825,6 +841,7 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
             if (l >= 2) {
                 nw = loff[l] - loff[l - 1];
                 W = inpar + loff[l - 1];
+                w_lds = false;
                 from_inpar = true;
             }
             l -= 1;
