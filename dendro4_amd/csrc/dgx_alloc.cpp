@@ -15,9 +15,11 @@
 // frees a block, so a caller can measure what one piece of work holds.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "dgx_common.h"
 
@@ -213,4 +215,46 @@ int64_t dgx_alloc_tag_peak(uint64_t tag) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_tags.find(tag);
     return it == g_tags.end() ? 0 : it->second.peak;
+}
+
+// ------------------------------------------------------------- read-back ----
+namespace {
+constexpr size_t kReadBackBytes = 4096;
+std::mutex g_rb_mu;
+std::vector<void *> g_rb_free;  // pinned buffers of threads that have ended
+struct ReadBackBuf {
+    void *p = nullptr;
+    ~ReadBackBuf() {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(g_rb_mu);
+        g_rb_free.push_back(p);
+    }
+};
+void *read_back_buf() {
+    thread_local ReadBackBuf b;
+    if (!b.p) {
+        {
+            std::lock_guard<std::mutex> lk(g_rb_mu);
+            if (!g_rb_free.empty()) {
+                b.p = g_rb_free.back();
+                g_rb_free.pop_back();
+            }
+        }
+        if (!b.p) HIP_OK(hipHostMalloc(&b.p, kReadBackBytes, hipHostMallocDefault));
+    }
+    return b.p;
+}
+}  // namespace
+
+void dgx_read_back(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    if (bytes > kReadBackBytes) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return;
+    }
+    void *buf = read_back_buf();
+    HIP_OK(hipMemcpyAsync(buf, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::memcpy(dst, buf, bytes);
 }

@@ -413,6 +413,10 @@ void radix_sort_keys(Key *keys, uint32_t *vals, size_t n, hipStream_t s, int beg
 void radix_sort_pairs(Key *keys, uint32_t *vals, size_t n, hipStream_t s, uint64_t lo_max, uint64_t hi_max);
 // The pinned host word the look-back kernels set on a wait past its bound.
 uint32_t *dgx_fault_word();
+// Copies bytes from device memory to host memory on s and waits for it.  Up
+// to 4 KiB go through a pinned buffer of the calling thread (kept for later
+// threads when it exits), so the copy needs no staging.
+void dgx_read_back(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Throws DgxError if a sort kernel reported a look-back wait that exceeded its
 // bound since the last check (the flag is cleared).
 void sort_fault_check();

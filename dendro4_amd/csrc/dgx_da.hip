@@ -344,8 +344,7 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
                                                 da->bdy_node.p, bcount.p);
     HIP_OK(hipMemcpyAsync(&da->n_bdy_nodes, bcount.p, 4, hipMemcpyDeviceToHost, s));
     uint32_t hc[4];
-    HIP_OK(hipMemcpyAsync(hc, ctr.p, 16, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(hc, ctr.p, 16, s);
     da->n_unresolved = hc[0];
     da->n_ghost_scans = hc[1];
     da->node_size = hc[2];
@@ -838,8 +837,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s, bool colour) {
         scan_exclusive_u32_async(shf.p, sidx.p, nn, tot.p, s);
         scan_exclusive_u32_async(cnt2.p, base.p, nn, tot.p + 1, s);
         uint32_t ht[2];
-        HIP_OK(hipMemcpyAsync(ht, tot.p, sizeof ht, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        dgx_read_back(ht, tot.p, sizeof ht, s);
         const uint64_t nsn = ht[0];
         ns = ht[1];
         if (ns) {
@@ -863,8 +861,7 @@ void da_build_blocks(DaDevice &da, hipStream_t s, bool colour) {
     HIP_OK(hipMemsetAsync(mx.p, 0, 4, s));
     k_blk_maxnodes<<<grid_for(nblk, B), B, 0, s>>>(da.bnode_off.p, nblk, mx.p);
     uint32_t hmx = 0;
-    HIP_OK(hipMemcpyAsync(&hmx, mx.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&hmx, mx.p, 4, s);
     if ((uint64_t)hmx * 8 > 65535) {  // lconn holds 16-bit byte offsets of 8-byte slots
         dgx_set_error("DA: an element block touches %u nodes, more than the 8191 its records can address", hmx);
         throw DgxError();

@@ -407,8 +407,7 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
     k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, keys.p, vals.p, cnt.p);
     uint32_t hcnt = 0, herr = 0;
     HIP_OK(hipMemcpyAsync(&hcnt, cnt.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&herr, err.p, 4, s);
     if (herr) {
         dgx_set_error("multigrid: interpolation lookup failed (code %u)", herr);
         throw DgxError();
@@ -600,8 +599,7 @@ static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
     }
     HIP_OK(hipGetLastError());
     uint32_t hbad = 0;
-    HIP_OK(hipMemcpyAsync(&hbad, bad.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&hbad, bad.p, 4, s);
     if (hbad) {
         dgx_set_error("multigrid: coarse matrix is not positive definite (a Gauss-Jordan pivot is not positive)");
         throw DgxError();
@@ -1028,8 +1026,7 @@ void run_graph(Graph &g, const SolveOpts &o, bool use_graph, hipStream_t s, F bo
 
 double read_scalar(const double *p, hipStream_t s) {
     double h;
-    HIP_OK(hipMemcpyAsync(&h, p, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&h, p, sizeof(double), s);
     return h;
 }
 

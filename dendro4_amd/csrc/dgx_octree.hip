@@ -921,8 +921,7 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
         off.alloc(n + 1, s);
         k_gap_count<<<grid_for(n + 1, B), B, 0, s>>>(leafkey.p, NL.p, ctr.p, n, cnt.p, ctr.p + 3);
         scan_exclusive_u32_async(cnt.p, off.p, n + 1, ctr.p + 1, s);
-        HIP_OK(hipMemcpyAsync(hc, ctr.p, sizeof hc, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        dgx_read_back(hc, ctr.p, sizeof hc, s);
         if (hc[2]) {
             dgx_set_error("points2Octree: coordinates must lie in [0,1)");
             throw DgxError();
@@ -972,8 +971,7 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
                                      (uint32_t)(cap - rused), dres.p);
         HIP_OK(hipGetLastError());
         ReplayResult hr;
-        HIP_OK(hipMemcpyAsync(&hr, dres.p, sizeof(hr), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        dgx_read_back(&hr, dres.p, sizeof(hr), s);
         if (hr.err) {
             dgx_set_error("points2Octree: cursor replay failed (code %u) after leaf %u", hr.err, k);
             throw DgxError();
@@ -1017,8 +1015,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     HIP_OK(hipMemsetAsync(lh.p, 0, 128, s));
     k_level_hist<<<grid_for(n, B * LH_ITEMS), B, 0, s>>>(in, n, lh.p);
     uint32_t hl[32];
-    HIP_OK(hipMemcpyAsync(hl, lh.p, 128, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(hl, lh.p, 128, s);
     uint32_t lmin = 31, lmax = 0;
     for (uint32_t l = 0; l < 32; ++l)
         if (hl[l]) {
@@ -1100,8 +1097,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
                                              tab, leaves.p, (uint32_t)used, fw0.p, fw1.p, fq0.p, fq1.p, fst.p);
             HIP_OK(hipGetLastError());
             FuseState h;
-            HIP_OK(hipMemcpyAsync(&h, fst.p, sizeof h, hipMemcpyDeviceToHost, s));
-            HIP_OK(hipStreamSynchronize(s));
+            dgx_read_back(&h, fst.p, sizeof h, s);
             used = h.used;
             st.levels += h.levels;
             st.seeds += h.seeds;
@@ -1154,8 +1150,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
             HIP_OK(hipGetLastError());
         }
         uint32_t hc[3];
-        HIP_OK(hipMemcpyAsync(hc, cnt.p, 12, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        dgx_read_back(hc, cnt.p, 12, s);
         used += hc[1];
         st.levels++;
         st.seeds += nw;
@@ -1179,8 +1174,7 @@ DevBuf<Key> balance_build(const Key *in, size_t n, uint32_t md, int curve, hipSt
     if (total) k_sort_hi_runs<<<grid_for(total, B), B, 0, s>>>(leaves.p, total, longrun.p);
     HIP_OK(hipGetLastError());
     uint32_t hlong = 0;
-    HIP_OK(hipMemcpyAsync(&hlong, longrun.p, sizeof hlong, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&hlong, longrun.p, sizeof hlong, s);
     if (hlong) {
         radix_sort_keys(leaves.p, nullptr, total, s, 8, 128);
         HIP_OK(hipStreamSynchronize(s));

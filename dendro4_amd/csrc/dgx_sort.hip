@@ -443,8 +443,7 @@ uint64_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, size_t n, hipStre
     DevBuf<uint32_t> tot(1, s);
     scan_exclusive_u32_async(in, out, n, tot.p, s);
     uint32_t h = 0;
-    HIP_OK(hipMemcpyAsync(&h, tot.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&h, tot.p, sizeof(uint32_t), s);
     return h;
 }
 
