@@ -342,7 +342,7 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     HIP_OK(hipMemsetAsync(bcount.p, 0, 4, s));
     k_da_bdy_nodes<<<grid_for(nn, B), B, 0, s>>>(da->n2l.p, (uint32_t)nn, da->x.p, da->y.p, da->z.p, md,
                                                 da->bdy_node.p, bcount.p);
-    HIP_OK(hipMemcpyAsync(&da->n_bdy_nodes, bcount.p, 4, hipMemcpyDeviceToHost, s));
+    dgx_read_back(&da->n_bdy_nodes, bcount.p, 4, s);
     uint32_t hc[4];
     dgx_read_back(hc, ctr.p, 16, s);
     da->n_unresolved = hc[0];

@@ -406,7 +406,7 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
     HIP_OK(hipMemsetAsync(cnt.p, 0, 4, s));
     k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, keys.p, vals.p, cnt.p);
     uint32_t hcnt = 0, herr = 0;
-    HIP_OK(hipMemcpyAsync(&hcnt, cnt.p, 4, hipMemcpyDeviceToHost, s));
+    dgx_read_back(&hcnt, cnt.p, 4, s);
     dgx_read_back(&herr, err.p, 4, s);
     if (herr) {
         dgx_set_error("multigrid: interpolation lookup failed (code %u)", herr);

@@ -956,8 +956,7 @@ DevBuf<Key> p2o_build(const double *d_xyz, size_t n, uint32_t md, uint32_t m, in
     DevBuf<uint32_t> trig(ntrig, s);
     k_compact_idx<<<grid_for(nT, B), B, 0, s>>>(flag.p, fpos.p, nT, trig.p);
     std::vector<uint32_t> htrig(ntrig);
-    HIP_OK(hipMemcpyAsync(htrig.data(), trig.p, ntrig * 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(htrig.data(), trig.p, ntrig * 4, s);
     struct Seg { int src; size_t begin, count; };  // src 0: T, 1: replay buffer
     std::vector<Seg> segs;
     const uint32_t cap = 1u << 20;
@@ -1199,8 +1198,7 @@ DevBuf<Key> import_treenodes(const uint32_t *d_tn, size_t n, uint32_t md, int cu
     if (n) k_import_tn<<<grid_for(n, B), B, 0, s>>>(d_tn, n, md, make_packed(curve), k.p, bad.p);
     HIP_OK(hipGetLastError());
     uint32_t hb = 0;
-    HIP_OK(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    dgx_read_back(&hb, bad.p, 4, s);
     if (hb) {
         dgx_set_error("TreeNode array: anchor not aligned to its level or outside the domain");
         throw DgxError();
