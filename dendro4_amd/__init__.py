@@ -37,7 +37,8 @@ EXPORTED = [
     "dgx_sfc_tables", "dgx_da_export_anchors", "dgx_mg_set_operator", "dgx_server_create", "dgx_server_submit",
     "dgx_server_wait", "dgx_server_clock", "dgx_server_free", "dgx_server_peek", "dgx_read_pts", "dgx_write_pts",
     "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
-    "dgx_write_nodes_binary", "dgx_write_vtk",
+    "dgx_write_nodes_binary", "dgx_write_vtk", "dgx_server_get_stats", "dgx_server_inject_fault",
+    "dgx_mg_coarse_info", "dgx_direct_plan_host", "dgx_direct_plan_export", "dgx_direct_plan_free",
 ]
 
 
@@ -80,6 +81,20 @@ class Served(C.Structure):
                 ("s_hierarchy", C.c_double), ("s_solve", C.c_double)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("budget", "reserved", "resident", "resident_bytes", "queued", "running",
+                                          "evictions")]
+
+
+SERVER_FAULT_RESOLVE = 1
+
+
+class DirectSizes(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("interior", "fronts", "links", "levels", "entries", "idx_len", "inv_len",
+                                          "fmap_len", "a_nnz", "a_src_len", "upd_len")] + \
+               [("max_m", C.c_uint32), ("max_s", C.c_uint32)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double), ("seconds", C.c_double)]
 
@@ -115,6 +130,12 @@ def load_library():
         "dgx_server_clock": ([vp], C.c_double),
         "dgx_server_peek": ([vp, C.c_uint64, C.POINTER(Served)], i32),
         "dgx_server_free": ([vp], i32),
+        "dgx_server_get_stats": ([vp, C.POINTER(ServerStats)], i32),
+        "dgx_mg_coarse_info": ([vp, vp], i32),
+        "dgx_direct_plan_host": ([u32, u64, vp, vp, vp, vp, u32, u32, C.POINTER(vp), C.POINTER(DirectSizes)], i32),
+        "dgx_direct_plan_export": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+        "dgx_direct_plan_free": ([vp], i32),
+        "dgx_server_inject_fault": ([vp, i32], i32),
         "dgx_balance_octree": ([vp, i32, C.POINTER(vp)], i32),
         "dgx_octree_import": ([vp, sz, u32, i32, i32, i32, C.POINTER(vp)], i32),
         "dgx_octree_size": ([vp, C.POINTER(sz)], i32),
@@ -408,6 +429,10 @@ class MG:
         _check(_lib.dgx_mg_finest_da(self._h, C.byref(h)))
         self.da = DA._borrow(h, self)
         self.n_nodes = self.level_nodes[0]
+        ci = (C.c_uint64 * 6)()
+        _check(_lib.dgx_mg_coarse_info(self._h, ci))
+        self.coarse = dict(zip(("interior", "fronts", "tree_levels", "max_front", "max_separator", "front_entries"),
+                               list(ci)))
 
     def rhs(self, f, b):
         _check(_lib.dgx_mg_rhs(self._h, f.ptr, b.ptr))
@@ -447,6 +472,36 @@ def sfc_tables(curve=CURVE_HILBERT):
     c = np.zeros(192, np.uint8)
     L.dgx_sfc_tables(curve, _ptr(r), _ptr(m), _ptr(c))
     return r.reshape(24, 8), m.reshape(24, 8), c.reshape(24, 8)
+
+
+def direct_plan(conn, meta, bdy, xyz, leaf=0, dense=2048):
+    """Symbolic analysis of the coarsest-level direct solver on host arrays
+    (dgx_direct_plan_host; no device): conn (E, 8) node ids, meta (E,) as the
+    DA's element meta (mask | child << 8 | level << 11), bdy (n,) boundary
+    flags, xyz (n, 3) node coordinates.  Returns the plan's arrays."""
+    L = load_library()
+    conn = np.ascontiguousarray(np.asarray(conn, np.uint32).T)  # [8][E]
+    meta = np.ascontiguousarray(meta, np.uint16)
+    bdy = np.ascontiguousarray(bdy, np.uint8)
+    xyz = np.ascontiguousarray(xyz, np.uint32)
+    n, E = len(bdy), conn.shape[1]
+    h = C.c_void_p()
+    sz = DirectSizes()
+    _check(L.dgx_direct_plan_host(n, E, _ptr(conn), _ptr(meta), _ptr(bdy), _ptr(xyz), leaf, dense, C.byref(h),
+                                  C.byref(sz)))
+    try:
+        out = {k: getattr(sz, k) for k, _ in DirectSizes._fields_}
+        a = {"front9": np.zeros((sz.fronts, 9), np.uint64), "link3": np.zeros((sz.links, 3), np.uint64),
+             "inv": np.zeros(sz.inv_len, np.int32), "fmap": np.zeros(sz.fmap_len, np.uint32),
+             "idx": np.zeros(sz.idx_len, np.uint32), "a_off": np.zeros(sz.a_nnz + 1, np.uint32),
+             "a_src": np.zeros(sz.a_src_len, np.uint32), "a_dst": np.zeros(sz.a_nnz, np.uint64),
+             "inode": np.zeros(sz.interior, np.uint32)}
+        _check(L.dgx_direct_plan_export(h, *[_ptr(a[k]) for k in ("front9", "link3", "inv", "fmap", "idx", "a_off",
+                                                                    "a_src", "a_dst", "inode")]))
+        out.update(a)
+        return out
+    finally:
+        L.dgx_direct_plan_free(h)
 
 
 # On-disk formats (host only, no init needed) -------------------------------
@@ -586,6 +641,14 @@ class Server:
 
     def clock(self):
         return _lib.dgx_server_clock(self._h)
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.dgx_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def inject_fault(self, kind=SERVER_FAULT_RESOLVE):
+        _check(_lib.dgx_server_inject_fault(self._h, kind))
 
     def close(self):
         if self._h:

@@ -413,6 +413,17 @@ int dgx_mg_info(dgx_mg m, int *nlevels, uint64_t *octants, uint64_t *nodes) {
     }
     return 0;
 }
+int dgx_mg_coarse_info(dgx_mg m, uint64_t out[6]) {
+    if (!m || !out) { dgx_set_error("null argument"); return -1; }
+    const CoarseSolver &c = *m->mg->coarse;
+    out[0] = c.ni;
+    out[1] = c.nfronts;
+    out[2] = c.levels;
+    out[3] = c.plan.max_m;
+    out[4] = c.plan.max_s;
+    out[5] = c.plan.front_entries;
+    return 0;
+}
 int dgx_mg_finest_da(dgx_mg m, dgx_da *da) {
     if (!m || !da) { dgx_set_error("null argument"); return -1; }
     *da = &m->finest;

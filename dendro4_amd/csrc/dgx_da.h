@@ -71,5 +71,7 @@ void da_dirichlet_jacobi(const DaDevice &da, T *x, const T *b, const T *dinv, T 
                          T *scratch, hipStream_t s);
 template <typename T>
 void da_dirichlet_residual(const DaDevice &da, const T *x, const T *b, T *r, T lap_fac, T mass_fac, hipStream_t s);
-// Dense n_nodes x n_nodes matrix of the Dirichlet operator (row-major, FP64).
-void da_assemble_dense_dirichlet(const DaDevice &da, double lap_fac, double mass_fac, double *A, hipStream_t s);
+// Element matrices of lap_fac K + mass_fac M in each element's child frame:
+// vals[e*64 + k*8 + j] is the contribution to A[id[j^c], id[k^c]], id the
+// element's 8 node ids (conn) and c its child number (meta).
+void da_element_matrices(const DaDevice &da, double lap_fac, double mass_fac, double *vals, hipStream_t s);

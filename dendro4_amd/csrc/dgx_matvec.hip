@@ -690,48 +690,31 @@ void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double ma
 }
 
 // ---------------------------------------------------------------------------
-// Dense assembly of the Dirichlet operator (coarsest multigrid level).
+// Element matrices (coarsest-level direct solver).
 // ---------------------------------------------------------------------------
 namespace {
-
-__global__ void __launch_bounds__(B) k_assemble_dense(const uint32_t *__restrict__ conn,
-                                                      const uint16_t *__restrict__ meta, size_t ne,
-                                                      const uint8_t *__restrict__ bdy, double lap_fac,
-                                                      double mass_fac, double *A, uint32_t n) {
+// vals[e*64 + k*8 + j] = entry (row j, column k) of element e's matrix in its
+// child frame: the contribution to A[id[j^c], id[k^c]] (c the child number).
+__global__ void __launch_bounds__(B) k_element_matrices(const uint16_t *__restrict__ meta, size_t ne, double lap_fac,
+                                                        double mass_fac, double *__restrict__ vals) {
     size_t e = (size_t)blockIdx.x * B + threadIdx.x;
     if (e >= ne) return;
     const uint32_t mt = meta[e], c = (mt >> 8) & 7u, mask = mt & 255u;
-    uint32_t cm = 0;  // hanging mask in the child frame
+    uint32_t cm = 0;
     for (uint32_t j = 0; j < 8; ++j)
         if ((mask >> (j ^ c)) & 1u) cm |= 1u << j;
     cm |= (mt >> 11) << 8;
-    uint32_t id[8];
-    for (int j = 0; j < 8; ++j) id[j] = conn[(size_t)j * ne + e];
-    for (int k = 0; k < 8; ++k) {  // column k (child frame) of the element matrix
+    for (int k = 0; k < 8; ++k) {
         double xv[8], r[8];
         for (int j = 0; j < 8; ++j) xv[j] = (j == k) ? 1.0 : 0.0;
         element_child_frame<double>(cm, lap_fac, mass_fac, xv, r);
-        const uint32_t col = id[k ^ c];
-        if (bdy[col]) continue;
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t row = id[j ^ c];
-            if (bdy[row]) continue;
-            atomicAdd(&A[(size_t)row * n + col], r[j]);
-        }
+        for (int j = 0; j < 8; ++j) vals[e * 64 + k * 8 + j] = r[j];
     }
 }
-__global__ void k_dense_bdy_identity(const uint8_t *bdy, uint32_t n, double *A) {
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (i < n && bdy[i]) A[(size_t)i * n + i] = 1.0;
-}
-
 }  // namespace
 
-void da_assemble_dense_dirichlet(const DaDevice &da, double lap_fac, double mass_fac, double *A, hipStream_t s) {
-    const uint32_t n = da.n_nodes;
-    HIP_OK(hipMemsetAsync(A, 0, (size_t)n * n * sizeof(double), s));
-    k_assemble_dense<<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, da.bdy_node.p, lap_fac,
-                                                         mass_fac, A, n);
-    k_dense_bdy_identity<<<grid_for(n, B), B, 0, s>>>(da.bdy_node.p, n, A);
+void da_element_matrices(const DaDevice &da, double lap_fac, double mass_fac, double *vals, hipStream_t s) {
+    if (!da.n_elem) return;
+    k_element_matrices<<<grid_for(da.n_elem, B), B, 0, s>>>(da.meta.p, da.n_elem, lap_fac, mass_fac, vals);
     HIP_OK(hipGetLastError());
 }

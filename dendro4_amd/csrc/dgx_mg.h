@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "dgx_da.h"
+#include "dgx_direct.h"
 
 // ot::coarsenOctree (src/coarsen.cpp:53-104): every complete set of 8 sibling
 // leaves is replaced by its parent, once.
@@ -38,9 +39,8 @@ struct MgHierarchy {
     int curve = 0;
     std::vector<MgLevel> lev;  // lev[0] = finest
     double lap_fac = 1.0, mass_fac = 1.0;
-    // coarsest-level direct solver: dense inverse of the Dirichlet operator (FP64)
-    uint32_t nc = 0;
-    DevBuf<double> cinv;
+    // coarsest-level sparse FP64 direct solver of the Dirichlet operator
+    std::unique_ptr<CoarseSolver> coarse;
     MgWork *work = nullptr;
     ~MgHierarchy();
 };
@@ -51,7 +51,11 @@ void mg_mark_idle(MgHierarchy &mg);
 // Changes the operator factors: recomputes the level diagonals and the
 // coarse inverse, drops captured graphs.
 void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s);
-// With overlap, each level's DA, its diagonal and the coarse inverse are built
+// A coarsest level of more nodes than this makes the hierarchy coarsen past
+// the requested number of levels (the direct solver's fronts would exceed its
+// limits).
+constexpr uint32_t kMaxDirectNodes = 131072;
+// With overlap, each level's DA, its diagonal and the coarse solver are built
 // on a second host thread and stream while the next levels are coarsened
 // (faster when nothing else drives the device; a caller that already runs
 // several set-ups at once passes false).  colour as for da_build.

@@ -257,29 +257,6 @@ __global__ void k_ell_to_csr(const uint32_t *ecol, const double *eval, uint32_t 
         ++o;
     }
 }
-// dense y = A x (n x n row-major), one wave per row
-template <typename T>
-__global__ void k_dense_gemv(const double *A, uint32_t n, const T *x, T *y) {
-    const uint32_t row = blockIdx.x * (B / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (row >= n) return;
-    // eight entries per lane are loaded ahead of their products; each lane
-    // still sums its entries in column order
-    double acc = 0.0;
-    const double *Ar = A + (size_t)row * n;
-    for (uint32_t j0 = lane; j0 < n; j0 += 64 * 8) {
-        double a[8], xv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const uint32_t j = j0 + 64 * u;
-            a[u] = j < n ? Ar[j] : 0.0;
-            xv[u] = j < n ? (double)x[j] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) y[row] = (T)acc;
-}
 __global__ void k_f64_to_f32(const double *a, uint32_t n, float *b) {
     uint32_t i = blockIdx.x * B + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -438,172 +415,12 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
     HIP_OK(hipGetLastError());
 }
 
-// Coarsest level: the dense Dirichlet matrix (SPD) is inverted in place in
-// FP64 by blocked Gauss-Jordan elimination (no pivoting: every pivot block is
-// a Schur complement of an SPD matrix, hence SPD), and the inverse is applied
-// as a GEMV per V-cycle.  The reference factors its coarsest matrix with
-// PETSc LU (blockDiag.cpp:780-1047).  Per block step of kGJ rows/columns K:
-//   Dinv = A[K,K]^-1 (one workgroup, in LDS), T = Dinv A[K,:], C = A[:,K];
-//   A[i,j] -= C[i,:] T[:,j] (i, j outside K), A[K,j] = T[:,j],
-//   A[i,K] = -C[i,:] Dinv, A[K,K] = Dinv.
-constexpr uint32_t kGJ = 32;
-
-// Dinv = inverse of the kGJ x kGJ block at (k0, k0) (the part inside n), by
-// Gauss-Jordan in LDS; rows/columns past n are an identity.  Thread t owns
-// column t % kGJ of rows t / kGJ + kGJR u.  *bad is set for a pivot that is
-// not positive and finite.
-constexpr uint32_t kGJD = 1024, kGJR = kGJD / kGJ, kGJU = kGJ / kGJR;
-__global__ void __launch_bounds__(kGJD) k_gj_diag(const double *A, uint32_t n, uint32_t k0, double *dinv,
-                                                 uint32_t *bad) {
-    __shared__ double D[kGJ][kGJ + 1];
-    const uint32_t c = threadIdx.x % kGJ, r0 = threadIdx.x / kGJ;
-#pragma unroll
-    for (uint32_t u = 0; u < kGJU; ++u) {
-        const uint32_t r = r0 + kGJR * u;
-        const bool in = k0 + r < n && k0 + c < n;
-        D[r][c] = in ? A[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
-    }
-    for (uint32_t p = 0; p < kGJ; ++p) {
-        __syncthreads();
-        const double piv = D[p][p], dpc = D[p][c];
-        double f[kGJU];
-#pragma unroll
-        for (uint32_t u = 0; u < kGJU; ++u) f[u] = D[r0 + kGJR * u][p];
-        __syncthreads();
-        if (threadIdx.x == 0 && !(piv > 0.0 && piv < INFINITY)) atomicOr(bad, 1u);
-#pragma unroll
-        for (uint32_t u = 0; u < kGJU; ++u) {
-            const uint32_t r = r0 + kGJR * u;
-            double v;
-            if (r == p) v = (c == p) ? 1.0 / piv : dpc / piv;
-            else v = (c == p) ? -f[u] / piv : D[r][c] - f[u] * (dpc / piv);
-            D[r][c] = v;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t u = 0; u < kGJU; ++u) {
-        const uint32_t r = r0 + kGJR * u;
-        dinv[r * kGJ + c] = D[r][c];
-    }
-}
-// T[q][j] = sum_c Dinv[q][c] A[k0 + c][j] for every column j, and
-// C[i][c] = A[i][k0 + c] for every row i (the old panels, read before the
-// update kernel overwrites them).  A workgroup takes kGPC columns (and the
-// same range of rows for C): it stages the panel rows of its columns in LDS,
-// and thread group g computes rows q = g, g + kGPG, ... of T.
-constexpr uint32_t kGPC = 32, kGPG = B / kGPC;
-__global__ void __launch_bounds__(B) k_gj_panels(const double *A, uint32_t n, uint32_t k0, uint32_t kb,
-                                                 const double *dinv, double *T, double *C) {
-    __shared__ double Ds[kGJ * kGJ];
-    __shared__ double As[kGJ][kGPC + 1];
-    for (uint32_t t = threadIdx.x; t < kGJ * kGJ; t += B) Ds[t] = dinv[t];
-    const uint32_t jj = threadIdx.x % kGPC, g = threadIdx.x / kGPC;
-    const uint32_t j = blockIdx.x * kGPC + jj;
-    for (uint32_t c = g; c < kGJ; c += kGPG) As[c][jj] = (c < kb && j < n) ? A[(size_t)(k0 + c) * n + j] : 0.0;
-    __syncthreads();
-    if (j < n)
-        for (uint32_t q = g; q < kb; q += kGPG) {
-            double t = 0.0;
-#pragma unroll 8
-            for (uint32_t c = 0; c < kGJ; ++c) t += Ds[q * kGJ + c] * As[c][jj];
-            T[(size_t)q * n + j] = t;
-        }
-    for (uint32_t t = threadIdx.x; t < kGPC * kGJ; t += B) {
-        const uint32_t i = blockIdx.x * kGPC + t / kGJ, c = t % kGJ;
-        if (i < n && c < kb) C[(size_t)i * kGJ + c] = A[(size_t)i * n + k0 + c];
-    }
-}
-// The rank-kb update and the block's new rows, columns and pivot block.  A
-// workgroup owns a 64 x 64 tile; each thread 4 x 4 entries.
-constexpr uint32_t kGJT = 64;
-__global__ void __launch_bounds__(B) k_gj_update(double *A, uint32_t n, uint32_t k0, uint32_t kb, const double *dinv,
-                                                 const double *T, const double *C) {
-    __shared__ double Cs[kGJT][kGJ + 1];
-    __shared__ double Ts[kGJ][kGJT + 1];
-    __shared__ double Ds[kGJ][kGJ + 1];
-    const uint32_t i0 = blockIdx.y * kGJT, j0 = blockIdx.x * kGJT;
-    for (uint32_t t = threadIdx.x; t < kGJT * kGJ; t += B) {
-        const uint32_t rr = t / kGJ, cc = t % kGJ, i = i0 + rr;
-        Cs[rr][cc] = (i < n && cc < kb) ? C[(size_t)i * kGJ + cc] : 0.0;
-        const uint32_t q = t / kGJT, jj = t % kGJT, j = j0 + jj;
-        Ts[q][jj] = (j < n && q < kb) ? T[(size_t)q * n + j] : 0.0;
-    }
-    // tiles that hold columns of the block also need Dinv (for A[i,K])
-    const bool holds_k = j0 < k0 + kb && k0 < j0 + kGJT;
-    if (holds_k)
-        for (uint32_t t = threadIdx.x; t < kGJ * kGJ; t += B) Ds[t / kGJ][t % kGJ] = dinv[t];
-    __syncthreads();
-    const uint32_t tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
-    double acc[4][4] = {};
-#pragma unroll 8
-    for (uint32_t c = 0; c < kGJ; ++c) {
-        double cv[4], tv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            cv[u] = Cs[tr + u][c];
-            tv[u] = Ts[c][tc + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += cv[u] * tv[v];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const uint32_t i = i0 + tr + u;
-        if (i >= n) continue;
-        const bool iK = i >= k0 && i < k0 + kb;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const uint32_t j = j0 + tc + v;
-            if (j >= n) continue;
-            const bool jK = j >= k0 && j < k0 + kb;
-            double *a = &A[(size_t)i * n + j];
-            if (iK && jK) {
-                *a = dinv[(i - k0) * kGJ + (j - k0)];
-            } else if (iK) {
-                *a = Ts[i - k0][tc + v];
-            } else if (jK) {
-                double s = 0.0;
-                for (uint32_t c = 0; c < kb; ++c) s += Cs[tr + u][c] * Ds[c][j - k0];
-                *a = -s;
-            } else {
-                *a -= acc[u][v];
-            }
-        }
-    }
-}
-
-static void build_coarse_inverse(MgHierarchy &mg, hipStream_t s) {
+// Coarsest level: symbolic analysis (nested dissection, host) and numeric
+// factorisation (device) of the sparse FP64 direct solver (dgx_direct.hip).
+static void build_coarse(MgHierarchy &mg, hipStream_t s) {
     const DaDevice &d = *mg.lev.back().da;
-    const uint32_t n = d.n_nodes;
-    const uint32_t kMaxDense = 16384;
-    if (n > kMaxDense) {
-        dgx_set_error("multigrid: coarsest level has %u nodes (> %u for the dense FP64 solve); use more levels", n,
-                      kMaxDense);
-        throw DgxError();
-    }
-    mg.nc = n;
-    mg.cinv.alloc((size_t)n * n, s);
-    da_assemble_dense_dirichlet(d, mg.lap_fac, mg.mass_fac, mg.cinv.p, s);
-    DevBuf<double> dinv(kGJ * kGJ, s), T((size_t)kGJ * n, s), C((size_t)n * kGJ, s);
-    DevBuf<uint32_t> bad(1, s);
-    HIP_OK(hipMemsetAsync(bad.p, 0, 4, s));
-    const dim3 tiles(grid_for(n, kGJT), grid_for(n, kGJT));
-    for (uint32_t k0 = 0; k0 < n; k0 += kGJ) {
-        const uint32_t kb = std::min(kGJ, n - k0);
-        k_gj_diag<<<1, kGJD, 0, s>>>(mg.cinv.p, n, k0, dinv.p, bad.p);
-        k_gj_panels<<<grid_for(n, kGPC), B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
-        k_gj_update<<<tiles, B, 0, s>>>(mg.cinv.p, n, k0, kb, dinv.p, T.p, C.p);
-    }
-    HIP_OK(hipGetLastError());
-    uint32_t hbad = 0;
-    dgx_read_back(&hbad, bad.p, 4, s);
-    if (hbad) {
-        dgx_set_error("multigrid: coarse matrix is not positive definite (a Gauss-Jordan pivot is not positive)");
-        throw DgxError();
-    }
+    mg.coarse = coarse_analyse(d, s);
+    coarse_factor(*mg.coarse, d, mg.lap_fac, mg.mass_fac, s);
 }
 
 namespace {
@@ -763,17 +580,13 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
         });
     };
     size_t last = build_da(&L0);
-    // the coarsest level must fit the dense FP64 coarse solve; its Cholesky and
-    // inverse cost n^3, so coarsening goes on until it has at most 2048 nodes
-    // (a lower limit adds levels whose launches cost every V-cycle more than
-    // the smaller factorisation saves once)
-    const uint32_t kMaxDense = 2048;
+    // nlevels is honoured (blockDiag.cpp:1384-1425).  Only a coarsest level
+    // beyond the sparse direct solver's reach (kMaxDirectNodes) makes the
+    // hierarchy coarsen further.
     for (int k = 1; k < 30; ++k) {
-        // nlevels is honoured, but the hierarchy keeps coarsening while the
-        // coarsest level is too large for the dense FP64 coarse solve
         if (k >= nlevels) {
             helper.wait(last);
-            if (mg->lev.back().da->n_nodes <= kMaxDense) break;
+            if (mg->lev.back().da->n_nodes <= kMaxDirectNodes) break;
         }
         MgLevel &prev = mg->lev.back();
         DevBuf<Key> c = coarsen_build(prev.keys.p, prev.n_oct, s);
@@ -784,14 +597,11 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
         L.n_oct = b.n;
         L.keys = std::move(b);
         last = build_da(&L);
-        if (L.n_oct < 500) {  // blockDiag.cpp:1417-1420
-            helper.wait(last);
-            if (L.da->n_nodes <= kMaxDense) break;
-        }
+        if (L.n_oct < 500) break;  // blockDiag.cpp:1417-1420
     }
     helper.wait_all();
-    // the coarse inverse on the helper, the transfers here
-    helper.submit([&mg](hipStream_t hs) { build_coarse_inverse(*mg, hs); });
+    // the coarse solver on the helper, the transfers here
+    helper.submit([&mg](hipStream_t hs) { build_coarse(*mg, hs); });
     for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
     helper.wait_all();
     HIP_OK(hipStreamSynchronize(s));
@@ -803,7 +613,7 @@ void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream
     mg.lap_fac = lap_fac;
     mg.mass_fac = mass_fac;
     for (auto &L : mg.lev) build_level_diag(L, lap_fac, mass_fac, s);
-    build_coarse_inverse(mg, s);
+    coarse_factor(*mg.coarse, *mg.lev.back().da, lap_fac, mass_fac, s);
     mg_reset_graphs(mg);
 }
 
@@ -841,8 +651,8 @@ void vcycle(MgHierarchy &mg, Work<T> &w, size_t k, const SolveOpts &o, hipStream
     const uint32_t n = L.da->n_nodes;
     const unsigned g = grid_for(n, B);
     T *x = w.x[k].p, *b = w.b[k].p, *y = w.y[k].p;
-    if (k + 1 == mg.lev.size()) {  // coarsest: dense FP64 inverse
-        k_dense_gemv<T><<<grid_for(n, B / 64), B, 0, s>>>(mg.cinv.p, n, b, x);
+    if (k + 1 == mg.lev.size()) {  // coarsest: sparse FP64 direct solve
+        coarse_solve<T>(*mg.coarse, L.da->bdy_node.p, b, x, s);
         return;
     }
     const T om = (T)o.omega, lap = (T)mg.lap_fac, mass = (T)mg.mass_fac;
@@ -896,6 +706,8 @@ struct Graph {
     }
     ~Graph() { reset(); }
 };
+
+std::mutex g_capture_mu;
 
 bool same_opts(const SolveOpts &a, const SolveOpts &b) {
     return a.fp32_vcycle == b.fp32_vcycle && a.pre == b.pre && a.post == b.post && a.omega == b.omega;
@@ -999,6 +811,10 @@ void run_graph(Graph &g, const SolveOpts &o, bool use_graph, hipStream_t s, F bo
     }
     if (!g.valid || !same_opts(g.key, o)) {
         g.reset();
+        // One capture at a time in the process: request-server slots solve on
+        // their own threads and streams, and the capture and instantiation of
+        // a graph then never run concurrently with another thread's.
+        std::lock_guard<std::mutex> one_capture(g_capture_mu);
         hipGraph_t graph = nullptr;
         // the allocator queues no frees on s while it is captured
         dgx_alloc_capture_begin(s);

@@ -25,6 +25,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/dendro_mi355x.h"
@@ -99,9 +100,12 @@ struct dgx_server_s {
     std::condition_variable cv_work, cv_done;
     std::deque<Request> queue;
     std::unordered_map<uint64_t, Done> done;
+    std::unordered_set<uint64_t> consumed;  // ids whose result dgx_server_wait has handed out
     std::list<Resident> cache;  // front = most recently used
     uint64_t reserved = 0;      // estimates of running requests + bytes of resident problems
     uint64_t next_id = 1;
+    uint64_t running = 0, evictions = 0;
+    int fail_next_resolve = 0;  // fault injection for tests: the next resident re-solve fails
     bool stop = false;
     std::vector<std::thread> workers;
     std::vector<hipStream_t> streams;
@@ -129,6 +133,7 @@ struct dgx_server_s {
             if (victim == cache.end()) return false;
             reserved -= victim->bytes;
             cache.erase(victim);
+            ++evictions;
         }
         return true;
     }
@@ -156,6 +161,7 @@ struct dgx_server_s {
                 rq = std::move(queue.front());
                 queue.pop_front();
                 reserved += rq.est;
+                ++running;
             }
             Done d;
             d.r.id = rq.id;
@@ -182,6 +188,7 @@ struct dgx_server_s {
                 std::lock_guard<std::mutex> lk(mu);
                 reserved -= rq.est;
                 reserved += keep_bytes;
+                --running;
                 if (rq.npts >= 65536 && d.r.peak_bytes > 0)  // small ones are covered by the 64 MiB term
                     bytes_per_point[mclass(rq.mnp)] =
                         std::max(bytes_per_point[mclass(rq.mnp)], (double)d.r.peak_bytes / (double)rq.npts);
@@ -195,6 +202,34 @@ struct dgx_server_s {
         std::lock_guard<std::mutex> lk(mu);
         streams.push_back(s);  // destroyed by the server after the resident problems
     }
+
+    // A resident problem taken by a request: on every exit path the request's
+    // stream is drained and the hierarchy marked idle, so another slot can
+    // solve it again; when the request failed, the problem is dropped from
+    // the cache and its bytes are returned to the budget instead, since its
+    // state may be partly overwritten.
+    struct ResidentLease {
+        dgx_server_s *sv = nullptr;
+        Resident *res = nullptr;
+        hipStream_t s = nullptr;
+        bool ok = false;
+        ~ResidentLease() {
+            if (!res) return;
+            (void)hipStreamSynchronize(s);
+            mg_mark_idle(*res->mg);
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (ok) {
+                res->busy = false;
+                return;
+            }
+            for (auto it = sv->cache.begin(); it != sv->cache.end(); ++it)
+                if (&*it == res) {
+                    sv->reserved -= it->bytes;
+                    sv->cache.erase(it);
+                    break;
+                }
+        }
+    };
 
     void run(Request &rq, hipStream_t s, Done &d, uint64_t &keep_bytes, uint64_t tag) {
         // resident hierarchy for this request?
@@ -210,6 +245,10 @@ struct dgx_server_s {
                     break;
                 }
         }
+        ResidentLease lease;
+        lease.sv = this;
+        lease.res = res;
+        lease.s = s;
         std::unique_ptr<MgHierarchy> own;
         MgHierarchy *mg = nullptr;
 
@@ -218,6 +257,18 @@ struct dgx_server_s {
             mg = res->mg.get();
             octants = res->octants;
             d.r.reused = 1;
+            bool inject = false;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (fail_next_resolve) {
+                    fail_next_resolve = 0;
+                    inject = true;
+                }
+            }
+            if (inject) {
+                dgx_set_error("server: injected failure of a resident re-solve");
+                throw DgxError();
+            }
         } else {
             const double ta = now();
             const size_t n = rq.xyz.size() / 3;
@@ -279,10 +330,12 @@ struct dgx_server_s {
         x.release();
         // what is still live under this request's tag is the new hierarchy
         const int64_t hier_bytes = res ? 0 : dgx_alloc_tag_live(tag);
-        std::lock_guard<std::mutex> lk(mu);
         if (res) {
-            res->busy = false;
-        } else if (o.keep_resident) {
+            lease.ok = true;  // the lease clears busy once this returns
+            return;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        if (o.keep_resident) {
             Resident r;
             r.key = rq.key;
             r.bytes = (uint64_t)std::max<int64_t>(hier_bytes, 1 << 20);  // measured footprint of the hierarchy
@@ -369,9 +422,18 @@ int dgx_server_wait(dgx_server sv, uint64_t id, dgx_served *res, double *x, size
             dgx_set_error("server: unknown request %llu", (unsigned long long)id);
             return -1;
         }
-        sv->cv_done.wait(lk, [&] { return sv->done.count(id) > 0; });
+        if (sv->consumed.count(id)) {
+            dgx_set_error("server: request %llu was already waited for", (unsigned long long)id);
+            return -1;
+        }
+        sv->cv_done.wait(lk, [&] { return sv->done.count(id) > 0 || sv->consumed.count(id) > 0; });
+        if (sv->consumed.count(id)) {  // another thread's wait took it meanwhile
+            dgx_set_error("server: request %llu was already waited for", (unsigned long long)id);
+            return -1;
+        }
         d = std::move(sv->done[id]);
         sv->done.erase(id);
+        sv->consumed.insert(id);
     }
     *res = d.r;
     if (d.r.status) {
@@ -398,8 +460,40 @@ int dgx_server_peek(dgx_server sv, uint64_t id, dgx_served *res) {
         dgx_set_error("server: unknown request %llu", (unsigned long long)id);
         return -1;
     }
-    sv->cv_done.wait(lk, [&] { return sv->done.count(id) > 0; });
+    sv->cv_done.wait(lk, [&] { return sv->done.count(id) > 0 || sv->consumed.count(id) > 0; });
+    if (sv->consumed.count(id)) {
+        dgx_set_error("server: request %llu was already waited for", (unsigned long long)id);
+        return -1;
+    }
     *res = sv->done[id].r;
+    return 0;
+}
+
+int dgx_server_get_stats(dgx_server sv, dgx_server_stats *st) {
+    if (!sv || !st) {
+        dgx_set_error("server: null argument");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    st->budget = sv->o.hbm_budget;
+    st->reserved = sv->reserved;
+    st->resident = sv->cache.size();
+    uint64_t rb = 0;
+    for (const Resident &r : sv->cache) rb += r.bytes;
+    st->resident_bytes = rb;
+    st->queued = sv->queue.size();
+    st->running = sv->running;
+    st->evictions = sv->evictions;
+    return 0;
+}
+
+int dgx_server_inject_fault(dgx_server sv, int kind) {
+    if (!sv || kind != DGX_SERVER_FAULT_RESOLVE) {
+        dgx_set_error("server: null argument or unknown fault kind");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->fail_next_resolve = 1;
     return 0;
 }
 

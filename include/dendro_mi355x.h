@@ -128,11 +128,12 @@ typedef struct dgx_mg_s *dgx_mg;
 /* DAMGCreateAndSetDA + DAMGSetKSP (include/omg/omg.h:470-528; src/omg/
  * blockDiag.cpp:780-1047, 1312-2190) on one rank: nlevels levels built by
  * coarsenOctree + balanceOctree from the finest octree (stopping early below
- * 500 octants, blockDiag.cpp:1417-1420; coarsening continues past nlevels
- * while the coarsest level has more than 2048 nodes, so that the dense FP64
- * coarse solve stays cheap), a DA per level, interpolation P and
- * restriction R = P^T between levels, Jacobi diagonals and the coarsest-level
- * FP64 direct solver.  The octree is not consumed. */
+ * 500 octants, blockDiag.cpp:1417-1420; only a coarsest level of more than
+ * 131072 nodes, beyond the direct solver's reach, makes it coarsen further),
+ * a DA per level, interpolation P and restriction R = P^T between levels,
+ * Jacobi diagonals and the coarsest-level sparse FP64 direct solver (nested
+ * dissection + multifrontal factorisation on the device, in place of the
+ * reference's PETSc LU, bin/optionsFBM:57).  The octree is not consumed. */
 int dgx_mg_create(dgx_octree finest, int nlevels, double lap_fac, double mass_fac, dgx_mg *out);
 /* The operator factors of DAMGSetKSP's Jacobian (-lapFac lap + massFac I,
  * omgJac.C:958-1057): recomputes the level diagonals and the coarse solver. */
@@ -162,6 +163,43 @@ typedef struct {
  * iteration on device vectors b, u of n_nodes entries (finest level). */
 int dgx_mg_solve(dgx_mg mg, const double *b, double *u, const dgx_solve_opts *opts, dgx_solve_stats *stats);
 int dgx_mg_free(dgx_mg mg);
+
+/* Coarsest-level direct solver of a hierarchy (the reference's coarse PCLU,
+ * bin/optionsFBM:57): out[0] interior nodes, [1] fronts, [2] levels of the
+ * elimination tree, [3] largest front, [4] largest separator, [5] doubles in
+ * the frontal matrices. */
+int dgx_mg_coarse_info(dgx_mg mg, uint64_t out[6]);
+
+/* Symbolic analysis of the coarsest-level solver on host arrays (no device):
+ * nested dissection of the interior-node graph of n nodes and n_elem elements
+ * (conn: 8 x n_elem node ids, conn[j*n_elem + e]; meta: mask | child << 8 |
+ * level << 11 per element; bdy: 1 for boundary nodes; xyz: 3 coordinates per
+ * node), frontal index sets and the places of the matrix entries in the fronts.
+ * Parts of at most `leaf` interior nodes are not split (0: 128); at most
+ * `dense` interior nodes make one front.  For inspection and tests. */
+typedef struct dgx_direct_plan_s *dgx_direct_plan;
+typedef struct {
+    uint64_t interior, fronts, links, levels, entries, idx_len, inv_len, fmap_len, a_nnz, a_src_len, upd_len;
+    uint32_t max_m, max_s;
+} dgx_direct_sizes;
+int dgx_direct_plan_host(uint32_t n, uint64_t n_elem, const uint32_t *conn, const uint16_t *meta, const uint8_t *bdy,
+                         const uint32_t *xyz, uint32_t leaf, uint32_t dense, dgx_direct_plan *out,
+                         dgx_direct_sizes *sz);
+/* Copies the plan out (any pointer may be NULL):
+ *   front9: per front (in elimination order) offset of its dense matrix, size m,
+ *     pivots s, offset of its index set in idx, of its update vector, of its
+ *     forward values, first link, number of children, height in the tree;
+ *   link3: per link child front, offset in inv (m of the parent entries: the
+ *     child's update index of each parent position or -1), offset in fmap
+ *     (parent position of each child update index);
+ *   idx: node ids of the fronts' index sets (pivots first);
+ *   a_off/a_src/a_dst: per matrix entry of the interior rows (columns sorted),
+ *     its element-matrix contributions e*64 + k*8 + j (row j, column k of the
+ *     element's child-frame matrix) and its offset in the fronts;
+ *   inode: interior index -> node id. */
+int dgx_direct_plan_export(dgx_direct_plan h, uint64_t *front9, uint64_t *link3, int32_t *inv, uint32_t *fmap,
+                           uint32_t *idx, uint32_t *a_off, uint32_t *a_src, uint64_t *a_dst, uint32_t *inode);
+int dgx_direct_plan_free(dgx_direct_plan h);
 
 /* ---- request server ----------------------------------------------------- */
 /* No counterpart in the reference (its drivers solve one problem per run):
@@ -207,6 +245,21 @@ int dgx_server_wait(dgx_server sv, uint64_t id, dgx_served *res, double *x, size
 /* Blocks until request id is done and reports it without consuming it. */
 int dgx_server_peek(dgx_server sv, uint64_t id, dgx_served *res);
 double dgx_server_clock(dgx_server sv);
+/* A result can be waited for once: a second dgx_server_wait or dgx_server_peek
+ * on a consumed id returns an error instead of blocking. */
+/* Admission state (no reference counterpart): the effective budget, the bytes
+ * reserved by running requests' estimates plus resident problems, the
+ * resident problems and their measured bytes, queued and running requests,
+ * and the number of evictions so far. */
+typedef struct {
+    uint64_t budget, reserved, resident, resident_bytes, queued, running, evictions;
+} dgx_server_stats;
+int dgx_server_get_stats(dgx_server sv, dgx_server_stats *st);
+/* Fault injection for tests of the error paths: DGX_SERVER_FAULT_RESOLVE makes
+ * the next re-solve of a resident problem fail; the problem is then dropped
+ * from the cache and its bytes are returned to the budget. */
+enum { DGX_SERVER_FAULT_RESOLVE = 1 };
+int dgx_server_inject_fault(dgx_server sv, int kind);
 /* Finishes the queued requests, then frees everything. */
 int dgx_server_free(dgx_server sv);
 

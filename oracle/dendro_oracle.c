@@ -20,6 +20,9 @@
 #include <string.h>
 #include <stdio.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "dendro_oracle.h"
 
@@ -241,10 +244,58 @@ static void tv_push(tvec *v, or_tn t) {
     v->a[v->n++] = t;
 }
 
+/* omp_par::merge_sort (include/ompUtils.tcc:78-133): the array is cut into */
+/* one run per thread, the runs are sorted in parallel and merged pairwise.  */
+/* Restated as a recursive task merge sort with the same comparator; equal   */
+/* TreeNodes are identical, so its output equals any correct sort's.         */
+static void merge_runs(const or_tn *a, size_t na, const or_tn *b, size_t nb, or_tn *out) {
+    size_t i = 0, j = 0, k = 0;
+    while (i < na && j < nb) out[k++] = (tn_cmp(&b[j], &a[i]) < 0) ? b[j++] : a[i++];
+    while (i < na) out[k++] = a[i++];
+    while (j < nb) out[k++] = b[j++];
+}
+static void msort_rec(or_tn *a, or_tn *tmp, size_t n, int depth) {
+    if (n < 16384 || depth <= 0) {
+        qsort(a, n, sizeof(or_tn), tn_cmp);
+        return;
+    }
+    size_t h = n / 2;
+#pragma omp task shared(a, tmp) if (depth > 0)
+    msort_rec(a, tmp, h, depth - 1);
+    msort_rec(a + h, tmp + h, n - h, depth - 1);
+#pragma omp taskwait
+    merge_runs(a, h, a + h, n - h, tmp);
+    memcpy(a, tmp, n * sizeof(or_tn));
+}
+static void par_sort(or_tn *a, size_t n) {
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    if (nt <= 1 || n < 32768) {
+        qsort(a, n, sizeof(or_tn), tn_cmp);
+        return;
+    }
+    int depth = 0;
+    while ((1 << depth) < 2 * nt) depth++;
+    or_tn *tmp = (or_tn *)malloc(n * sizeof(or_tn));
+#pragma omp parallel
+#pragma omp single
+    msort_rec(a, tmp, n, depth);
+    free(tmp);
+}
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* seq::makeVectorUnique (seqUtils.tcc:106-): sort + drop equal neighbours. */
 static void tv_sort_unique(tvec *v) {
     if (v->n < 2) return;
-    qsort(v->a, v->n, sizeof(or_tn), tn_cmp);
+    par_sort(v->a, v->n);
     size_t k = 1;
     for (size_t i = 1; i < v->n; i++)
         if (!tn_eq(&v->a[i], &v->a[k - 1])) v->a[k++] = v->a[i];
@@ -255,7 +306,7 @@ void or_free(void *p) { free(p); }
 
 /* Sort an array of TreeNodes with operator< (par::sampleSort on one rank  */
 /* reduces to omp_par::merge_sort, parUtils.tcc:1848-1876).                 */
-void or_sort(or_tn *a, size_t n) { qsort(a, n, sizeof(or_tn), tn_cmp); }
+void or_sort(or_tn *a, size_t n) { par_sort(a, n); }
 
 /* ------------------------------------------------------------------------- */
 /* points2Octree on one rank: src/construct.cpp:911-974 + p2oLocal :976-1077  */
@@ -344,7 +395,7 @@ int or_points2octree(const double *pts, size_t n, uint32_t maxDepth, uint32_t ma
     }
     or_tn *nodes = (or_tn *)malloc(n * sizeof(or_tn));
     or_quantise(pts, n, maxDepth, nodes);
-    qsort(nodes, n, sizeof(or_tn), tn_cmp);  /* omp_par::merge_sort, construct.cpp:955 */
+    par_sort(nodes, n);  /* omp_par::merge_sort, construct.cpp:955 */
     int rc = p2o_local(nodes, n, maxNumPts, maxDepth, &res);
     free(nodes);
     if (rc) { free(res.a); *out = NULL; *nout = 0; return rc; }
@@ -438,13 +489,18 @@ int or_balance(const or_tn *in, size_t n, int incCorner, int emulate_limit, or_t
 /* Reference checkers (src/test/testUtils.C), restated for linear/complete/  */
 /* balanced octrees.  Return 1 on success.                                    */
 /* ------------------------------------------------------------------------- */
+/* The checkers run their independent per-leaf tests on all threads. */
 int or_is_sorted_unique(const or_tn *a, size_t n) {
-    for (size_t i = 1; i < n; i++) if (!or_tn_less(&a[i - 1], &a[i])) return 0;
-    return 1;
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (size_t i = 1; i < n; i++) ok = ok && or_tn_less(&a[i - 1], &a[i]);
+    return ok;
 }
 int or_is_linear(const or_tn *a, size_t n) {
-    for (size_t i = 1; i < n; i++) if (tn_is_ancestor(&a[i - 1], &a[i])) return 0;
-    return 1;
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (size_t i = 1; i < n; i++) ok = ok && !tn_is_ancestor(&a[i - 1], &a[i]);
+    return ok;
 }
 /* complete (testUtils.C:28-80): the first leaf holds the root's first cell, */
 /* each leaf's SFC successor (getNext) is an ancestor-or-self of the next     */
@@ -456,10 +512,13 @@ int or_is_complete(const or_tn *a, size_t n) {
     or_tn root = tn_make(0, 0, 0, 0, md), first = root;
     while (LEV(&first) < md) first = tn_first_child(&first);
     if (!(tn_eq(&a[0], &first) || tn_is_ancestor(&a[0], &first))) return 0;
-    for (size_t i = 0; i + 1 < n; i++) {
+    int ok = 1;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+    for (size_t i = 0; i < n - 1; i++) {
         or_tn nx = tn_next(&a[i]);
-        if (!(tn_eq(&nx, &a[i + 1]) || tn_is_ancestor(&nx, &a[i + 1]))) return 0;
+        ok = ok && (tn_eq(&nx, &a[i + 1]) || tn_is_ancestor(&nx, &a[i + 1]));
     }
+    if (!ok) return 0;
     or_tn d0 = tn_dld(&root), d1 = tn_dld(&a[n - 1]);
     return tn_eq(&d0, &d1);
 }
@@ -483,10 +542,23 @@ static int max_lower_bound(const or_tn *a, size_t n, const or_tn *key, size_t *i
 /* cells adjacent to the shared feature is equivalent to checking from the   */
 /* finer leaf; since every pair is visited from both sides, checking only     */
 /* "neighbour cover is not coarser than level-1" suffices.                    */
+static int leaf_is_balanced(const or_tn *a, size_t n, size_t i);
 int or_is_balanced(const or_tn *a, size_t n) {
-    if (!n) return 1;
+    int ok = 1;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(&& : ok)
+    for (size_t i = 0; i < n; i++) ok = ok && leaf_is_balanced(a, n, i);
+    return ok;
+}
+/* The same check for the leaves a[idx[k]] only (a sample of a large tree). */
+int or_is_balanced_at(const or_tn *a, size_t n, const uint64_t *idx, size_t m) {
+    int ok = 1;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(&& : ok)
+    for (size_t k = 0; k < m; k++) ok = ok && idx[k] < n && leaf_is_balanced(a, n, (size_t)idx[k]);
+    return ok;
+}
+static int leaf_is_balanced(const or_tn *a, size_t n, size_t i) {
     uint32_t md = a[0].md, dom = 1u << md;
-    for (size_t i = 0; i < n; i++) {
+    {
         uint32_t l = LEV(&a[i]), len = 1u << (md - l);
         for (int dz = -1; dz <= 1; dz++)
             for (int dy = -1; dy <= 1; dy++)
@@ -561,7 +633,7 @@ int or_da_build(const or_tn *oct, size_t n, or_da *da) {
     }
     for (size_t i = 0; i < bdy.n; i++) tv_push(&all, bdy.a[i]);
     free(bdy.a);
-    qsort(all.a, all.n, sizeof(or_tn), tn_cmp); /* SFC_treeSort (odaFactory.cpp:186) */
+    par_sort(all.a, all.n); /* SFC_treeSort (odaFactory.cpp:186) */
     or_tn *in = all.a;
     size_t N = all.n;
 

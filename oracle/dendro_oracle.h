@@ -37,6 +37,7 @@ void or_get_tables(uint8_t *rot384, uint8_t *ht192);
 void or_get_etype_masks(uint8_t *out144);
 int or_tn_less(const or_tn *a, const or_tn *b);
 void or_sort(or_tn *a, size_t n);
+int or_num_threads(void); /* OpenMP threads the sorts and checkers use */
 void or_free(void *p);
 
 void or_quantise(const double *pts, size_t n, uint32_t maxDepth, or_tn *nodes);
@@ -47,6 +48,7 @@ int or_is_sorted_unique(const or_tn *a, size_t n);
 int or_is_linear(const or_tn *a, size_t n);
 int or_is_complete(const or_tn *a, size_t n);
 int or_is_balanced(const or_tn *a, size_t n);
+int or_is_balanced_at(const or_tn *a, size_t n, const uint64_t *idx, size_t m);
 
 int or_da_build(const or_tn *oct, size_t n, or_da *da);
 void or_da_free(or_da *da);

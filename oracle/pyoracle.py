@@ -55,6 +55,8 @@ def lib():
         L.or_balance.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(C.c_size_t)]
         for f in ("or_is_sorted_unique", "or_is_linear", "or_is_complete", "or_is_balanced"):
             getattr(L, f).argtypes = [vp, C.c_size_t]
+        L.or_is_balanced_at.argtypes = [vp, C.c_size_t, vp, C.c_size_t]
+        L.or_num_threads.restype = C.c_int
         L.or_da_build.argtypes = [vp, C.c_size_t, C.POINTER(_DA)]
         L.or_da_free.argtypes = [C.POINTER(_DA)]
         L.or_elem_type.argtypes = [C.c_uint32, C.c_uint32]
@@ -149,6 +151,18 @@ def is_complete(tn):
 def is_balanced(tn):
     tn = np.ascontiguousarray(tn, np.uint32)
     return bool(lib().or_is_balanced(_p(tn), len(tn)))
+
+
+def is_balanced_at(tn, idx):
+    """The balance check of is_balanced for the leaves tn[idx] only."""
+    tn = np.ascontiguousarray(tn, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint64)
+    return bool(lib().or_is_balanced_at(_p(tn), len(tn), _p(idx), len(idx)))
+
+
+def num_threads():
+    """OpenMP threads of the oracle's sorts and checkers."""
+    return int(lib().or_num_threads())
 
 
 class DA:

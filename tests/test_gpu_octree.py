@@ -240,19 +240,43 @@ def test_uniform_256k_full_parity(dgx, oracle, stencils):
     check_pipeline(dgx, oracle, pts, 30, 1, *stencils)
 
 
-def test_lognormal_4m_full_size_properties(dgx, oracle):
+@pytest.mark.timeout(600)
+def test_lognormal_4m_full_parity(dgx, oracle, stencils):
     """Config 2 at its full size (4,194,304 log-normal points, maxNumPts=8):
-    octree and balanced octree bit-exact against the oracle, and the operator's
-    conservation properties hold."""
+    the octree, the balanced octree and the DA maps (nlist, hanging masks,
+    oct levels, node compaction) bit-exact against the oracle, and 100 FP64
+    -lap+I matvecs on independent N(0,1) vectors within 1e-12 of the oracle's
+    element loop with the reference's type-2 tables (odaJac.C:275-292)."""
+    L, M = stencils
     pts = dgx.workload_points("lognormal-4M")
     ou = dgx.points2octree(pts, 30, 8)
-    tu = ou.treenodes()
-    ru = oracle.points2octree(pts, 30, 8)
-    assert np.array_equal(tu, ru)
     ob = dgx.balance_octree(ou)
-    tb = ob.treenodes()
-    assert oracle.is_complete(tb) and oracle.is_balanced(tb)
     da = dgx.DA(ob)
+    tu, tb = ou.treenodes(), ob.treenodes()
+    nl, mk, ol, l2n = da.maps()
+    print("lognormal-4M: GPU octree, balance and DA built", len(tu), len(tb), da.n_nodes, flush=True)
+    ru = oracle.points2octree(pts, 30, 8)
+    assert np.array_equal(tu, ru), "points2Octree"
+    print("points2Octree bit-exact", flush=True)
+    rb = oracle.balance(ru)
+    assert np.array_equal(tb, rb), "balanceOctree"
+    print("balanceOctree bit-exact", flush=True)
+    rda = oracle.DA(rb)
+    assert da.element_size == rda.n_elem and da.local_buffer_size == rda.n_local
+    assert da.node_size == rda.node_size and da.boundary_node_size == rda.bnd_node_size
+    assert np.array_equal(nl, rda.nlist), "DA nlist"
+    assert np.array_equal(mk, rda.mask), "DA hanging masks"
+    assert np.array_equal(ol, rda.oct_levels), "DA oct levels"
+    assert np.array_equal(l2n, rda.local_to_node), "DA node compaction"
+    print("DA maps bit-exact", flush=True)
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for k in range(100):
+        x = rng.standard_normal(da.n_nodes)
+        worst = max(worst, rel_err(da.matvec_host(x), rda.matvec(x, L, M)))
+        if k % 25 == 24:
+            print(f"{k + 1} matvecs, worst relative error {worst:.2e}", flush=True)
+    assert worst < MATVEC_RTOL
     one = np.ones(da.n_nodes)
     assert abs(da.matvec_host(one, 0.0, 1.0).sum() - 1.0) < 1e-12
     assert np.abs(da.matvec_host(one, 1.0, 0.0)).max() < 1e-12

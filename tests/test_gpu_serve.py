@@ -87,3 +87,71 @@ def test_bad_requests_fail_loudly(dgx):
     with pytest.raises(dgx.DgxError):
         sv.wait(rid)
     sv.close()
+
+
+def test_wait_twice_is_an_error(dgx):
+    (req, pts), = small_requests(dgx, 1, seed=21)
+    sv = dgx.Server(slots=1, hbm_budget=8e9, nlevels=3, keep_resident=False)
+    rid = sv.submit(pts, 30, req["max_num_pts"], 1)
+    assert sv.wait(rid)["status"] == 0
+    with pytest.raises(dgx.DgxError, match="already waited"):
+        sv.wait(rid)
+    with pytest.raises(dgx.DgxError, match="already waited"):
+        sv.wait(rid, solution=True)
+    sv.close()
+
+
+def test_failed_resident_resolve_releases_its_entry(dgx):
+    """A resident problem whose re-solve fails is dropped and its bytes go
+    back to the budget; a later request that needs that room is admitted."""
+    (req, pts), (req2, pts2) = small_requests(dgx, 2, seed=22)
+    sv = dgx.Server(slots=2, hbm_budget=8e9, nlevels=3, keep_resident=True)
+    a = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 1))
+    st = sv.stats()
+    assert st["resident"] == 1 and st["reserved"] == st["resident_bytes"] == a["resident_bytes"] > 0
+    sv.inject_fault()
+    with pytest.raises(dgx.DgxError, match="injected"):
+        sv.wait(sv.submit(pts, 30, req["max_num_pts"], 1))
+    st = sv.stats()
+    assert st["resident"] == 0 and st["reserved"] == 0 and st["running"] == 0
+    # the same problem is built again (not reused), and another one runs
+    b = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 1))
+    c = sv.wait(sv.submit(pts2, 30, req2["max_num_pts"], 2))
+    assert b["reused"] == 0 and b["status"] == 0 and c["status"] == 0
+    assert b["rel_residual"] <= 1e-8 and c["rel_residual"] <= 1e-8
+    sv.close()
+
+
+def test_eviction_accounting_across_slots(dgx):
+    """Resident bytes are counted per request, whichever slot frees them: after
+    a stream with evictions on two slots, the reserved bytes equal the
+    measured bytes of the resident problems and stay within the budget."""
+    reqs = small_requests(dgx, 20, seed=23)
+    budget = 400 << 20
+    sv = dgx.Server(slots=2, hbm_budget=budget, nlevels=3, keep_resident=True)
+    ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
+    res = [sv.wait(i) for i in ids]
+    st = sv.stats()
+    sv.close()
+    assert all(r["status"] == 0 and r["rel_residual"] <= 1e-8 for r in res)
+    assert st["running"] == 0 and st["queued"] == 0
+    assert st["reserved"] == st["resident_bytes"] <= st["budget"]
+    assert st["resident"] >= 1
+    kept = sum(r["resident_bytes"] for r in res)
+    assert kept >= st["resident_bytes"]
+    if kept > budget:
+        assert st["evictions"] >= 1
+
+
+def test_resident_resolves_on_two_slots_with_graphs(dgx):
+    """Resident problems re-solved at once on two slots (graph capture on both
+    threads) give the first solve's result."""
+    reqs = small_requests(dgx, 4, seed=24)
+    sv = dgx.Server(slots=2, hbm_budget=64e9, nlevels=4, keep_resident=True, return_solution=True)
+    first = [sv.wait(sv.submit(c, 30, r["max_num_pts"], r["seed"]), solution=True) for r, c in reqs]
+    ids = [sv.submit(c, 30, r["max_num_pts"], r["seed"]) for r, c in reqs]
+    again = [sv.wait(i, solution=True) for i in ids]
+    sv.close()
+    for a, b in zip(first, again):
+        assert b["status"] == 0 and b["reused"] == 1
+        assert np.linalg.norm(a["x"] - b["x"]) <= 1e-10 * np.linalg.norm(a["x"])

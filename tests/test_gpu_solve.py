@@ -180,3 +180,97 @@ def test_matvec_path_for_large_blocks(dgx, problem, stencils, monkeypatch):
     assert np.abs(da.matvec_dirichlet_host(x) - yr).max() / np.abs(yr).max() < 1e-12
     assert np.abs(da.matvec_host(x) - A @ x).max() / np.abs(A @ x).max() < 1e-12
     solve_against_direct(dgx, ob, rda, bd, *stencils, 1.0, 1.0)
+
+
+def test_coarse_direct_solve_is_exact(dgx, problem):
+    """One level: the V-cycle is the sparse direct solve of the whole Dirichlet
+    matrix (nested dissection, many fronts), so preconditioned CG converges at
+    once, to the direct solution, and the solve is deterministic."""
+    ob, rda, A, bd = problem
+    mg = dgx.MG(ob, 1)
+    assert mg.nlevels == 1 and mg.coarse["fronts"] > 8 and mg.coarse["tree_levels"] >= 3, mg.coarse
+    n = mg.n_nodes
+    f = np.random.default_rng(6).standard_normal(n)
+    df = dgx.DeviceArray.from_host(f)
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    br = db.to_host()
+    AD = dirichlet(A, bd)
+    uref = spla.spsolve(AD.tocsc(), br)
+    outs = []
+    for rep in range(2):
+        du = dgx.DeviceArray((n,), np.float64)
+        st = mg.solve(db, du, fp32=False, rtol=1e-12, maxit=5)
+        u = du.to_host()
+        assert st["iterations"] <= 2 and st["rel_residual"] <= 1e-12, st
+        assert np.linalg.norm(u - uref) / np.linalg.norm(uref) <= 1e-11
+        outs.append(u)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.timeout(300)
+def test_gauss4_5_levels_match_direct_solve(dgx, oracle, stencils):
+    """Config 3's problem at a size scipy solves directly: 100,000 points of the
+    4-Gaussian cloud (sigma 0.02, seed 2), maxNumPts 4, 5-level V(2,2) Jacobi
+    (0.8) PCG to 1e-8; FP64 and FP32 V-cycles within 1e-7 of the direct solve
+    of the operator assembled from the oracle's DA and the reference tables.
+    The hierarchy has exactly the 5 requested levels (blockDiag.cpp:1384-1425)."""
+    L, M = stencils
+    pts = dgx.workload_points("gauss4-16M", 100000)
+    ob = dgx.balance_octree(dgx.points2octree(pts, 30, 4))
+    rda = oracle.DA(ob.treenodes())
+    A = assemble(rda, L, M)
+    bd = boundary_nodes(rda)
+    AD = dirichlet(A, bd)
+    mg = dgx.MG(ob, 5)
+    print("levels", mg.nlevels, mg.level_octants, mg.level_nodes, mg.coarse, flush=True)
+    assert mg.nlevels == 5
+    n = mg.n_nodes
+    f = np.random.default_rng(7).standard_normal(n)
+    df = dgx.DeviceArray.from_host(f)
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    br = db.to_host()
+    lu = spla.splu(AD.tocsc(), permc_spec="MMD_AT_PLUS_A", options=dict(SymmetricMode=True))
+    uref = lu.solve(br)
+    print("direct solve done", flush=True)
+    res = {}
+    for fp32 in (False, True):
+        du = dgx.DeviceArray((n,), np.float64)
+        st = mg.solve(db, du, fp32=fp32, pre=2, post=2, omega=0.8, rtol=1e-8, maxit=300)
+        u = du.to_host()
+        true_res = np.linalg.norm(br - AD @ u) / np.linalg.norm(br)
+        err = np.linalg.norm(u - uref) / np.linalg.norm(uref)
+        print(f"fp32={fp32} it={st['iterations']} res={true_res:.2e} err={err:.2e}", flush=True)
+        assert st["rel_residual"] <= 1e-8 and true_res <= 1.5e-8 and err <= 1e-7
+        res[fp32] = u
+    assert np.linalg.norm(res[True] - res[False]) / np.linalg.norm(res[False]) <= 1e-7
+
+
+@pytest.mark.timeout(600)
+def test_gauss4_16m_full_size_solve(dgx):
+    """Config 3 at its full size (16,777,216 points, 4 Gaussians, maxNumPts 4):
+    5-level hierarchy as requested; the FP64 solve reaches 1e-8 in the FP64
+    residual recomputed with the Dirichlet operator, and the FP32 V-cycle
+    solution agrees with it to 1e-7."""
+    pts = dgx.workload_points("gauss4-16M")
+    ob = dgx.balance_octree(dgx.points2octree(pts, 30, 4))
+    del pts
+    mg = dgx.MG(ob, 5)
+    print("levels", mg.nlevels, mg.level_octants, mg.level_nodes, mg.coarse, flush=True)
+    assert mg.nlevels == 5
+    n = mg.n_nodes
+    df = dgx.DeviceArray.from_host(np.random.default_rng(2).standard_normal(n))
+    db = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    b = db.to_host()
+    u = {}
+    for fp32 in (False, True):
+        du = dgx.DeviceArray((n,), np.float64)
+        st = mg.solve(db, du, fp32=fp32, rtol=1e-8, maxit=100)
+        u[fp32] = du.to_host()
+        r = b - mg.da.matvec_dirichlet_host(u[fp32])
+        rel = np.linalg.norm(r) / np.linalg.norm(b)
+        print(f"fp32={fp32} it={st['iterations']} res={rel:.2e}", flush=True)
+        assert st["rel_residual"] <= 1e-8 and rel <= 1.5e-8
+    assert np.linalg.norm(u[True] - u[False]) / np.linalg.norm(u[False]) <= 1e-7
