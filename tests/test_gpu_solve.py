@@ -185,7 +185,9 @@ def test_matvec_path_for_large_blocks(dgx, problem, stencils, monkeypatch):
 def test_coarse_direct_solve_is_exact(dgx, problem):
     """One level: the V-cycle is the sparse direct solve of the whole Dirichlet
     matrix (nested dissection, many fronts), so preconditioned CG converges at
-    once, to the direct solution, and the solve is deterministic."""
+    once, to the direct solution.  Repeated solves agree to rounding: the
+    direct solve sums in a fixed order, but CG's matvec accumulates element
+    contributions with LDS atomics, whose order is not fixed."""
     ob, rda, A, bd = problem
     mg = dgx.MG(ob, 1)
     assert mg.nlevels == 1 and mg.coarse["fronts"] > 8 and mg.coarse["tree_levels"] >= 3, mg.coarse
@@ -205,7 +207,7 @@ def test_coarse_direct_solve_is_exact(dgx, problem):
         assert st["iterations"] <= 2 and st["rel_residual"] <= 1e-12, st
         assert np.linalg.norm(u - uref) / np.linalg.norm(uref) <= 1e-11
         outs.append(u)
-    assert np.array_equal(outs[0], outs[1])
+    assert np.linalg.norm(outs[0] - outs[1]) <= 1e-13 * np.linalg.norm(outs[0])
 
 
 @pytest.mark.timeout(300)
