@@ -39,6 +39,7 @@ EXPORTED = [
     "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
     "dgx_write_nodes_binary", "dgx_write_vtk", "dgx_server_get_stats", "dgx_server_inject_fault",
     "dgx_mg_coarse_info", "dgx_direct_plan_host", "dgx_direct_plan_export", "dgx_direct_plan_free",
+    "dgx_da_matvec_noalloc",
 ]
 
 
@@ -148,6 +149,7 @@ def load_library():
         "dgx_da_export": ([vp, vp, vp, vp, vp], i32),
         "dgx_da_matvec": ([vp, vp, vp, d, d], i32),
         "dgx_da_matvec_f32": ([vp, vp, vp, C.c_float, C.c_float], i32),
+        "dgx_da_matvec_noalloc": ([vp, vp, vp, d, d], i32),
         "dgx_da_diagonal": ([vp, vp, d, d], i32),
         "dgx_da_free": ([vp], i32),
         "dgx_gen_points": ([i32, u64, sz, i32, d, d, d, d, i32, vp], i32),

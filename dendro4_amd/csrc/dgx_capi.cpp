@@ -128,6 +128,8 @@ int dgx_init(int device) {
     // library's caching allocator absorbs the churn of temporaries.
     uint64_t thr = uint64_t(4) << 30;
     HIP_OK(hipMemPoolSetAttribute(g_ctx.pool, hipMemPoolAttrReleaseThreshold, &thr));
+    // device address of the static scratch array of dgx_da_matvec_noalloc
+    da_noalloc_init();
     g_ctx.device = device;
     g_ctx.ready = true;
     API_END
@@ -361,6 +363,13 @@ int dgx_da_matvec(dgx_da d, const double *x, double *y, double lap_fac, double m
     ensure_ctx();
     if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
     da_matvec_f64(d->get(), x, y, lap_fac, mass_fac, S());
+    API_END
+}
+int dgx_da_matvec_noalloc(dgx_da d, const double *x, double *y, double lap_fac, double mass_fac) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !x || !y) { dgx_set_error("matvec: null argument"); return -1; }
+    da_matvec_f64_noalloc(d->get(), x, y, lap_fac, mass_fac, S());
     API_END
 }
 int dgx_da_matvec_f32(dgx_da d, const float *x, float *y, float lap_fac, float mass_fac) {

@@ -55,6 +55,15 @@ void da_build_blocks(DaDevice &da, hipStream_t s, bool colour = true);
 // FP64 on compact node vectors (Jacobian1MatMult, odaJac.C:275-366).
 void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s);
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s);
+// The FP64 matvec with the shared nodes' partial sums in a static device array
+// of kNoAllocScratchBytes (one per device) instead of the DA's buffer: no
+// allocation.  Throws DgxError when the DA's partial slots exceed the array.
+// Calls that overlap on one device race on the array.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+void da_matvec_f64_noalloc(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac,
+                           hipStream_t s);
+// Takes the array's device address for the current device (once per device).
+void da_noalloc_init();
 // Diagonal of the same operator (Jacobian1MatGetDiagonal, odaJac.C:160-227).
 void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double mass_fac, hipStream_t s);
 // Homogeneous-Dirichlet operator (DirichletJacobianMatMult, omgJac.C:958-1057):

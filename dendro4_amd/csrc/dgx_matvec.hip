@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -562,15 +563,17 @@ unsigned pipe_grid(const void *kernel, uint32_t nblk) {
 // EPI as for k_matvec_pipe; with EPI_JACOBI, y is x and scratch (n_nodes
 // entries) holds A x on the path for DAs whose blocks exceed the pipelined
 // kernel's 1024 nodes.
+// partial_at: where the shared nodes' partial sums go (nullptr: the DA's own
+// buffer of partial_slots entries).
 template <typename T, bool DIR, int EPI>
 void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, const T *bvec, const T *dinv,
-                   T omega, T *scratch, hipStream_t s) {
+                   T omega, T *scratch, hipStream_t s, T *partial_at = nullptr) {
     if (!da.n_elem) return;
     if (da.elem_block != PER * B) {
         dgx_set_error("matvec: DA built with element blocks of %u, kernel expects %d", da.elem_block, PER * B);
         throw DgxError();
     }
-    T *partial = reinterpret_cast<T *>(da.partial.p);
+    T *partial = partial_at ? partial_at : reinterpret_cast<T *>(da.partial.p);
     const uint32_t mn = da.max_block_nodes;
     // DGX_MATVEC_BLOCKED=1 takes the path of DAs with large blocks for every
     // DA (a test hook: real meshes rarely reach it)
@@ -613,6 +616,56 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
 void da_matvec_f64(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac, hipStream_t s) {
     matvec_blocks<double, false, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0, nullptr, s);
 }
+// ---------------------------------------------------------------------------
+// Matvec without allocation: the shared nodes' partial sums go to a static
+// device array instead of the DA's buffer.  Each device holds its own copy of
+// the array; host code takes its address from hipGetSymbolAddress once per
+// device and passes that pointer to the kernels (the array's name in host code
+// is not a device address).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchDoubles = kNoAllocScratchBytes / sizeof(double);
+__device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+double *scratch_ptr_of_current_device() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        dgx_set_error("matvec (no allocation): device %d out of range", dev);
+        throw DgxError();
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            dgx_set_error("matvec (no allocation): hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+            throw DgxError();
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+void da_noalloc_init() { (void)scratch_ptr_of_current_device(); }
+
+void da_matvec_f64_noalloc(const DaDevice &da, const double *x, double *y, double lap_fac, double mass_fac,
+                           hipStream_t s) {
+    if (da.partial_slots > kNoAllocScratchDoubles) {
+        dgx_set_error("matvec (no allocation): the DA has %llu shared-node partial slots, above the %zu that the "
+                      "static scratch array of %zu bytes holds",
+                      (unsigned long long)da.partial_slots, kNoAllocScratchDoubles, kNoAllocScratchBytes);
+        throw DgxError();
+    }
+    matvec_blocks<double, false, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0, nullptr, s,
+                                        scratch_ptr_of_current_device());
+}
+
 void da_matvec_f32(const DaDevice &da, const float *x, float *y, float lap_fac, float mass_fac, hipStream_t s) {
     matvec_blocks<float, false, EPI_Y>(da, x, y, lap_fac, mass_fac, nullptr, nullptr, 0.0f, nullptr, s);
 }

@@ -113,6 +113,15 @@ int dgx_da_export_anchors(dgx_da da, uint32_t *xyz);
  * options -lapMatPropFac / -massMatPropFac (odaJac.C:313-317). */
 int dgx_da_matvec(dgx_da da, const double *x, double *y, double lap_fac, double mass_fac);
 int dgx_da_matvec_f32(dgx_da da, const float *x, float *y, float lap_fac, float mass_fac);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dgx_da_matvec with the shared nodes' partial sums in a
+ * static device array of 64 MiB (8,388,608 doubles, one per GPU, held until
+ * the process ends) instead of the DA's own buffer; it calls no allocation
+ * routine.  Fails with an error when the DA has more partial slots than the
+ * array holds (dgx_da_info.shared_pairs > 8,388,608).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int dgx_da_matvec_noalloc(dgx_da da, const double *x, double *y, double lap_fac, double mass_fac);
 /* Jacobian1MatGetDiagonal (odaJac.C:160-227) on a compact node vector. */
 int dgx_da_diagonal(dgx_da da, double *diag, double lap_fac, double mass_fac);
 int dgx_da_free(dgx_da da);
