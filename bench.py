@@ -6,23 +6,31 @@ clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8.  One step =
 points2Octree + balanceOctree over the point cloud already resident in HBM.
 
 Also reported on the same line:
-  roofline      the FP64 -lap+I matvec on that octree's DA (100 launches
-                back to back, one hipEvent pair on the library stream around
-                them), algorithmic bytes 34 E + 16 Nn per launch against 8 TB/s;
-  solve         Poisson -lap u + u = f, homogeneous Dirichlet, MG(5)-V(2,2)
-                Jacobi(0.8)-preconditioned CG to 1e-8, all-FP64 against the
-                FP32 V-cycle inside FP64 CG: solves/s, p50/p99 latency per
-                solve, error of the mixed solve against the FP64 one; on the
-                lognormal-4M octree and on gauss4-16M (16,777,216 points,
-                4 Gaussians sigma 0.02, seed 2, maxNumPts 4);
-  serve         a served stream of Poisson problems: the first --serve-requests
-                (default 400) of a 2,000-request stream (point counts
-                log-uniform in [2^12, 2^22], three distributions, maxNumPts
-                1/4/8, Poisson arrivals at 200/s, 200 GB HBM budget, problems
-                kept resident): solves/s and p50/p99 latency, FP64 and FP32-MG;
-  cpu_baseline  the oracle (CPU restatement, one thread) timed on the headline
-                workload (all its points by default, about 15 s; --cpu-sample N
-                takes the first N).
+  build_balance  the headline step against the HBM roofline: algorithmic bytes
+                 24 N + 2 * 28 U + 28 B per step;
+  roofline       the FP64 -lap+I matvec (both passes) on the gauss4-16M DA,
+                 whose 628 MB per matvec exceed the 256 MB Infinity Cache: 100
+                 launches back to back between one hipEvent pair on the library
+                 stream, algorithmic bytes 34 E + 16 Nn per launch against 8 TB/s;
+                 `traffic` is the FETCH_SIZE + WRITE_SIZE measurement committed
+                 under profiles/;
+  matvec         the same on the lognormal-4M DA (cache-resident);
+  solve          Poisson -lap u + u = f, homogeneous Dirichlet, MG(5)-V(2,2)
+                 Jacobi(0.8)-preconditioned CG to 1e-8, all-FP64 against the
+                 FP32 V-cycle inside FP64 CG: solves/s, p50/p99 latency per
+                 solve, error of the mixed solve against the FP64 one; on the
+                 lognormal-4M octree and on gauss4-16M (16,777,216 points,
+                 4 Gaussians sigma 0.02, seed 2, maxNumPts 4);
+  large          gauss16-128M (134,217,728 points, 16 Gaussians, seed 4,
+                 maxNumPts 4): build + balance octants/s, DA, FP64 matvec GB/s,
+                 7-level MG set-up and solves, peak device memory;
+  serve          the 2,000-request served stream (point counts log-uniform in
+                 [2^12, 2^22], three distributions, maxNumPts 1/4/8, Poisson
+                 arrivals at 200/s, 200 GB HBM budget, problems kept resident):
+                 solves/s and p50/p99 latency, FP64 and FP32-MG;
+  cpu_baseline   the oracle (CPU restatement, OpenMP sorts as the reference's
+                 omp_par::merge_sort) on all host cores, timed on the whole
+                 headline workload.
 """
 import argparse
 import ctypes as C
@@ -37,7 +45,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_f64_lognormal-4M_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "matvec_f64_gauss4-16M_traffic.txt")
+MATVEC_KERNEL = "k_matvec_pipe<double,false,4,0> + k_matvec_shared<double,false,0>"
 
 
 class HipEvents:
@@ -49,7 +58,6 @@ class HipEvents:
         self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
         self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
-        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
 
     def event(self):
         e = C.c_void_p()
@@ -83,23 +91,41 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def time_matvecs(dgx, ev, stream, da, count):
+    """Back-to-back FP64 matvecs between one event pair: GPU time per matvec."""
+    x = dgx.DeviceArray.from_host(np.random.default_rng(1).standard_normal(da.n_nodes))
+    y = dgx.DeviceArray((da.n_nodes,), np.float64)
+    for _ in range(5):
+        da.matvec(x, y)
+    dgx.synchronize()
+    s0, s1 = ev.event(), ev.event()
+    ev.record(s0, stream)
+    for _ in range(count):
+        da.matvec(x, y)
+    ev.record(s1, stream)
+    dgx.synchronize()
+    ms = ev.elapsed_ms(s0, s1) / count
+    E, Nn = da.element_size, da.n_nodes
+    nbytes = 34 * E + 16 * Nn
+    return {"elements": E, "nodes": Nn, "avg_ms": ms, "algorithmic_bytes": nbytes,
+            "GB_per_s": nbytes / (ms * 1e-3) / 1e9, "flops": 332 * E,
+            "GFLOP_per_s": 332 * E / (ms * 1e-3) / 1e9, "da_blocks": da.block_info}
+
+
 def solve_section(dgx, ob, nlevels, solves, warmup, seed, graph=True):
     """FP64 and FP32-V-cycle solves of the Dirichlet problem on octree ob."""
-    # the hierarchy is built twice: the first build in a process also pays
-    # the one-time start-up of rocSOLVER (coarse Cholesky)
     setups = []
-    for _ in range(2):
+    for _ in range(2):  # the first set-up in a process also pays one-time runtime costs
         t0 = time.perf_counter()
         mg = dgx.MG(ob, nlevels)
         dgx.synchronize()
         setups.append(time.perf_counter() - t0)
-    t_setup = setups[-1]
     n = mg.n_nodes
     f = dgx.DeviceArray.from_host(np.random.default_rng(seed).standard_normal(n))
     b = dgx.DeviceArray((n,), np.float64)
     mg.rhs(f, b)
     out = {"levels": mg.nlevels, "level_octants": mg.level_octants, "level_nodes": mg.level_nodes,
-           "mg_setup_s": t_setup, "mg_first_setup_s": setups[0]}
+           "coarse_solver": mg.coarse, "mg_setup_s": setups[-1], "mg_first_setup_s": setups[0]}
     sol = {}
     for name, fp32 in (("fp64", False), ("fp32_mg", True)):
         u = dgx.DeviceArray((n,), np.float64)
@@ -124,6 +150,55 @@ def solve_section(dgx, ob, nlevels, solves, warmup, seed, graph=True):
     return out
 
 
+def build_section(dgx, dpts, md, mnp, steps, warmup):
+    """Timed points2Octree + balanceOctree steps on device-resident points."""
+    for _ in range(warmup):
+        ob = dgx.balance_octree(dgx.points2octree(dpts, md, mnp))
+        del ob
+    dgx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ou = dgx.points2octree(dpts, md, mnp)
+        ob = dgx.balance_octree(ou)
+    dgx.synchronize()
+    return ou, ob, (time.perf_counter() - t0) / steps
+
+
+def large_section(dgx, ev, stream, args):
+    """gauss16-128M: build + balance, DA, matvec, 7-level MG and solves, peak memory."""
+    W = "gauss16-128M"
+    dgx.memory_info(reset_peak=True)
+    t0 = time.perf_counter()
+    pts = dgx.workload_points(W)
+    t_gen = time.perf_counter() - t0
+    N = len(pts)
+    dpts = dgx.DeviceArray.from_host(pts)
+    del pts
+    dgx.synchronize()
+    ou, ob, t_build = build_section(dgx, dpts, 30, dgx.workload_max_num_pts(W), 2, 1)
+    U, Bn = len(ou), len(ob)
+    mem_build = dgx.memory_info()
+    del ou, dpts
+    log(f"large: build+balance {t_build * 1e3:.1f} ms ({Bn} octants)")
+    t0 = time.perf_counter()
+    da = dgx.DA(ob)
+    dgx.synchronize()
+    t_da = time.perf_counter() - t0
+    mv = time_matvecs(dgx, ev, stream, da, 20)
+    del da
+    log(f"large: matvec {mv['avg_ms'] * 1e3:.0f} us, {mv['GB_per_s']:.0f} GB/s")
+    s = solve_section(dgx, ob, 7, 3, 1, 4, not args.no_graph)
+    mem = dgx.memory_info()
+    out = {"workload": "gauss16-128M: 134,217,728 points, 16 Gaussians (sigma log-U[0.005,0.05], centres "
+                       "U[0.1,0.9]^3, Dirichlet(1) weights), seed 4, maxDepth 30, maxNumPts 4",
+           "points": N, "generate_s": t_gen, "octants": U, "balanced_octants": Bn,
+           "build_balance_s": t_build, "octants_per_s": Bn / t_build, "da_build_s": t_da, "matvec": mv,
+           "solve": s,
+           "peak_device_bytes": max(mem["library_peak"], mem_build["library_peak"]),
+           "device_used_after_solves": mem["device_used"]}
+    return out
+
+
 def serve_section(dgx, count, slots, budget):
     """Served stream: `count` requests (log-uniform 2^12..2^22 points, three
     distributions, maxNumPts 1/4/8, seed 3) submitted at their Poisson arrival
@@ -132,6 +207,7 @@ def serve_section(dgx, count, slots, budget):
     t0 = time.perf_counter()
     clouds = [dgx.serve_points(r) for r in reqs]
     t_gen = time.perf_counter() - t0
+    log(f"serve: {count} point clouds generated in {t_gen:.1f} s")
     out = {"requests": count, "rate_per_s": 200.0, "slots": slots, "hbm_budget_GB": budget / 1e9,
            "points_total": int(sum(r["n"] for r in reqs)), "generate_s": t_gen}
     for name, fp32 in (("fp64", False), ("fp32_mg", True)):
@@ -148,7 +224,7 @@ def serve_section(dgx, count, slots, budget):
         res = []
         for k, i in enumerate(ids):
             res.append(sv.wait(i))
-            if (k + 1) % 100 == 0:
+            if (k + 1) % 250 == 0:
                 log(f"serve {name}: {k + 1}/{count} done at {time.perf_counter() - start:.1f} s")
         elapsed = time.perf_counter() - start
         sv.close()
@@ -158,6 +234,9 @@ def serve_section(dgx, count, slots, budget):
                      "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
                      "service_p50_ms": float(np.percentile(run, 50) * 1e3),
                      "service_p99_ms": float(np.percentile(run, 99) * 1e3),
+                     "octree_s": float(sum(x["s_octree"] for x in res)),
+                     "hierarchy_s": float(sum(x["s_hierarchy"] for x in res)),
+                     "solve_s": float(sum(x["s_solve"] for x in res)),
                      "max_rel_residual": float(max(x["rel_residual"] for x in res)),
                      "iterations_max": int(max(x["iterations"] for x in res)),
                      "octants_total": int(sum(x["octants"] for x in res))}
@@ -165,6 +244,43 @@ def serve_section(dgx, count, slots, budget):
     out["speedup_fp32_mg_vs_fp64"] = out["fp32_mg"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     del clouds
     return out
+
+
+def cpu_section(pts, md, mnp):
+    from oracle import pyoracle as O
+    if not os.path.exists(O.LIB_PATH):
+        import __graft_entry__
+        __graft_entry__.build_oracle()
+    threads = O.num_threads()
+    t0 = time.perf_counter()
+    ru = O.points2octree(pts, md, mnp)
+    rb = O.balance(ru)
+    tc = time.perf_counter() - t0
+    return {"value": len(rb) / tc, "unit": "octants/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
+            "sample": f"oracle points2Octree+balanceOctree (OpenMP merge sorts) on all {len(pts)} points of the "
+                      f"lognormal-4M cloud ({len(rb)} balanced octants, {tc:.2f} s, {threads} threads; "
+                      f"host reports {os.cpu_count()} CPUs)"}
+
+
+def matvec_only(dgx, ev, stream, W):
+    """Counter runs: build workload W's DA, run 20 matvecs and three 256 MiB
+    device copies (the copy scales FETCH_SIZE / WRITE_SIZE), print one line."""
+    dpts = dgx.DeviceArray.from_host(dgx.workload_points(W))
+    ob = dgx.balance_octree(dgx.points2octree(dpts, 30, dgx.workload_max_num_pts(W)))
+    del dpts
+    da = dgx.DA(ob)
+    mv = time_matvecs(dgx, ev, stream, da, 20)
+    nb = 256 << 20
+    src = dgx.DeviceArray((nb // 8,), np.float64)
+    dst = dgx.DeviceArray((nb // 8,), np.float64)
+    hip = ev.hip
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    for _ in range(3):
+        assert hip.hipMemcpyAsync(dst.ptr, src.ptr, nb, 3, stream) == 0  # hipMemcpyDeviceToDevice
+    dgx.synchronize()
+    print(json.dumps({"workload": W, "matvec_avg_ms": mv["avg_ms"], "algorithmic_bytes": mv["algorithmic_bytes"],
+                      "GB_per_s": mv["GB_per_s"], "copy_bytes": nb}))
 
 
 def main():
@@ -175,15 +291,14 @@ def main():
     ap.add_argument("--matvecs", type=int, default=100)
     ap.add_argument("--solves", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true", help="skip the Poisson solves")
-    ap.add_argument("--no-large", action="store_true", help="skip the gauss4-16M solve")
+    ap.add_argument("--no-large", action="store_true", help="skip gauss16-128M")
     ap.add_argument("--no-graph", action="store_true", help="solve without hipGraph replay (for tracing runs)")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served stream, a prefix of the 2,000-request stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=0, help="points in the oracle's CPU sample (0: all)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--matvec-only", action="store_true",
-                    help="build once, run the matvecs and a 256 MiB device copy (for counter runs), print nothing else")
+    ap.add_argument("--matvec-only", metavar="WORKLOAD", default=None,
+                    help="build the workload's DA, run matvecs and a 256 MiB device copy (counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -194,6 +309,9 @@ def main():
     dgx.init(0)
     stream = dgx.stream()
     ev = HipEvents()
+    if args.matvec_only:
+        matvec_only(dgx, ev, stream, args.matvec_only)
+        return
 
     W = "lognormal-4M"
     MD, MP = 30, dgx.workload_max_num_pts(W)
@@ -201,117 +319,58 @@ def main():
     N = len(pts)
     dpts = dgx.DeviceArray.from_host(pts)
     dgx.synchronize()
-
-    def step():
-        ou = dgx.points2octree(dpts, MD, MP)
-        ob = dgx.balance_octree(ou)
-        return ou, ob
-
     for _ in range(args.warmup):
-        step()
+        dgx.balance_octree(dgx.points2octree(dpts, MD, MP))
     dgx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ou = dgx.points2octree(dpts, MD, MP)
     dgx.synchronize()
     t_p2o = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ou, ob = step()
-    dgx.synchronize()
-    t_build = (time.perf_counter() - t0) / args.steps
+    ou, ob, t_build = build_section(dgx, dpts, MD, MP, args.steps, 0)
     U, Bn = len(ou), len(ob)
     octants_per_s = Bn / t_build
     build_bytes = 24 * N + 2 * 28 * U + 28 * Bn
     log(f"build+balance {t_build * 1e3:.2f} ms/step ({Bn} octants)")
 
-    # DA + FP64 matvec
     da = dgx.DA(ob)
     E, Nn = da.element_size, da.n_nodes
-    x = dgx.DeviceArray.from_host(np.random.default_rng(1).standard_normal(Nn))
-    y = dgx.DeviceArray((Nn,), np.float64)
-    for _ in range(5):
-        da.matvec(x, y)
-    dgx.synchronize()
-    evs = [(ev.event(), ev.event()) for _ in range(args.matvecs)]
-    for a, b in evs:
-        ev.record(a, stream)
-        da.matvec(x, y)
-        ev.record(b, stream)
-    dgx.synchronize()
-    mv_ms = [ev.elapsed_ms(a, b) for a, b in evs]
-    # back-to-back launches: one event pair around the whole run
-    s0, s1 = ev.event(), ev.event()
-    ev.record(s0, stream)
-    for _ in range(args.matvecs):
-        da.matvec(x, y)
-    ev.record(s1, stream)
-    dgx.synchronize()
-    # average launch duration = GPU time per matvec when matvecs run back to
-    # back (the host enqueues ahead); the per-launch event pairs above also
-    # hold the event overhead and any host gap before each launch
-    mv_avg_ms = ev.elapsed_ms(s0, s1) / args.matvecs
-    mv_bracket_ms = float(np.mean(mv_ms))
-    log(f"matvec per-launch events {mv_bracket_ms * 1e3:.1f} us, back-to-back {mv_avg_ms * 1e3:.1f} us")
-    mv_bytes = 34 * E + 16 * Nn
-    achieved = mv_bytes / (mv_avg_ms * 1e-3) / 1e9
-    log(f"matvec {mv_avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
-    block_info = da.block_info
-    del da, x, y
-    if args.matvec_only:
-        # reference copy kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE counters
-        nb = 256 << 20
-        src = dgx.DeviceArray((nb // 8,), np.float64)
-        dst = dgx.DeviceArray((nb // 8,), np.float64)
-        hip = ev.hip
-        hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
-        for _ in range(3):
-            assert hip.hipMemcpyAsync(dst.ptr, src.ptr, nb, 3, stream) == 0  # hipMemcpyDeviceToDevice
-        dgx.synchronize()
-        print(json.dumps({"matvec_avg_ms": mv_avg_ms, "algorithmic_bytes": mv_bytes, "GB_per_s": achieved,
-                          "copy_bytes": nb}))
-        return
+    mv4 = time_matvecs(dgx, ev, stream, da, args.matvecs)
+    del da
+    log(f"matvec lognormal-4M {mv4['avg_ms'] * 1e3:.1f} us, {mv4['GB_per_s']:.0f} GB/s")
 
     solve = None
     if not args.no_solve:
-        solve = {"lognormal-4M": solve_section(dgx, ob, 5, args.solves, 2, 2, not args.no_graph)}
-        log(f"solve lognormal-4M: {json.dumps(solve['lognormal-4M'])}")
-        if not args.no_large:
-            del ou, ob, dpts
-            W3 = "gauss4-16M"
-            p3 = dgx.DeviceArray.from_host(dgx.workload_points(W3))
-            dgx.synchronize()
-            t0 = time.perf_counter()
-            o3 = dgx.balance_octree(dgx.points2octree(p3, MD, dgx.workload_max_num_pts(W3)))
-            dgx.synchronize()
-            t3 = time.perf_counter() - t0
-            del p3
-            s3 = solve_section(dgx, o3, 5, max(5, args.solves // 2), 2, 2, not args.no_graph)
-            s3["build_balance_s"] = t3
-            s3["balanced_octants"] = len(o3)
-            solve[W3] = s3
-            log(f"solve {W3}: {json.dumps(s3)}")
-            del o3
+        solve = {W: solve_section(dgx, ob, 5, args.solves, 2, 2, not args.no_graph)}
+        log(f"solve {W}: {json.dumps(solve[W])}")
+    del ou, ob, dpts
+
+    # gauss4-16M: the roofline matvec (628 MB per launch) and the solve of config 3
+    W3 = "gauss4-16M"
+    p3 = dgx.DeviceArray.from_host(dgx.workload_points(W3))
+    dgx.synchronize()
+    o3u, o3, t3 = build_section(dgx, p3, MD, dgx.workload_max_num_pts(W3), 2, 1)
+    del p3, o3u
+    da3 = dgx.DA(o3)
+    mv16 = time_matvecs(dgx, ev, stream, da3, args.matvecs)
+    del da3
+    log(f"matvec {W3} {mv16['avg_ms'] * 1e3:.1f} us, {mv16['GB_per_s']:.0f} GB/s")
+    g16 = {"build_balance_s": t3, "balanced_octants": len(o3), "octants_per_s": len(o3) / t3, "matvec": mv16}
+    if not args.no_solve:
+        g16["solve"] = solve_section(dgx, o3, 5, max(5, args.solves // 2), 2, 2, not args.no_graph)
+        log(f"solve {W3}: {json.dumps(g16['solve'])}")
+    del o3
+
+    large = None
+    if not args.no_large:
+        large = large_section(dgx, ev, stream, args)
+        log(f"large: {json.dumps(large)}")
 
     serve = None
     if args.serve_requests > 0:
         serve = serve_section(dgx, args.serve_requests, args.serve_slots, 200e9)
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import pyoracle as O
-        if not os.path.exists(O.LIB_PATH):
-            import __graft_entry__
-            __graft_entry__.build_oracle()
-        sub = pts[: args.cpu_sample] if args.cpu_sample > 0 else pts
-        t0 = time.perf_counter()
-        ru = O.points2octree(sub, MD, MP)
-        rb = O.balance(ru)
-        tc = time.perf_counter() - t0
-        which = "all" if len(sub) == len(pts) else "the first"
-        cpu = {"value": len(rb) / tc, "unit": "octants/s", "cores": 1, "kind": "port",
-               "sample": f"oracle points2Octree+balanceOctree on {which} {len(sub)} points of the lognormal-4M "
-                         f"cloud ({len(rb)} balanced octants, {tc:.1f} s, 1 thread)"}
+    cpu = None if args.no_cpu else cpu_section(pts, MD, MP)
 
     out = {
         "metric": "octants/s (build+2:1 balance); Poisson solves/s & p99 latency, FP32-MG vs FP64",
@@ -327,19 +386,20 @@ def main():
         "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": "lognormal-4M: points2Octree + balanceOctree(corner) of 4,194,304 points, "
-                               "coordinates exp(N(-1,0.5)) clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8; "
-                               "100 FP64 -lap+I matvecs on its DA",
+                               "coordinates exp(N(-1,0.5)) clipped to [0,1), seed 1, maxDepth 30, maxNumPts 8",
                    "points": N, "octants": U, "balanced_octants": Bn, "elements": E, "nodes": Nn},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
-                     "kernel": "FP64 -lap+I matvec: k_matvec_pipe<double,false,4,0> + k_matvec_shared<double,false,0>",
-                     "algorithmic_bytes": mv_bytes, "avg_launch_ms": mv_avg_ms},
-        "matvec": {"da_blocks": block_info, "avg_ms": mv_avg_ms, "bracketed_avg_ms": mv_bracket_ms,
-                   "bracketed_min_ms": float(np.min(mv_ms)),
-                   "GB_per_s": achieved, "flops": 332 * E, "GFLOP_per_s": 332 * E / (mv_avg_ms * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "achieved": mv16["GB_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": mv16["GB_per_s"] / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": f"FP64 -lap+I matvec on the gauss4-16M DA: {MATVEC_KERNEL}",
+                     "algorithmic_bytes": mv16["algorithmic_bytes"], "avg_launch_ms": mv16["avg_ms"]},
         "build_balance": {"s_per_step": t_build, "p2o_s": t_p2o, "algorithmic_bytes": build_bytes,
-                          "GB_per_s": build_bytes / t_build / 1e9},
+                          "roofline": {"bound": "hbm", "achieved": build_bytes / t_build / 1e9,
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": build_bytes / t_build / 1e9 / HBM_PEAK_GBS}},
+        "matvec": mv4,
         "solve": solve,
+        "gauss4-16M": g16,
+        "large": large,
         "serve": serve,
         "cpu_baseline": cpu,
     }

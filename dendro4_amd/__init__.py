@@ -39,7 +39,8 @@ EXPORTED = [
     "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
     "dgx_write_nodes_binary", "dgx_write_vtk", "dgx_server_get_stats", "dgx_server_inject_fault",
     "dgx_mg_coarse_info", "dgx_direct_plan_host", "dgx_direct_plan_export", "dgx_direct_plan_free",
-    "dgx_da_matvec_noalloc",
+    "dgx_da_matvec_noalloc", "dgx_memory_info", "dgx_mg_level_da", "dgx_da_boundary_flags",
+    "dgx_da_compress_maps", "dgx_da_decode_maps",
 ]
 
 
@@ -133,6 +134,11 @@ def load_library():
         "dgx_server_free": ([vp], i32),
         "dgx_server_get_stats": ([vp, C.POINTER(ServerStats)], i32),
         "dgx_mg_coarse_info": ([vp, vp], i32),
+        "dgx_memory_info": ([vp, i32], i32),
+        "dgx_mg_level_da": ([vp, i32, C.POINTER(vp)], i32),
+        "dgx_da_boundary_flags": ([vp, vp], i32),
+        "dgx_da_compress_maps": ([vp, vp, vp, vp, vp, sz, C.POINTER(sz)], i32),
+        "dgx_da_decode_maps": ([vp, vp, vp, vp, vp, sz, vp], i32),
         "dgx_direct_plan_host": ([u32, u64, vp, vp, vp, vp, u32, u32, C.POINTER(vp), C.POINTER(DirectSizes)], i32),
         "dgx_direct_plan_export": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
         "dgx_direct_plan_free": ([vp], i32),
@@ -200,6 +206,14 @@ def _lib_ready():
 
 def stream():
     return _lib_ready().dgx_stream()
+
+
+def memory_info(reset_peak=False):
+    """Device memory: total, used on the device, the library's live bytes, their
+    peak (reset after reading when reset_peak), bytes in the block cache."""
+    out = (C.c_uint64 * 5)()
+    _check(_lib_ready().dgx_memory_info(out, 1 if reset_peak else 0))
+    return dict(zip(("total", "device_used", "library_live", "library_peak", "cached"), list(out)))
 
 
 def synchronize():
@@ -380,6 +394,34 @@ class DA:
         dy = DeviceArray((self.n_nodes,), dt)
         self.matvec(dx, dy, lap_fac, mass_fac)
         return dy.to_host()
+
+    def compressed_maps(self):
+        """compressLut maps encoded on the device (dgx_da_compress_maps):
+        remainders (E, 8) u8, quotient masks (E,), sort orders (E,), quotients u16."""
+        E = self.element_size
+        nq = C.c_size_t()
+        _check(_lib.dgx_da_compress_maps(self._h, None, None, None, None, 0, C.byref(nq)))
+        rem = np.zeros((E, 8), np.uint8)
+        qm = np.zeros(E, np.uint8)
+        order = np.zeros(E, np.uint8)
+        quot = np.zeros(max(nq.value, 1), np.uint16)
+        _check(_lib.dgx_da_compress_maps(self._h, _ptr(rem), _ptr(qm), _ptr(order), _ptr(quot), len(quot),
+                                         C.byref(nq)))
+        return rem, qm, order, quot[: nq.value]
+
+    def decode_maps(self, rem, qmask, order, quot):
+        """DA::getNodeIndices over compressed maps, on the device (dgx_da_decode_maps)."""
+        arrs = [np.ascontiguousarray(a) for a in (rem, qmask, order, np.asarray(quot, np.uint16))]
+        out = np.zeros((self.element_size, 8), np.uint32)
+        _check(_lib.dgx_da_decode_maps(self._h, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]), _ptr(arrs[3]),
+                                       len(arrs[3]), _ptr(out)))
+        return out
+
+    def boundary_flags(self):
+        """Dirichlet flags per local-buffer octant (dgx_da_boundary_flags)."""
+        out = np.zeros(self.local_buffer_size, np.uint8)
+        _check(_lib.dgx_da_boundary_flags(self._h, _ptr(out)))
+        return out
 
     def diagonal_host(self, lap_fac=1.0, mass_fac=1.0):
         dd = DeviceArray((self.n_nodes,), np.float64)

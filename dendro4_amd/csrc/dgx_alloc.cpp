@@ -15,6 +15,7 @@
 // frees a block, so a caller can measure what one piece of work holds.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -43,6 +44,7 @@ std::unordered_map<void *, Live> g_live;
 std::unordered_map<uint64_t, TagUse> g_tags;  // tag 0 (untagged) is not tracked
 std::multimap<size_t, Cached> g_cache;        // class bytes -> free block
 size_t g_cached_bytes = 0;
+size_t g_live_bytes = 0, g_live_peak = 0;     // bytes handed out and not freed, and their peak
 constexpr size_t kCacheCap = size_t(8) << 30;  // bytes held back from the pool at most
 
 std::unordered_map<hipStream_t, int> g_capturing;   // streams inside a graph capture
@@ -58,6 +60,8 @@ size_t size_class(size_t b) {
 
 void note_alloc_locked(void *p, size_t cls, hipStream_t s) {
     g_live[p] = Live{cls, s, t_tag};
+    g_live_bytes += cls;
+    g_live_peak = std::max(g_live_peak, g_live_bytes);
     if (t_tag) {
         TagUse &u = g_tags[t_tag];
         u.live += (int64_t)cls;
@@ -143,6 +147,7 @@ void dgx_free(void *p, hipStream_t s) {
         return;
     }
     const size_t cls = it->second.bytes;
+    g_live_bytes -= cls;
     const hipStream_t as = t_free_override ? t_free_override : it->second.s;
     if (it->second.tag) {
         auto t = g_tags.find(it->second.tag);
@@ -171,6 +176,13 @@ void dgx_alloc_trim() {
 size_t dgx_alloc_cached_bytes() {
     std::lock_guard<std::mutex> lk(g_mu);
     return g_cached_bytes;
+}
+
+void dgx_alloc_live(size_t *live, size_t *peak, int reset_peak) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (live) *live = g_live_bytes;
+    if (peak) *peak = g_live_peak;
+    if (reset_peak) g_live_peak = g_live_bytes;
 }
 
 void dgx_alloc_release_stream(hipStream_t s) {

@@ -83,6 +83,7 @@ struct dgx_da_s {
 struct dgx_mg_s {
     std::unique_ptr<MgHierarchy> mg;
     dgx_da_s finest;
+    std::vector<std::unique_ptr<dgx_da_s>> levels;  // borrowed views of every level (dgx_mg_level_da)
 };
 
 extern "C" {
@@ -143,6 +144,21 @@ int dgx_synchronize(void) {
     API_BEGIN
     ensure_ctx();
     HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+
+int dgx_memory_info(uint64_t out[5], int reset_peak) {
+    API_BEGIN
+    ensure_ctx();
+    if (!out) { dgx_set_error("memory info: null argument"); return -1; }
+    size_t fr = 0, tot = 0, live = 0, peak = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    dgx_alloc_live(&live, &peak, reset_peak);
+    out[0] = tot;
+    out[1] = tot - fr;
+    out[2] = live;
+    out[3] = peak;
+    out[4] = dgx_alloc_cached_bytes();
     API_END
 }
 
@@ -358,6 +374,81 @@ int dgx_da_export_anchors(dgx_da d, uint32_t *xyz) {
     API_END
 }
 
+int dgx_da_boundary_flags(dgx_da d, uint8_t *flags) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !flags) { dgx_set_error("null argument"); return -1; }
+    const DaDevice &a = d->get();
+    std::vector<uint32_t> x(a.n_local), y(a.n_local), z(a.n_local), l2n(a.n_local);
+    if (a.n_local) {
+        HIP_OK(hipMemcpyAsync(x.data(), a.x.p, a.n_local * 4, hipMemcpyDeviceToHost, S()));
+        HIP_OK(hipMemcpyAsync(y.data(), a.y.p, a.n_local * 4, hipMemcpyDeviceToHost, S()));
+        HIP_OK(hipMemcpyAsync(z.data(), a.z.p, a.n_local * 4, hipMemcpyDeviceToHost, S()));
+        HIP_OK(hipMemcpyAsync(l2n.data(), a.l2n.p, a.n_local * 4, hipMemcpyDeviceToHost, S()));
+    }
+    HIP_OK(hipStreamSynchronize(S()));
+    // the domain is [0, 2^(md-1)] in the embedded coordinates
+    const uint32_t top = 1u << (a.md - 1);
+    for (size_t i = 0; i < a.n_local; ++i) {
+        if (l2n[i] == 0xffffffffu) {
+            flags[i] = 0;
+            continue;
+        }
+        flags[i] = (uint8_t)((x[i] == 0 || x[i] == top) + (y[i] == 0 || y[i] == top) + (z[i] == 0 || z[i] == top));
+    }
+    API_END
+}
+
+int dgx_da_compress_maps(dgx_da d, uint8_t *rem, uint8_t *qmask, uint8_t *order, uint16_t *quot, size_t quot_cap,
+                         size_t *n_quot) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !n_quot) { dgx_set_error("null argument"); return -1; }
+    const DaDevice &a = d->get();
+    LutMaps m;
+    const size_t nq = da_lut_compress(a, m, S());
+    *n_quot = nq;
+    if (quot && quot_cap < nq) {
+        dgx_set_error("compressed maps: %zu quotients do not fit a buffer of %zu", nq, quot_cap);
+        return -1;
+    }
+    const size_t ne = a.n_elem;
+    if (rem && ne) HIP_OK(hipMemcpyAsync(rem, m.rem.p, 8 * ne, hipMemcpyDeviceToHost, S()));
+    if (qmask && ne) HIP_OK(hipMemcpyAsync(qmask, m.qmask.p, ne, hipMemcpyDeviceToHost, S()));
+    if (order && ne) HIP_OK(hipMemcpyAsync(order, m.order.p, ne, hipMemcpyDeviceToHost, S()));
+    if (quot && nq) HIP_OK(hipMemcpyAsync(quot, m.quot.p, 2 * nq, hipMemcpyDeviceToHost, S()));
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+int dgx_da_decode_maps(dgx_da d, const uint8_t *rem, const uint8_t *qmask, const uint8_t *order,
+                       const uint16_t *quot, size_t n_quot, uint32_t *nlist) {
+    API_BEGIN
+    ensure_ctx();
+    if (!d || !rem || !qmask || !order || !nlist || (n_quot && !quot)) {
+        dgx_set_error("null argument");
+        return -1;
+    }
+    const DaDevice &a = d->get();
+    const size_t ne = a.n_elem;
+    LutMaps m;
+    m.rem.alloc(8 * ne, S());
+    m.qmask.alloc(ne, S());
+    m.order.alloc(ne, S());
+    m.quot.alloc(std::max<size_t>(n_quot, 1), S());
+    m.nq = n_quot;
+    DevBuf<uint32_t> out(8 * ne, S());
+    if (ne) {
+        HIP_OK(hipMemcpyAsync(m.rem.p, rem, 8 * ne, hipMemcpyHostToDevice, S()));
+        HIP_OK(hipMemcpyAsync(m.qmask.p, qmask, ne, hipMemcpyHostToDevice, S()));
+        HIP_OK(hipMemcpyAsync(m.order.p, order, ne, hipMemcpyHostToDevice, S()));
+    }
+    if (n_quot) HIP_OK(hipMemcpyAsync(m.quot.p, quot, 2 * n_quot, hipMemcpyHostToDevice, S()));
+    da_lut_decode(a, m, out.p, S());
+    if (ne) HIP_OK(hipMemcpyAsync(nlist, out.p, 32 * ne, hipMemcpyDeviceToHost, S()));
+    HIP_OK(hipStreamSynchronize(S()));
+    API_END
+}
+
 int dgx_da_matvec(dgx_da d, const double *x, double *y, double lap_fac, double mass_fac) {
     API_BEGIN
     ensure_ctx();
@@ -431,6 +522,18 @@ int dgx_mg_coarse_info(dgx_mg m, uint64_t out[6]) {
     out[3] = c.plan.max_m;
     out[4] = c.plan.max_s;
     out[5] = c.plan.front_entries;
+    return 0;
+}
+int dgx_mg_level_da(dgx_mg m, int level, dgx_da *da) {
+    if (!m || !da) { dgx_set_error("null argument"); return -1; }
+    const int nl = (int)m->mg->lev.size();
+    if (level < 0 || level >= nl) { dgx_set_error("DAMG: level %d out of range (%d levels)", level, nl); return -1; }
+    if (m->levels.empty())
+        for (int i = 0; i < nl; ++i) {
+            m->levels.emplace_back(new dgx_da_s());
+            m->levels.back()->borrowed = m->mg->lev[i].da.get();
+        }
+    *da = m->levels[level].get();
     return 0;
 }
 int dgx_mg_finest_da(dgx_mg m, dgx_da *da) {

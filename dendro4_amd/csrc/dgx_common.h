@@ -341,6 +341,9 @@ void *dgx_alloc(size_t bytes, hipStream_t s);
 void dgx_free(void *p, hipStream_t s);
 void dgx_alloc_trim();            // return every cached block to the pool
 size_t dgx_alloc_cached_bytes();  // bytes held in the cache
+// Bytes of the library's device allocations in use, and their peak since the
+// process started or the last reset.
+void dgx_alloc_live(size_t *live, size_t *peak, int reset_peak);
 // Return the cached blocks of stream s to the pool before s is destroyed
 // (no live allocation may still be tagged with s).
 void dgx_alloc_release_stream(hipStream_t s);

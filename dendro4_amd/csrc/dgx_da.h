@@ -80,6 +80,21 @@ void da_dirichlet_jacobi(const DaDevice &da, T *x, const T *b, const T *dinv, T 
                          T *scratch, hipStream_t s);
 template <typename T>
 void da_dirichlet_residual(const DaDevice &da, const T *x, const T *b, T *r, T lap_fac, T mass_fac, hipStream_t s);
+// Compressed element-to-node maps in the reference's form (compressLut,
+// odaBuildNlist.cpp:3120-3262): 8 remainders, a quotient mask and a sort
+// order per element, the quotients in element order, and (derived here) each
+// element's first quotient.
+struct LutMaps {
+    DevBuf<uint8_t> rem, qmask, order;
+    DevBuf<uint32_t> qoff;
+    DevBuf<uint16_t> quot;
+    size_t nq = 0;
+};
+// Encodes da's nlist; returns the number of quotients.
+size_t da_lut_compress(const DaDevice &da, LutMaps &m, hipStream_t s);
+// DA::getNodeIndices over compressed maps (oda.tcc:1304-3950) for every
+// element: nlist (device, 8 per element) receives the decoded indices.
+void da_lut_decode(const DaDevice &da, LutMaps &m, uint32_t *nlist, hipStream_t s);
 // Element matrices of lap_fac K + mass_fac M in each element's child frame:
 // vals[e*64 + k*8 + j] is the contribution to A[id[j^c], id[k^c]], id the
 // element's 8 node ids (conn) and c its child number (meta).

@@ -7,9 +7,11 @@
 #include "dgx_direct.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <thread>
 
 #include "../../include/dendro_mi355x.h"
 
@@ -131,6 +133,23 @@ struct Dissection {
     }
 };
 
+// Runs body(lo, hi, t) over [0, n) in up to 8 contiguous ranges on host
+// threads; each range writes only its own outputs, so the result does not
+// depend on the split.
+template <typename F>
+void parallel_ranges(size_t n, size_t grain, F body) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned t = (unsigned)std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n / grain));
+    if (t <= 1) {
+        body(size_t(0), n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < t; ++k) th.emplace_back(body, n * k / t, n * (k + 1) / t, k);
+    body(size_t(0), n / t, 0u);
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta, const uint8_t *bdy,
@@ -173,26 +192,30 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
     // matrix pattern over the interior rows (diagonal included), columns in
     // the order the incident elements first reach them
     std::vector<uint32_t> roff(ni + 1, 0), rcol;
-    rcol.reserve((size_t)ni * 27);
     {
-        std::vector<uint32_t> mark(ni, 0xffffffffu), row;
-        for (uint32_t i = 0; i < ni; ++i) {
-            row.clear();
-            for (uint32_t q = ioff[i]; q < ioff[i + 1]; ++q) {
-                const int32_t *ids = &eid[(inc[q] >> 3) * 8];
-                for (int j = 0; j < 8; ++j) {
-                    const int32_t w = ids[j];
-                    if (w >= 0 && mark[w] != i) {
-                        mark[w] = i;
-                        row.push_back((uint32_t)w);
+        std::vector<std::vector<uint32_t>> part(8);
+        parallel_ranges(ni, 4096, [&](size_t lo, size_t hi, unsigned t) {
+            std::vector<uint32_t> mark(ni, 0xffffffffu), &out = part[t];
+            out.reserve((hi - lo) * 27);
+            for (uint32_t i = (uint32_t)lo; i < hi; ++i) {
+                const size_t before = out.size();
+                for (uint32_t q = ioff[i]; q < ioff[i + 1]; ++q) {
+                    const int32_t *ids = &eid[(inc[q] >> 3) * 8];
+                    for (int j = 0; j < 8; ++j) {
+                        const int32_t w = ids[j];
+                        if (w >= 0 && mark[w] != i) {
+                            mark[w] = i;
+                            out.push_back((uint32_t)w);
+                        }
                     }
                 }
+                if (out.size() == before) out.push_back(i);  // an interior node of no element keeps its diagonal
+                roff[i + 1] = (uint32_t)(out.size() - before);
             }
-            if (row.empty()) row.push_back(i);  // an interior node of no element keeps its diagonal
-            roff[i] = (uint32_t)rcol.size();
-            rcol.insert(rcol.end(), row.begin(), row.end());
-        }
-        roff[ni] = (uint32_t)rcol.size();
+        });
+        for (uint32_t i = 0; i < ni; ++i) roff[i + 1] += roff[i];
+        rcol.reserve(roff[ni]);
+        for (auto &pv : part) rcol.insert(rcol.end(), pv.begin(), pv.end());
     }
     const uint32_t nnz = roff[ni];
     std::vector<uint32_t> cxyz(3 * (size_t)ni);
@@ -317,64 +340,83 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
             invo += F.m;
             fmo += (uint32_t)uc.size();
         }
-        // The entries this front owns: (i, j) whose first-eliminated node is in
-        // S.  They lie in the rows of S (columns eliminated at S or later) and
-        // in the rows of U (columns in S).
-        auto put = [&](uint32_t k, uint32_t pi, uint32_t pj) {
-            const uint32_t a = pm[pi], b = pm[pj];
-            const bool ok = b < F.m && (b < F.s ? sstart[f] + b : U[f][b - F.s]) == pj;
-            if (!ok) {
-                dgx_set_error("coarse solver: a matrix entry falls outside its front");
-                throw DgxError();
-            }
-            P.a_dst[k] = F.off + (uint64_t)a * F.m + b;
-        };
-        for (uint32_t v : D.S[f])
-            for (uint32_t k = roff[v]; k < roff[v + 1]; ++k) {
-                const uint32_t pj = D.pos[rcol[k]];
-                if (pj >= sstart[f]) put(k, D.pos[v], pj);
-            }
-        for (uint32_t pu : U[f]) {
-            const uint32_t u = at_pos[pu];
-            for (uint32_t k = roff[u]; k < roff[u + 1]; ++k) {
-                const uint32_t pj = D.pos[rcol[k]];
-                if (pj >= sstart[f] && pj < send[f]) put(k, pu, pj);
+    }
+    // The entries each front owns: (i, j) whose first-eliminated node is in S.
+    // They lie in the rows of S (columns eliminated at S or later) and in the
+    // rows of U (columns in S).  Fronts are independent: each range of fronts
+    // keeps its own position map.
+    std::atomic<bool> outside{false};
+    parallel_ranges(nf, 16, [&](size_t lo, size_t hi, unsigned) {
+        std::vector<uint32_t> lpm(ni, 0xffffffffu);
+        for (size_t f = lo; f < hi; ++f) {
+            const MfFront &F = P.fronts[f];
+            for (uint32_t p = 0; p < F.s; ++p) lpm[sstart[f] + p] = p;
+            for (uint32_t p = 0; p < (uint32_t)U[f].size(); ++p) lpm[U[f][p]] = F.s + p;
+            auto put = [&](uint32_t k, uint32_t pi, uint32_t pj) {
+                const uint32_t a = lpm[pi], b = lpm[pj];
+                if (b >= F.m || (b < F.s ? sstart[f] + b : U[f][b - F.s]) != pj) {
+                    outside = true;
+                    return;
+                }
+                P.a_dst[k] = F.off + (uint64_t)a * F.m + b;
+            };
+            for (uint32_t v : D.S[f])
+                for (uint32_t k = roff[v]; k < roff[v + 1]; ++k) {
+                    const uint32_t pj = D.pos[rcol[k]];
+                    if (pj >= sstart[f]) put(k, D.pos[v], pj);
+                }
+            for (uint32_t pu : U[f]) {
+                const uint32_t u = at_pos[pu];
+                for (uint32_t k = roff[u]; k < roff[u + 1]; ++k) {
+                    const uint32_t pj = D.pos[rcol[k]];
+                    if (pj >= sstart[f] && pj < send[f]) put(k, pu, pj);
+                }
             }
         }
+    });
+    if (outside) {
+        dgx_set_error("coarse solver: a matrix entry falls outside its front");
+        throw DgxError();
     }
     // element-matrix contributions of each entry, in element order: row i's
     // entries come from its incident elements (in element order), each
     // element adding vals[e*64 + k*8 + j] to (ids[j^c], ids[k^c])
     P.a_off.assign(nnz + 1, 0);
-    P.a_src.clear();
-    P.a_src.reserve(ioff[ni] * 8);
     {
-        std::vector<uint32_t> slot(ni, 0), cnt(32), start(32);
-        std::vector<std::pair<uint32_t, uint32_t>> buf;  // (entry in row, contribution)
-        for (uint32_t i = 0; i < ni; ++i) {
-            const uint32_t r0 = roff[i], rn = roff[i + 1] - r0;
-            for (uint32_t k = r0; k < roff[i + 1]; ++k) slot[rcol[k]] = k - r0;
-            cnt.assign(rn + 1, 0);
-            buf.clear();
-            for (uint32_t q = ioff[i]; q < ioff[i + 1]; ++q) {
-                const uint32_t e = inc[q] >> 3, jj = inc[q] & 7u;
-                const uint32_t c = (meta[e] >> 8) & 7u, j = jj ^ c;  // row i is child-frame vertex j
-                const int32_t *ids = &eid[8 * (size_t)e];
-                for (uint32_t kk = 0; kk < 8; ++kk) {
-                    const int32_t col = ids[kk];
-                    if (col < 0) continue;
-                    const uint32_t k = kk ^ c, a = slot[col];
-                    buf.push_back({a, (uint32_t)(e * 64 + k * 8 + j)});
-                    ++cnt[a + 1];
+        std::vector<std::vector<uint32_t>> part(8);
+        parallel_ranges(ni, 4096, [&](size_t lo, size_t hi, unsigned t) {
+            std::vector<uint32_t> slot(ni, 0), cnt, start, &out = part[t];
+            std::vector<std::pair<uint32_t, uint32_t>> buf;  // (entry in row, contribution)
+            out.reserve((size_t)(ioff[hi] - ioff[lo]) * 8);
+            for (uint32_t i = (uint32_t)lo; i < hi; ++i) {
+                const uint32_t r0 = roff[i], rn = roff[i + 1] - r0;
+                for (uint32_t k = r0; k < roff[i + 1]; ++k) slot[rcol[k]] = k - r0;
+                cnt.assign(rn + 1, 0);
+                buf.clear();
+                for (uint32_t q = ioff[i]; q < ioff[i + 1]; ++q) {
+                    const uint32_t e = inc[q] >> 3, jj = inc[q] & 7u;
+                    const uint32_t c = (meta[e] >> 8) & 7u, j = jj ^ c;  // row i is child-frame vertex j
+                    const int32_t *ids = &eid[8 * (size_t)e];
+                    for (uint32_t kk = 0; kk < 8; ++kk) {
+                        const int32_t col = ids[kk];
+                        if (col < 0) continue;
+                        const uint32_t k = kk ^ c, a = slot[col];
+                        buf.push_back({a, (uint32_t)(e * 64 + k * 8 + j)});
+                        ++cnt[a + 1];
+                    }
                 }
+                for (uint32_t a = 0; a < rn; ++a) cnt[a + 1] += cnt[a];
+                const size_t base = out.size();
+                out.resize(base + buf.size());
+                start.assign(cnt.begin(), cnt.end());
+                for (const auto &pr : buf) out[base + start[pr.first]++] = pr.second;  // stable: element order
+                for (uint32_t a = 0; a < rn; ++a) P.a_off[r0 + a + 1] = cnt[a + 1] - cnt[a];
             }
-            for (uint32_t a = 0; a < rn; ++a) cnt[a + 1] += cnt[a];
-            const size_t base = P.a_src.size();
-            P.a_src.resize(base + buf.size());
-            start.assign(cnt.begin(), cnt.end());
-            for (const auto &pr : buf) P.a_src[base + start[pr.first]++] = pr.second;  // stable: element order
-            for (uint32_t a = 0; a < rn; ++a) P.a_off[r0 + a + 1] = (uint32_t)(base + cnt[a + 1]);
-        }
+        });
+        for (uint32_t k = 0; k < nnz; ++k) P.a_off[k + 1] += P.a_off[k];
+        P.a_src.clear();
+        P.a_src.reserve(P.a_off[nnz]);
+        for (auto &pv : part) P.a_src.insert(P.a_src.end(), pv.begin(), pv.end());
     }
     // levels by height, with their work items and scratch
     uint32_t H = 0;
@@ -587,6 +629,40 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Each wave of a 16-row work item takes 4 consecutive rows: acc[q] = row
+// (rb + q) . v over columns [0, ncol), rows of stride ld starting at A.  A lane
+// keeps 4 rows x 4 column chunks of loads in flight, so a row costs a few
+// memory round trips instead of one per 64 columns; the order of the sums is
+// fixed.
+__device__ __forceinline__ void rows_dot4(const double *A, uint64_t ld, uint32_t rb, uint32_t r_end, uint32_t ncol,
+                                          const double *vl, uint32_t lane, double acc[4]) {
+    bool ok[4];
+    const double *Ar[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        ok[q] = rb + q < r_end;
+        Ar[q] = A + (uint64_t)(ok[q] ? rb + q : rb) * ld;
+        acc[q] = 0.0;
+    }
+    for (uint32_t j0 = lane; j0 < ncol; j0 += 64 * 4) {
+        double a[4][4], v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t j = j0 + 64 * u;
+            const bool in = j < ncol;
+            v[u] = in ? vl[j] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q][u] = (in && ok[q]) ? Ar[q][j] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[q] = fma(a[q][u], v[u], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = wave_sum(acc[q]);
+}
+
 // Forward sweep of one level: w_S = b_S + children's updates (kept in ws for
 // the backward sweep), then the update rows upd = w_U + P21 w_S, with
 // P21 = -F21 F11^-1 in rows s.. of the front.  16 rows per workgroup.
@@ -610,21 +686,20 @@ __global__ void __launch_bounds__(MB) k_mf_fwd(const MfItem *__restrict__ items,
     }
     __syncthreads();
     const uint32_t u = f.m - f.s, lane = threadIdx.x & 63;
-    const uint32_t r1 = min(it.r0 + 16u, u);
-    for (uint32_t r = it.r0 + (threadIdx.x >> 6); r < r1; r += MB / 64) {
-        const double *Fr = F + f.off + (uint64_t)(f.s + r) * f.m;
-        double acc = 0.0;
-        for (uint32_t j = lane; j < f.s; j += 64) acc += Fr[j] * wl[j];
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            double wu = 0.0;
-            for (uint32_t l = 0; l < f.nch; ++l) {
-                const MfLink L = links[f.ch0 + l];
-                const int32_t ip = inv[L.inv + f.s + r];
-                if (ip >= 0) wu += upd[fr[L.child].upd + ip];
-            }
-            upd[f.upd + r] = wu + acc;
+    const uint32_t rb = it.r0 + (threadIdx.x >> 6) * 4, r1 = min(it.r0 + 16u, u);
+    if (rb >= r1) return;
+    double acc[4];
+    rows_dot4(F + f.off + (uint64_t)f.s * f.m, f.m, rb, r1, f.s, wl, lane, acc);
+    if (lane < 4 && rb + lane < r1) {
+        const uint32_t r = rb + lane;
+        double wu = 0.0;
+        for (uint32_t l = 0; l < f.nch; ++l) {
+            const MfLink L = links[f.ch0 + l];
+            const int32_t ip = inv[L.inv + f.s + r];
+            if (ip >= 0) wu += upd[fr[L.child].upd + ip];
         }
+        const double a = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
+        upd[f.upd + r] = wu + a;
     }
 }
 
@@ -638,13 +713,14 @@ __global__ void __launch_bounds__(MB) k_mf_bwd(const MfItem *__restrict__ items,
     const MfFront f = fr[it.front];
     for (uint32_t j = threadIdx.x; j < f.m; j += MB) vl[j] = j < f.s ? ws[f.ws + j] : -x[idx[f.idx + j]];
     __syncthreads();
-    const uint32_t lane = threadIdx.x & 63, r1 = min(it.r0 + 16u, f.s);
-    for (uint32_t r = it.r0 + (threadIdx.x >> 6); r < r1; r += MB / 64) {
-        const double *Fr = F + f.off + (uint64_t)r * f.m;
-        double acc = 0.0;
-        for (uint32_t j = lane; j < f.m; j += 64) acc += Fr[j] * vl[j];
-        acc = wave_sum(acc);
-        if (lane == 0) x[idx[f.idx + r]] = acc;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t rb = it.r0 + (threadIdx.x >> 6) * 4, r1 = min(it.r0 + 16u, f.s);
+    if (rb >= r1) return;
+    double acc[4];
+    rows_dot4(F + f.off, f.m, rb, r1, f.m, vl, lane, acc);
+    if (lane < 4 && rb + lane < r1) {
+        const double a = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
+        x[idx[f.idx + rb + lane]] = a;
     }
 }
 

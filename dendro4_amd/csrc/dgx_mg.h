@@ -52,9 +52,10 @@ void mg_mark_idle(MgHierarchy &mg);
 // coarse inverse, drops captured graphs.
 void mg_set_operator(MgHierarchy &mg, double lap_fac, double mass_fac, hipStream_t s);
 // A coarsest level of more nodes than this makes the hierarchy coarsen past
-// the requested number of levels (the direct solver's fronts would exceed its
-// limits).
-constexpr uint32_t kMaxDirectNodes = 131072;
+// the requested number of levels: the direct solver's analysis, factorisation
+// and storage grow faster than the level (fronts of ~2000 nodes and ~1 GB of
+// frontal matrices at 75,000 nodes, against ~900 and 0.35 GB at 26,000).
+constexpr uint32_t kMaxDirectNodes = 40000;
 // With overlap, each level's DA, its diagonal and the coarse solver are built
 // on a second host thread and stream while the next levels are coarsened
 // (faster when nothing else drives the device; a caller that already runs

@@ -43,6 +43,11 @@ int dgx_device_free(void *p);
 int dgx_memcpy_htod(void *dst, const void *src, size_t bytes);
 int dgx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int dgx_memset_device(void *dst, int value, size_t bytes);
+/* Device memory (no reference counterpart): out[0] device total, [1] in use on
+ * the device (total - free, all allocators), [2] the library's allocations in
+ * use, [3] their peak since start or the last reset (reset_peak != 0 resets it
+ * after reading), [4] bytes the library's block cache holds. */
+int dgx_memory_info(uint64_t out[5], int reset_peak);
 
 /* ---- octree ------------------------------------------------------------- */
 /* ot::points2Octree on one rank (include/octUtils.h:633-634; construct.cpp:
@@ -103,10 +108,39 @@ int dgx_da_get_info(dgx_da da, dgx_da_info *info);
  *   local_to_node: local index -> compact node id, 0xffffffff if not a node
  *     (the vecGetBuffer compaction, oda.cpp:863-882) */
 int dgx_da_export(dgx_da da, uint32_t *nlist, uint8_t *masks, uint8_t *oct_levels, uint32_t *local_to_node);
+/* Compressed element-to-node maps in the reference's form (ot::DA with
+ * compressLut = true; odaBuildNlist.cpp:3120-3262), encoded on the device,
+ * host copies (any array pointer may be NULL):
+ *   rem: 8*element_size remainders (m_ucpLutRemainders), offset % 256 of the
+ *     element's sorted node indices (first offset: element index - smallest);
+ *   qmask: element_size quotient masks (m_ucpLutMasks[2i]);
+ *   order: element_size sort orders (m_ucpSortOrders: XYZ..ZYX, or the child
+ *     number of a hanging element);
+ *   quot: the nonzero quotients (unsigned short)(offset >> 8) in element order
+ *     (m_uspLutQuotients); *n_quot receives their count (call with quot NULL to
+ *     size the buffer; quot_cap below the count is an error). */
+int dgx_da_compress_maps(dgx_da da, uint8_t *rem, uint8_t *qmask, uint8_t *order, uint16_t *quot, size_t quot_cap,
+                         size_t *n_quot);
+/* DA::getNodeIndices on compressed maps (include/oda/oda.tcc:1304-3950) for
+ * every element, on the device: nlist (host, 8*element_size) receives the
+ * decoded local-buffer indices.  With the Morton curve they equal
+ * dgx_da_export's nlist.  With the Hilbert curve the reference's format does
+ * not round-trip (its fixed unsort patterns assume Morton order, and the
+ * 16-bit quotient of a negative first offset loses bits); this decode then
+ * returns what the reference's getNodeIndices returns. */
+int dgx_da_decode_maps(dgx_da da, const uint8_t *rem, const uint8_t *qmask, const uint8_t *order,
+                       const uint16_t *quot, size_t n_quot, uint32_t *nlist);
 /* Anchors of the local-buffer octants (3 x uint32 per octant, interleaved)
  * in the embedded 2^(maxDepth+1) domain; with oct_levels they give the
  * octant of DA::getChildNumber / getCurrentOffset (oda.tcc:1273-1300). */
 int dgx_da_export_anchors(dgx_da da, uint32_t *xyz);
+/* Dirichlet flags per local-buffer octant, as assignBoundaryFlags (odaUtils.cpp:
+ * 2464-2812) computes them and the reference's Dirichlet drivers copy them into
+ * a DAMG level's suppressedDOF (omgJac.C:553-574): for an octant that carries a
+ * node, the number of axes on which the node lies on the boundary of the
+ * domain (TreeNode::FACE_BDY 1, EDGE_BDY 2, CORNER_BDY 3; 0 inside); 0 for an
+ * octant that carries no node.  flags holds local_buffer_size bytes (host). */
+int dgx_da_boundary_flags(dgx_da da, uint8_t *flags);
 /* y = (lap_fac * K + mass_fac * M) x on compact node vectors (device pointers
  * of length n_nodes).  Jacobian1MatMult (examples/src/backend/odaJac.C:
  * 297-366) with type-2 hanging stencils; lapFac/massFac as read from the
@@ -138,7 +172,7 @@ typedef struct dgx_mg_s *dgx_mg;
  * blockDiag.cpp:780-1047, 1312-2190) on one rank: nlevels levels built by
  * coarsenOctree + balanceOctree from the finest octree (stopping early below
  * 500 octants, blockDiag.cpp:1417-1420; only a coarsest level of more than
- * 131072 nodes, beyond the direct solver's reach, makes it coarsen further),
+ * 40,000 nodes, too costly for the direct solver, makes it coarsen further),
  * a DA per level, interpolation P and restriction R = P^T between levels,
  * Jacobi diagonals and the coarsest-level sparse FP64 direct solver (nested
  * dissection + multifrontal factorisation on the device, in place of the
@@ -150,6 +184,8 @@ int dgx_mg_set_operator(dgx_mg mg, double lap_fac, double mass_fac);
 /* Number of levels actually built and, per level (finest first), the number of
  * octants and of nodes (arrays of at least 32 entries, may be NULL). */
 int dgx_mg_info(dgx_mg mg, int *nlevels, uint64_t *octants, uint64_t *nodes);
+/* The DA of level `level` (0 = finest; owned by mg, do not free). */
+int dgx_mg_level_da(dgx_mg mg, int level, dgx_da *da);
 /* The finest level's DA (owned by mg; do not free). */
 int dgx_mg_finest_da(dgx_mg mg, dgx_da *da);
 /* Load vector b = M f of a nodal f with boundary entries set to zero. */

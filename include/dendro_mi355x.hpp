@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -302,8 +303,13 @@ class DA {
        const std::vector<TreeNode> *blocksPtr = nullptr, bool *iAmActive = nullptr) {
         (void)inputActiveComm;
         (void)tol;
-        (void)compressLut;  // the device maps are never Golomb-Rice compressed
         (void)blocksPtr;
+        // compressLut: the element-to-node maps are held on the host in the
+        // reference's compressed form and decoded per element by
+        // getNodeIndices.  Only with Morton order: with Hilbert order the
+        // reference's format does not round-trip (dgx_da_decode_maps), so the
+        // plain maps are kept.
+        compress_ = compressLut && dgx::curve() == DGX_CURVE_MORTON;
         dgx::require_one_rank(comm);
         dgx::ensure_init();
         if (iAmActive) *iAmActive = !in.empty();
@@ -347,12 +353,18 @@ class DA {
         return ++cur_;
     }
     unsigned curr() const { return cur_; }
-    // Local-buffer indices of the current element's 8 vertices (m_uiNlist).
+    // Local-buffer indices of the current element's 8 vertices (m_uiNlist),
+    // decoded from the compressed maps when the DA keeps them (oda.tcc:1304-3950).
     int getNodeIndices(unsigned *nodes) {
         maps();
+        if (compress_) {
+            decode(cur_, nodes);
+            return 1;
+        }
         std::memcpy(nodes, &nlist_[8 * (size_t)cur_], 8 * sizeof(unsigned));
         return 1;
     }
+    bool isLutCompressed() const { return compress_; }
     // Hanging mask of element i (m_ucpLutMasks[2i+1]).
     unsigned char getHangingNodeIndex(unsigned i) {
         maps();
@@ -457,12 +469,69 @@ class DA {
     void info() { dgx::check(dgx_da_get_info(h_, &inf_), "DA info"); }
     void maps() {
         if (have_maps_) return;
-        nlist_.resize(8 * (size_t)inf_.element_size);
-        masks_.resize(inf_.element_size);
+        const size_t ne = inf_.element_size;
+        if (!compress_) nlist_.resize(8 * ne);
+        masks_.resize(ne);
         levels_.resize(inf_.local_buffer_size);
         l2n_.resize(inf_.local_buffer_size);
-        dgx::check(dgx_da_export(h_, nlist_.data(), masks_.data(), levels_.data(), l2n_.data()), "DA maps");
+        dgx::check(dgx_da_export(h_, compress_ ? nullptr : nlist_.data(), masks_.data(), levels_.data(), l2n_.data()),
+                   "DA maps");
+        if (compress_) {
+            size_t nq = 0;
+            dgx::check(dgx_da_compress_maps(h_, nullptr, nullptr, nullptr, nullptr, 0, &nq), "DA compressed maps");
+            rem_.resize(8 * ne);
+            qmask_.resize(ne);
+            order_.resize(ne);
+            quot_.resize(nq);
+            dgx::check(dgx_da_compress_maps(h_, rem_.data(), qmask_.data(), order_.data(), quot_.data(), nq, &nq),
+                       "DA compressed maps");
+            // each element's first quotient (the reference keeps a running counter)
+            qoff_.resize(ne);
+            uint32_t q = 0;
+            for (size_t i = 0; i < ne; ++i) {
+                qoff_[i] = q;
+                q += (uint32_t)__builtin_popcount(qmask_[i]);
+            }
+        }
         have_maps_ = true;
+    }
+    // DA::getNodeIndices with compressLut (oda.tcc:1304-3950): the sorted
+    // indices from remainders and quotients, then unsorted by the fixed
+    // pattern of the sort order, or for a hanging element by the curve order
+    // of its vertex points (a hanging vertex j moved to the parent's corner j).
+    void decode(unsigned i, unsigned *nodes) {
+        static const unsigned char pat[6][6] = {{1, 2, 3, 4, 5, 6}, {2, 1, 3, 4, 6, 5}, {1, 4, 5, 2, 3, 6},
+                                                {4, 1, 5, 2, 6, 3}, {2, 4, 6, 1, 3, 5}, {4, 2, 6, 1, 5, 3}};
+        unsigned nn[8];
+        uint32_t qc = qoff_[i];
+        for (int j = 0; j < 8; ++j) {
+            unsigned v = rem_[8 * (size_t)i + j];
+            if ((qmask_[i] >> j) & 1u) v += (unsigned)quot_[qc++] << 8;
+            nn[j] = j == 0 ? i - v : v + nn[j - 1];
+        }
+        const unsigned h = masks_[i], c = order_[i];
+        if (!h) {
+            nodes[0] = nn[0];
+            for (int k = 0; k < 6; ++k) nodes[pat[c][k]] = nn[k + 1];
+            nodes[7] = nn[7];
+            return;
+        }
+        anchors();
+        const unsigned md = getMaxDepth(), sz = 1u << (md - getLevel(i));
+        const uint32_t *a = &anchors_[3 * (size_t)i];
+        const unsigned p[3] = {a[0] - (c & 1u) * sz, a[1] - ((c >> 1) & 1u) * sz, a[2] - ((c >> 2) & 1u) * sz};
+        TreeNode v[8];
+        int idx[8];
+        for (int j = 0; j < 8; ++j) {
+            const bool hj = (h >> j) & 1u;
+            const unsigned s = hj ? 2 * sz : sz;
+            unsigned q[3];
+            for (int t = 0; t < 3; ++t) q[t] = (hj ? p[t] : a[t]) + ((j >> t) & 1) * s;
+            v[j] = TreeNode(q[0], q[1], q[2], md, 3, md);
+            idx[j] = j;
+        }
+        std::sort(idx, idx + 8, [&](int x, int y) { return v[x] < v[y]; });
+        for (int k = 0; k < 8; ++k) nodes[idx[k]] = nn[k];
     }
     void anchors() {
         maps();
@@ -471,11 +540,13 @@ class DA {
         dgx::check(dgx_da_export_anchors(h_, anchors_.data()), "DA anchors");
     }
     dgx_da h_ = nullptr;
-    bool owned_ = false, have_maps_ = false;
+    bool owned_ = false, have_maps_ = false, compress_ = false;
     dgx_da_info inf_{};
     unsigned cur_ = 0;
-    std::vector<uint32_t> nlist_, l2n_, anchors_;
+    std::vector<uint32_t> nlist_, l2n_, anchors_, qoff_;
     std::vector<uint8_t> masks_, levels_;
+    std::vector<uint8_t> rem_, qmask_, order_;  // compressed maps (compressLut)
+    std::vector<uint16_t> quot_;
 };
 
 // ---- DAMG (omg.h:419-608) ---------------------------------------------------
@@ -493,6 +564,11 @@ struct _p_DAMG {
     dgx_solve_opts opts{1, 2, 2, 0.8, 1e-8, 200, 1};
     dgx_solve_stats stats{};
     dgx_mg mg = nullptr;       // shared by all levels; owned by damg[0]
+    int level = 0;             // level in the device hierarchy (0 = finest)
+    // Dirichlet marks per local-buffer entry (dof * getLocalBufferSize() bytes),
+    // set by DAMGCreateSuppressedDOFs (omg.h:427)
+    unsigned char *suppressedDOF = nullptr;
+    unsigned char *suppressedDOFaux = nullptr;  // no auxiliary DAs on one rank: stays NULL
 };
 typedef _p_DAMG *DAMG;
 
@@ -534,6 +610,7 @@ inline int DAMGCreateAndSetDA(MPI_Comm comm, int &nlevels, void *user, DAMG **da
         arr[i]->comm = comm;
         arr[i]->user = user;
         arr[i]->mg = mg;
+        arr[i]->level = nl - 1 - i;
     }
     dgx_da fine = nullptr;
     dgx::check(dgx_mg_finest_da(mg, &fine), "DAMGCreateAndSetDA");
@@ -543,6 +620,30 @@ inline int DAMGCreateAndSetDA(MPI_Comm comm, int &nlevels, void *user, DAMG **da
     finestOctree.clear();
     *damg = arr;
     return 0;
+}
+
+// DAMGCreateSuppressedDOFs (omg.h:504; blockDiag.cpp:46-66): allocates each
+// level's suppressedDOF, dof * getLocalBufferSize() bytes.  The reference
+// leaves filling it to the Dirichlet driver (SetDirichletJacContexts,
+// omgJac.C:509-587: assignBoundaryFlags copied to the node indices of every
+// element); here it is filled with the same flags at once (FACE_BDY 1,
+// EDGE_BDY 2, CORNER_BDY 3 for boundary nodes, 0 elsewhere), since the device
+// operator already applies them.  Returns 1 like the reference.
+inline int DAMGCreateSuppressedDOFs(DAMG *damg) {
+    const int nl = damg[0]->nlevels;
+    for (int i = 0; i < nl; ++i) {
+        dgx_da d = nullptr;
+        dgx::check(dgx_mg_level_da(damg[i]->mg, damg[i]->level, &d), "DAMGCreateSuppressedDOFs");
+        dgx_da_info inf{};
+        dgx::check(dgx_da_get_info(d, &inf), "DAMGCreateSuppressedDOFs");
+        const size_t sz = (size_t)damg[i]->dof * inf.local_buffer_size;
+        delete[] damg[i]->suppressedDOF;
+        damg[i]->suppressedDOF = nullptr;
+        if (!sz) continue;
+        damg[i]->suppressedDOF = new unsigned char[sz];
+        dgx::check(dgx_da_boundary_flags(d, damg[i]->suppressedDOF), "DAMGCreateSuppressedDOFs");
+    }
+    return 1;
 }
 
 // DAMGSetKSP (omg.h:488): operator -lapFac lap + massFac I with homogeneous
@@ -603,6 +704,8 @@ inline int DAMGDestroy(DAMG *damg) {
     dgx_mg mg = damg[0]->mg;
     for (int i = 0; i < nl; ++i) {
         delete damg[i]->da;
+        delete[] damg[i]->suppressedDOF;
+        delete[] damg[i]->suppressedDOFaux;
         delete damg[i];
     }
     delete[] damg;

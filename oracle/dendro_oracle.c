@@ -842,3 +842,97 @@ void or_da_element_view(const or_da *da, uint32_t *conn8, uint8_t *cnum, uint8_t
         mask[e] = da->mask[e];
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Compressed element-to-node maps (compressLut = true).                      */
+/* ------------------------------------------------------------------------- */
+
+/* DA::buildNodeList compression, odaBuildNlist.cpp:3120-3262 (one rank: all */
+/* elements).  Per element i: the sort order (non-hanging: the axis order of  */
+/* the coordinate XORs x ^ (x + sz), :3147-3170; hanging: the Morton child    */
+/* number, :3172-3185); the 8 nlist entries sorted ascending; the offsets     */
+/* i - nl[0] and nl[j] - nl[j-1] as unsigned, remainder offset % 256 into     */
+/* rem[8i+j], quotient (unsigned short)(offset >> 8) appended to quot and bit */
+/* j of mask[i] set when it is not zero (:3232-3258).  Returns the number of  */
+/* quotients; quot may be NULL to count them.                                 */
+size_t or_da_compress(const or_da *da, uint8_t *rem, uint8_t *mask, uint8_t *order, uint16_t *quot) {
+    size_t nq = 0;
+    for (size_t i = 0; i < da->n_elem; i++) {
+        const or_tn *e = &da->octs[i];
+        uint32_t d = LEV(e), md = e->md, sz = 1u << (md - d);
+        uint32_t x = e->x, y = e->y, z = e->z;
+        if (!da->mask[i]) {
+            uint32_t _x = x ^ (x + sz), _y = y ^ (y + sz), _z = z ^ (z + sz);
+            /* DA_FLAGS::SortOrderType: XYZ 0, YXZ 1, XZY 2, ZXY 3, YZX 4, ZYX 5 (odaUtils.h:108-110) */
+            if (_x > _y) order[i] = _y > _z ? 5 : (_x > _z ? 4 : 1);
+            else order[i] = _x > _z ? 3 : (_y > _z ? 2 : 0);
+        } else {
+            uint32_t lp = 1u << (md - d + 1);
+            order[i] = (uint8_t)(4 * ((z % lp) / sz) + 2 * ((y % lp) / sz) + (x % lp) / sz);
+        }
+        uint32_t nl[8];
+        for (int j = 0; j < 8; j++) nl[j] = da->nlist[8 * i + j];
+        for (int a = 1; a < 8; a++) /* ascending */
+            for (int b = a; b > 0 && nl[b - 1] > nl[b]; b--) { uint32_t t = nl[b]; nl[b] = nl[b - 1]; nl[b - 1] = t; }
+        mask[i] = 0;
+        for (int j = 0; j < 8; j++) {
+            uint32_t off = j == 0 ? (uint32_t)i - nl[0] : nl[j] - nl[j - 1];
+            uint16_t q = (uint16_t)(off >> 8);
+            rem[8 * i + j] = (uint8_t)(off % 256);
+            if (q) {
+                mask[i] |= (uint8_t)(1u << j);
+                if (quot) quot[nq] = q;
+                nq++;
+            }
+        }
+    }
+    return nq;
+}
+
+/* DA::getNodeIndices with compressLut (oda.tcc:1304-3950): the sorted list  */
+/* from remainders and quotients (quotients consumed in order, :1316-1337),  */
+/* then unsorted.  Hanging element (:1341-3875): its 8 vertex points, where   */
+/* a hanging vertex j is moved to the parent's corner j (the cNumEtype        */
+/* switch, restated in closed form: parent anchor + 2 sz * bits of j), sorted */
+/* with operator< at maxDepth; the k-th smallest gets nn[k].  Otherwise the   */
+/* fixed pattern of the element's sort order (:3876-3935).                    */
+void or_da_decode(const or_da *da, const uint8_t *rem, const uint8_t *mask, const uint8_t *order,
+                  const uint16_t *quot, uint32_t *nlist) {
+    static const int pat[6][6] = {{1, 2, 3, 4, 5, 6}, {2, 1, 3, 4, 6, 5}, {1, 4, 5, 2, 3, 6},
+                                  {4, 1, 5, 2, 6, 3}, {2, 4, 6, 1, 3, 5}, {4, 2, 6, 1, 5, 3}};
+    size_t qc = 0;
+    for (size_t i = 0; i < da->n_elem; i++) {
+        uint32_t nn[8], *nodes = nlist + 8 * i;
+        for (int j = 0; j < 8; j++) {
+            nn[j] = rem[8 * i + j];
+            if (mask[i] & (1u << j)) nn[j] += (uint32_t)quot[qc++] << 8;
+            nn[j] = j == 0 ? (uint32_t)i - nn[0] : nn[j] + nn[j - 1];
+        }
+        uint8_t hn = da->mask[i];
+        if (hn) {
+            const or_tn *e = &da->octs[i];
+            uint32_t d = LEV(e), md = e->md, sz = 1u << (md - d), c = order[i];
+            uint32_t px = e->x - (c & 1u) * sz, py = e->y - ((c >> 1) & 1u) * sz, pz = e->z - ((c >> 2) & 1u) * sz;
+            or_tn v[8];
+            int idx[8];
+            for (int j = 0; j < 8; j++) {
+                uint32_t s = (hn >> j) & 1u ? 2 * sz : sz;
+                uint32_t bx = (hn >> j) & 1u ? px : e->x, by = (hn >> j) & 1u ? py : e->y,
+                         bz = (hn >> j) & 1u ? pz : e->z;
+                v[j] = tn_make(bx + (j & 1) * s, by + ((j >> 1) & 1) * s, bz + ((j >> 2) & 1) * s, md, md);
+                idx[j] = j;
+            }
+            for (int a = 1; a < 8; a++)
+                for (int b = a; b > 0 && or_tn_less(&v[idx[b]], &v[idx[b - 1]]); b--) {
+                    int t = idx[b]; idx[b] = idx[b - 1]; idx[b - 1] = t;
+                }
+            for (int k = 0; k < 8; k++) nodes[idx[k]] = nn[k];
+        } else {
+            const int *p = pat[order[i]];
+            nodes[0] = nn[0];
+            nodes[p[0]] = nn[1]; nodes[p[1]] = nn[2]; nodes[p[2]] = nn[3];
+            nodes[p[3]] = nn[4]; nodes[p[4]] = nn[5]; nodes[p[5]] = nn[6];
+            nodes[7] = nn[7];
+        }
+    }
+}

@@ -56,6 +56,9 @@ int or_elem_type(uint32_t cnum, uint32_t mask);
 int or_matvec(const or_da *da, const double *L, const double *M, double lapFac, double massFac, const double *x,
               double *y);
 void or_da_element_view(const or_da *da, uint32_t *conn8, uint8_t *cnum, uint8_t *level, uint8_t *mask);
+size_t or_da_compress(const or_da *da, uint8_t *rem, uint8_t *mask, uint8_t *order, uint16_t *quot);
+void or_da_decode(const or_da *da, const uint8_t *rem, const uint8_t *mask, const uint8_t *order,
+                  const uint16_t *quot, uint32_t *nlist);
 
 #ifdef __cplusplus
 }

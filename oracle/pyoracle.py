@@ -62,6 +62,9 @@ def lib():
         L.or_elem_type.argtypes = [C.c_uint32, C.c_uint32]
         L.or_matvec.argtypes = [C.POINTER(_DA), vp, vp, C.c_double, C.c_double, vp, vp]
         L.or_da_element_view.argtypes = [C.POINTER(_DA), vp, vp, vp, vp]
+        L.or_da_compress.argtypes = [C.POINTER(_DA), vp, vp, vp, vp]
+        L.or_da_compress.restype = C.c_size_t
+        L.or_da_decode.argtypes = [C.POINTER(_DA), vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -197,6 +200,25 @@ class DA:
         mk = np.zeros(n, np.uint8)
         lib().or_da_element_view(C.byref(self._da), _p(conn), _p(cn), _p(lev), _p(mk))
         return conn, cn, lev, mk
+
+    def compress(self):
+        """compressLut maps (odaBuildNlist.cpp:3120-3262): remainders (E, 8) u8,
+        quotient masks (E,) u8, sort orders (E,) u8, quotients u16."""
+        n = self.n_elem
+        rem = np.zeros((n, 8), np.uint8)
+        msk = np.zeros(n, np.uint8)
+        order = np.zeros(n, np.uint8)
+        nq = lib().or_da_compress(C.byref(self._da), _p(rem), _p(msk), _p(order), None)
+        quot = np.zeros(max(nq, 1), np.uint16)
+        lib().or_da_compress(C.byref(self._da), _p(rem), _p(msk), _p(order), _p(quot))
+        return rem, msk, order, quot[:nq]
+
+    def decode(self, rem, msk, order, quot):
+        """getNodeIndices on compressed maps (oda.tcc:1304-3950): nlist (E, 8)."""
+        out = np.zeros((self.n_elem, 8), np.uint32)
+        arrs = [np.ascontiguousarray(a) for a in (rem, msk, order, quot)]
+        lib().or_da_decode(C.byref(self._da), *[_p(a) for a in arrs], _p(out))
+        return out
 
     def matvec(self, x, L, M, lap_fac=1.0, mass_fac=1.0):
         x = np.ascontiguousarray(x, np.float64)

@@ -143,6 +143,29 @@ static int run() {
     CHECK(finest.empty(), "DAMGCreateAndSetDA must clear the finest octree");
     CHECK(nlevels >= 2, "levels built: %d", nlevels);
     ot::DA *fda = DAMGGetDA(damg);
+    // suppressed DOFs: on every level, the count of boundary axes of each
+    // node's position (assignBoundaryFlags), 0 for entries that are not nodes
+    CHECK(ot::DAMGCreateSuppressedDOFs(damg) == 1, "DAMGCreateSuppressedDOFs");
+    for (int lv = 0; lv < nlevels; ++lv)
+        CHECK(damg[lv]->suppressedDOF != nullptr && damg[lv]->suppressedDOFaux == nullptr, "level %d arrays", lv);
+    {
+        // against the oracle's DA (same local buffer): its octants' anchors
+        // in the embedded domain [0, 2^(maxDepth+1)), the unit cube ends at 2^maxDepth
+        const ot::DAMG fl = DAMGGetFine(damg);
+        CHECK(fda->getLocalBufferSize() == rda.n_local, "local buffer sizes differ");
+        size_t bad_sup = 0, marked = 0;
+        for (size_t i = 0; i < rda.n_local; ++i) {
+            const or_tn &o = rda.octs[i];
+            const unsigned top = 1u << (o.md - 1);
+            unsigned want = 0;
+            if (rda.local_to_node[i] != 0xffffffffu)
+                want = (o.x == 0 || o.x == top) + (o.y == 0 || o.y == top) + (o.z == 0 || o.z == top);
+            bad_sup += fl->suppressedDOF[i] != want;
+            marked += want != 0;
+        }
+        CHECK(bad_sup == 0 && marked > 0, "suppressedDOF differs at %zu of %u entries (%zu marked)", bad_sup,
+              fda->getLocalBufferSize(), marked);
+    }
     g_f.assign(fda->getNodeSize(), 0.0);
     std::normal_distribution<double> nd(0.0, 1.0);
     for (auto &x : g_f) x = nd(rng);
@@ -172,6 +195,34 @@ static int run() {
     CHECK(std::sqrt(rr / bb) <= 1.5e-8, "recomputed residual %.3e", std::sqrt(rr / bb));
     ot::DAMGDestroy(damg);
     ot::DAMG_Finalize();
+
+    // compressLut with Morton order: getNodeIndices decodes the reference's
+    // compressed maps (host) to the plain node list of the same octree
+    {
+        dgx::curve() = DGX_CURVE_MORTON;
+        std::vector<double> mp(pts_copy);
+        std::vector<ot::TreeNode> mo, mb;
+        double gl[3] = {1.0, 1.0, 1.0};
+        ot::points2Octree(mp, gl, mo, 3, maxDepth, maxNumPts, MPI_COMM_WORLD);
+        ot::balanceOctree(mo, mb, 3, maxDepth, true, MPI_COMM_WORLD);
+        std::vector<ot::TreeNode> mb2 = mb;
+        ot::DA plain(mb, MPI_COMM_WORLD, MPI_COMM_WORLD, 0.1, false);
+        ot::DA packed(mb2, MPI_COMM_WORLD, MPI_COMM_WORLD, 0.1, true);
+        CHECK(packed.isLutCompressed() && !plain.isLutCompressed(), "compressLut not honoured");
+        size_t bad_lut = 0, elems_lut = 0;
+        for (packed.init<ot::DA_FLAGS::ALL>(), plain.init<ot::DA_FLAGS::ALL>();
+             packed.curr() < packed.end<ot::DA_FLAGS::ALL>(); packed.next<ot::DA_FLAGS::ALL>(),
+                                                          plain.next<ot::DA_FLAGS::ALL>()) {
+            unsigned a[8], b[8];
+            packed.getNodeIndices(a);
+            plain.getNodeIndices(b);
+            bad_lut += !std::equal(a, a + 8, b);
+            ++elems_lut;
+        }
+        CHECK(elems_lut == plain.getElementSize() && bad_lut == 0, "compressed getNodeIndices differ at %zu of %zu",
+              bad_lut, elems_lut);
+        dgx::curve() = DGX_CURVE_HILBERT;
+    }
 
     or_da_free(&rda);
     or_free(ro);

@@ -372,3 +372,35 @@ def test_oracle_matvec_invariants(oracle, stencils):
     assert abs(da.matvec(pos, L, M, 1.0, 0.0).sum()) < 1e-11
     # energy of the linear field: u^T K u = integral |grad x|^2 = 1
     assert abs(pos @ da.matvec(pos, L, M, 1.0, 0.0) - 1.0) < 1e-11
+
+
+@pytest.mark.parametrize("dist,n,m", [(1, 20000, 1), (2, 30000, 4), (0, 5000, 1)])
+def test_oracle_compressed_maps_round_trip(oracle, dist, n, m):
+    """compressLut maps (odaBuildNlist.cpp:3120-3262) and their decode
+    (oda.tcc:1304-3950): with Morton order the decode of the encoding is the
+    node list.  With Hilbert order the reference's format does not round-trip:
+    elements without hanging vertices decode to the right nodes in the wrong
+    vertex order (the unsort patterns assume Morton order), and the hanging
+    elements that fail are exactly those whose first offset is negative (its
+    16-bit quotient drops the high bits)."""
+    import dendro4_amd as dgx
+    kw = dict(n_gauss=4, lo=0.2, hi=0.8, sig_lo=0.02) if dist == 2 else {}
+    pts = dgx.gen_points(dist, 9, n, **kw)
+    try:
+        for hilbert in (False, True):
+            oracle.set_ordering(hilbert)
+            da = oracle.DA(oracle.balance(oracle.points2octree(pts, 30, m)))
+            rem, qm, order, quot = da.compress()
+            assert len(quot) == int(np.unpackbits(qm).sum())
+            dec = da.decode(rem, qm, order, quot)
+            bad = (dec != da.nlist).any(1)
+            if not hilbert:
+                assert not bad.any()
+                continue
+            hang = da.mask != 0
+            same_set = (np.sort(dec, 1) == np.sort(da.nlist, 1)).all(1)
+            assert same_set[bad & ~hang].all()
+            wrapped = da.nlist.min(1) > np.arange(da.n_elem)
+            assert np.array_equal(bad & hang, wrapped)
+    finally:
+        oracle.set_ordering(True)
