@@ -36,15 +36,19 @@ def test_compressed_maps_match_oracle(dgx, oracle, curve, dist, n, m):
         oracle.set_ordering(True)
 
 
-def test_compressed_maps_full_lognormal_4m(dgx):
-    """Config 2's DA (4,194,304 log-normal points): bytes of the compressed
-    maps per element; with Morton order the decode reproduces the node list."""
-    pts = dgx.workload_points("lognormal-4M")
-    da = dgx.DA(dgx.balance_octree(dgx.points2octree(pts, 30, 8, 1)))
+def test_compressed_maps_full_gauss4_16m(dgx):
+    """Config 3's octree (16,777,216 points, 4 Gaussians) on the Morton curve:
+    bytes of the compressed maps per element, and the decode reproduces the
+    node list.  (The log-normal cloud of config 2 cannot be built on the Morton
+    curve: its clipped corner sends the reference's p2oLocal cursor past the
+    end of the domain, which this build reports as an error.)"""
+    pts = dgx.workload_points("gauss4-16M")
+    da = dgx.DA(dgx.balance_octree(dgx.points2octree(pts, 30, 4, 1)))
+    del pts
     nl = da.maps()[0]
     rem, qm, order, quot = da.compressed_maps()
     dec = da.decode_maps(rem, qm, order, quot)
     assert np.array_equal(dec, nl)
     nb = 10 * da.element_size + 2 * len(quot)
-    print(f"lognormal-4M (Morton): {da.element_size} elements, {nb / da.element_size:.2f} B/element", flush=True)
+    print(f"gauss4-16M (Morton): {da.element_size} elements, {nb / da.element_size:.2f} B/element", flush=True)
     assert nb / da.element_size < 14.0
