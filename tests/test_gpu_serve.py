@@ -155,3 +155,24 @@ def test_resident_resolves_on_two_slots_with_graphs(dgx):
     for a, b in zip(first, again):
         assert b["status"] == 0 and b["reused"] == 1
         assert np.linalg.norm(a["x"] - b["x"]) <= 1e-10 * np.linalg.norm(a["x"])
+
+
+@pytest.mark.timeout(300)
+def test_mix_with_a_2_22_point_request(dgx):
+    """Config 4's size range: a mix holding a 2^22-point request (the top of
+    the stream's log-uniform range) with small ones on two slots.  Every
+    result reaches the residual bar and matches its solve alone."""
+    big = {"n": 1 << 22, "dist": 0, "max_num_pts": 4, "seed": 4242}
+    reqs = [(big, dgx.serve_points(big))] + small_requests(dgx, 5, seed=25)
+    print("requests", [r["n"] for r, _ in reqs], flush=True)
+    mixed = run_all(dgx, reqs, slots=2)
+    for r, m in zip(reqs, mixed):
+        assert m["status"] == 0 and m["rel_residual"] <= 1e-8, (r, m)
+    assert mixed[0]["octants"] > 1000000
+    print("mix done: big request", mixed[0]["octants"], "octants", mixed[0]["levels"], "levels", flush=True)
+    for rc, m in zip(reqs, mixed):
+        a, = run_all(dgx, [rc], slots=1)
+        assert a["nodes"] == m["nodes"] and a["levels"] == m["levels"]
+        assert abs(a["iterations"] - m["iterations"]) <= 1
+        d = np.linalg.norm(a["x"] - m["x"]) / np.linalg.norm(a["x"])
+        assert d <= 1e-8, d
