@@ -410,6 +410,8 @@ int or_points2octree(const double *pts, size_t n, uint32_t maxDepth, uint32_t ma
 /* minimal 2:1 (face+edge+corner) balanced refinement.  emulate_limit != 0    */
 /* reproduces comboRipple's one-rank result for in.size() > maxNum (1e7,      */
 /* octUtils.h:477): splitInp[] is never filled, so the output is empty.       */
+/* emulate_limit == 1 uses maxNum = 1e7; a larger value is taken as maxNum    */
+/* (for tests of both sides of the limit on small inputs).                    */
 /* ------------------------------------------------------------------------- */
 int or_balance(const or_tn *in, size_t n, int incCorner, int emulate_limit, or_tn **out, size_t *nout) {
     (void)incCorner; /* in 3-D balanceSubtree always adds edge+corner neighbours (dim==3 || incCorner) */
@@ -419,7 +421,9 @@ int or_balance(const or_tn *in, size_t n, int incCorner, int emulate_limit, or_t
         *out = res.a; *nout = res.n;
         return 0;
     }
-    if (emulate_limit && n > 10000000u) { *out = NULL; *nout = 0; return 0; }
+    if (emulate_limit && n > (emulate_limit == 1 ? 10000000u : (size_t)emulate_limit)) {
+        *out = NULL; *nout = 0; return 0;
+    }
     uint32_t md = in[0].md;
     uint32_t minL = md, maxL = 0;
     for (size_t i = 0; i < n; i++) {

@@ -129,10 +129,14 @@ def points2octree(pts, max_depth=30, max_num_pts=1):
 
 
 def balance(tn, inc_corner=True, emulate_limit=False):
+    """balanceOctree on one rank.  emulate_limit=True reproduces comboRipple's
+    empty result above maxNum = 1e7 input octants; an integer > 1 is taken as
+    maxNum instead."""
     tn = np.ascontiguousarray(tn, np.uint32)
     ptr = C.c_void_p()
     n = C.c_size_t()
-    lib().or_balance(_p(tn), len(tn), 1 if inc_corner else 0, 1 if emulate_limit else 0, C.byref(ptr), C.byref(n))
+    lim = 0 if emulate_limit is False else (1 if emulate_limit is True else int(emulate_limit))
+    lib().or_balance(_p(tn), len(tn), 1 if inc_corner else 0, lim, C.byref(ptr), C.byref(n))
     return _take(ptr, n.value)
 
 

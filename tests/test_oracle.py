@@ -217,9 +217,12 @@ def test_balance_properties_and_limit(oracle):
     # refines the input: every input octant is an ancestor-or-self of output leaves
     assert len(b) >= len(u)
     assert np.array_equal(oracle.balance(b), b)  # idempotent
-    # comboRipple's one-rank path above 1e7 octants returns nothing (balance.cpp:821-853);
-    # the emulation flag is inert below the limit
+    # comboRipple's one-rank path above maxNum = 1e7 octants returns nothing
+    # (balance.cpp:821-853): the emulation is inert below the limit, and on
+    # both sides of a small maxNum it returns the balanced octree or nothing
     assert np.array_equal(oracle.balance(u, emulate_limit=True), b)
+    assert np.array_equal(oracle.balance(u, emulate_limit=len(u)), b)
+    assert len(oracle.balance(u, emulate_limit=len(u) - 1)) == 0
 
 
 # ---------------------------------------------------------------- stencils --
