@@ -940,3 +940,213 @@ void or_da_decode(const or_da *da, const uint8_t *rem, const uint8_t *mask, cons
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Multigrid hierarchy and inter-grid transfers.                              */
+/* ------------------------------------------------------------------------- */
+
+/* TreeNode::getChildNumber (TreeNode.tcc:23-60): the position among the     */
+/* siblings along the curve (Hilbert rank under HILBERT_ORDERING).           */
+static inline uint32_t tn_child_number(const or_tn *a) {
+    if (LEV(a) == 0) return 0;
+    or_tn par = tn_parent(a);
+    uint32_t m = tn_morton_index(a);
+    return g_hilbert ? (uint32_t)rot_entry(tn_rotation(&par), 8 + (int)m) : m;
+}
+
+/* coarsenOctree, sequential form (src/coarsen.cpp:64-104): a first child    */
+/* followed by at least 7 octants before the next first child starts a      */
+/* complete sibling group of leaves, which becomes its parent; everything   */
+/* else is copied.  Returns -1 when the input does not start with a first   */
+/* child (the reference asserts this).                                      */
+int or_coarsen(const or_tn *in, size_t n, or_tn **out, size_t *nout) {
+    tvec o = {0, 0, 0};
+    if (n < 8) {
+        for (size_t i = 0; i < n; i++) tv_push(&o, in[i]);
+        *out = o.a; *nout = o.n;
+        return 0;
+    }
+    if (tn_child_number(&in[0]) != 0) return -1;
+    size_t prev = 0;
+    while (prev < n) {
+        size_t curr = prev + 1;
+        while (curr < n && tn_child_number(&in[curr]) != 0) curr++;
+        if (curr >= prev + 8) {
+            or_tn p = tn_parent(&in[prev]);
+            p.w = in[prev].w;
+            tv_push(&o, p);
+            for (size_t i = prev + 8; i < curr; i++) tv_push(&o, in[i]);
+        } else {
+            for (size_t i = prev; i < curr; i++) tv_push(&o, in[i]);
+        }
+        prev = curr;
+    }
+    *out = o.a; *nout = o.n;
+    return 0;
+}
+
+/* Restriction restrictMatVecType1 (src/omg/restrictMatVec.cpp:486-619) with */
+/* the touched flags of dummyRestrictMatVecType1 (:410-484) and the macros   */
+/* INTERGRID_TRANSFER_LOOP_BLOCK(_DUMMY), ITLB_DUMMY_FINAL_SET_VALUE and      */
+/* ITLB_SET_VALUE_SUPPRESSED_DOFS (:71-400).  One rank: every coarse element */
+/* is independent and the ALL and WRITABLE loops both run over the elements. */
+/* prolongMatVecType1 (src/omg/prolongMatVec.cpp:154-292) applies the same   */
+/* entries transposed, so the entries themselves are the result: each        */
+/* contribution c[cidx] += Rval * f[fidx] is written out as (row = coarse    */
+/* compact node, col = fine compact node, Rval), in the reference's order.   */
+/* Tables: rmat1[8][8][18][8][8], rmat2[8][18][8][8] (cCtr-major) and the    */
+/* vertex maps map1[8][8][18][8], map2[8][8][8][18][8], map3[7][2][8][18][8], */
+/* map4[7][2][8][8][18][8] (Stencils/RmatType*Stencils.inp, vtxMap.inp).      */
+/* supf / supc: suppressed fine / coarse compact nodes (NULL: none).          */
+/* Fine elements under a coarse element: the reference advances the fine    */
+/* loop to the coarse anchor and then takes the next 8 elements as children */
+/* 0..7, which is the Morton order of children.  Under Hilbert ordering the  */
+/* children are not in that order, so here the group starts at the first     */
+/* fine element inside the coarse one and each child's stencil is selected   */
+/* by its own Morton child number; under Morton both readings coincide (and  */
+/* are checked to).                                                          */
+/* Returns 0; -1 on capacity, -2 on a fine/coarse mismatch, -3 on an unknown */
+/* element type, -4 on a non-node index.                                     */
+static inline int elem_inside_(const or_da *f, size_t e, const or_tn *c) {
+    const or_tn *o = &f->octs[e];
+    uint32_t sc = 1u << (c->md - LEV(c));
+    return o->x >= c->x && o->x < c->x + sc && o->y >= c->y && o->y < c->y + sc && o->z >= c->z &&
+           o->z < c->z + sc;
+}
+
+int or_restrict_entries(const or_da *daf, const or_da *dac, const double *rmat1, const double *rmat2,
+                        const uint16_t *map1, const uint16_t *map2, const uint16_t *map3, const uint16_t *map4,
+                        const uint8_t *supf, const uint8_t *supc, uint32_t *rows, uint32_t *cols, double *vals,
+                        size_t *nnz) {
+    const size_t cap = *nnz;
+    size_t cnt = 0;
+    uint8_t *dummy = (uint8_t *)calloc(daf->n_local * 16, 1);  /* FineTouchedDummyStatus::flags[16] */
+    uint8_t *touched = (uint8_t *)calloc(daf->n_local * 8, 1); /* FineTouchedStatus::flags[8] */
+    int rc = 0;
+    for (int pass = 0; pass < 2 && !rc; pass++) {
+        /* pass 0: INTERGRID_TRANSFER_LOOP_BLOCK_DUMMY; pass 1: INTERGRID_TRANSFER_LOOP_BLOCK */
+        size_t fp = 0;
+        for (size_t ce = 0; ce < dac->n_elem && !rc; ce++) {
+            const or_tn *co = &dac->octs[ce];
+            while (fp < daf->n_elem && !elem_inside_(daf, fp, co)) fp++;
+            if (fp >= daf->n_elem) { rc = -2; break; }
+            if (!g_hilbert && (daf->octs[fp].x != co->x || daf->octs[fp].y != co->y || daf->octs[fp].z != co->z)) {
+                rc = -2;
+                break;
+            }
+            uint32_t cNumCoarse = tn_morton_index(co);
+            int ctype = or_elem_type(cNumCoarse, dac->mask[ce]);
+            if (ctype < 0) { rc = -3; break; }
+            const uint32_t *cIdx = &dac->nlist[8 * ce];
+            int same = LEV(&daf->octs[fp]) == LEV(co);
+            int nf = same ? 1 : 8;
+            if (fp + nf > daf->n_elem) { rc = -2; break; }
+            for (int k = 0; k < nf && !rc; k++) {
+                size_t fe = fp + k;
+                if (!elem_inside_(daf, fe, co) || (!same && LEV(&daf->octs[fe]) != LEV(co) + 1)) { rc = -2; break; }
+                uint32_t cNumFine = same ? cNumCoarse : tn_morton_index(&daf->octs[fe]);
+                if (!g_hilbert && !same && cNumFine != (uint32_t)k) { rc = -2; break; }
+                uint8_t fhn = daf->mask[fe];
+                const uint32_t *fIdx = &daf->nlist[8 * fe];
+                const double *tab = same ? rmat2 + ((size_t)cNumCoarse * 18 + ctype) * 64
+                                         : rmat1 + (((size_t)cNumCoarse * 8 + cNumFine) * 18 + ctype) * 64;
+                for (int fCtr = 0; fCtr < 8; fCtr++) {
+                    if (fhn & (1u << fCtr)) continue;
+                    uint32_t fi = fIdx[fCtr];
+                    if (pass == 0) {
+                        uint32_t thisLev = LEV(&daf->octs[fe]), refLev = LEV(&daf->octs[fi]);
+                        uint8_t *d = &dummy[(size_t)fi * 16 + 2 * fCtr];
+                        uint8_t type = same ? (thisLev == refLev ? 0 : 2) : (thisLev == refLev ? 1 : 3);
+                        uint8_t scal = (thisLev != refLev && thisLev < refLev) ? 1 : 0;
+                        d[0] = (uint8_t)(1u | (scal << 1) | (type << 2) | (same ? 0 : (cNumFine << 4)));
+                        d[1] = (uint8_t)(cNumCoarse | ((uint32_t)ctype << 3));
+                        continue;
+                    }
+                    uint32_t fn = daf->local_to_node[fi];
+                    if (fn == 0xffffffffu) { rc = -4; break; }
+                    uint8_t fl = touched[(size_t)fi * 8 + fCtr];
+                    for (int cCtr = 0; cCtr < 8; cCtr++) {
+                        if (!(fl & (1u << cCtr))) continue;
+                        uint32_t cn = dac->local_to_node[cIdx[cCtr]];
+                        if (cn == 0xffffffffu) { rc = -4; break; }
+                        if ((supf && supf[fn]) || (supc && supc[cn])) continue;
+                        if (cnt >= cap) { rc = -1; break; }
+                        rows[cnt] = cn;
+                        cols[cnt] = fn;
+                        vals[cnt] = tab[cCtr * 8 + fCtr];
+                        cnt++;
+                    }
+                }
+            }
+            fp += nf;
+        }
+        if (pass == 0 && !rc) {
+            /* INTERGRID_TRANSFER_LOOP_BLOCK_DUMMY_FINAL_W / _A (:324-400) */
+            uint32_t maxD = daf->octs[0].md - 1, blen = 1u << maxD;
+            for (size_t fe = 0; fe < daf->n_elem; fe++) {
+                const or_tn *o = &daf->octs[fe];
+                uint8_t fhn = daf->mask[fe];
+                uint32_t len = 1u << (maxD - (LEV(o) - 1));
+                uint32_t bf = 0;
+                if (o->x + len == blen) bf |= 32u;
+                if (o->y + len == blen) bf |= 64u;
+                if (o->z + len == blen) bf |= 128u;
+                int nodes[8], nn = 0;
+                nodes[nn++] = 0;
+                if (bf) {
+                    if (bf & 32u) nodes[nn++] = 1;
+                    if (bf & 64u) nodes[nn++] = 2;
+                    if (bf & 128u) nodes[nn++] = 4;
+                    if ((bf & 96u) == 96u) nodes[nn++] = 3;
+                    if ((bf & 192u) == 192u) nodes[nn++] = 6;
+                    if ((bf & 160u) == 160u) nodes[nn++] = 5;
+                    if ((bf & 224u) == 224u) nodes[nn++] = 7;
+                }
+                for (int q = 0; q < nn; q++) {
+                    int nodeNum = nodes[q];
+                    if (fhn & (1u << nodeNum)) continue;
+                    size_t idx = nodeNum == 0 ? fe : daf->nlist[8 * fe + nodeNum];
+                    const uint8_t *d = &dummy[idx * 16];
+                    const uint16_t *mp[8];
+                    for (int df = 0; df < 8; df++) {
+                        mp[df] = NULL;
+                        uint8_t f0 = d[2 * df], f1 = d[2 * df + 1];
+                        if (!f0) continue;
+                        uint32_t st = (f0 & 12u) >> 2, cc = f1 & 7u, ct = (f1 & 248u) >> 3;
+                        uint32_t cf = (f0 & 112u) >> 4, sc = (f0 & 2u) >> 1;
+                        if (st >= 2 && df == 0) { rc = -2; break; }
+                        switch (st) {
+                            case 0: mp[df] = map1 + (((size_t)df * 8 + cc) * 18 + ct) * 8; break;
+                            case 1: mp[df] = map2 + ((((size_t)df * 8 + cf) * 8 + cc) * 18 + ct) * 8; break;
+                            case 2: mp[df] = map3 + (((((size_t)(df - 1) * 2 + sc) * 8 + cc) * 18) + ct) * 8; break;
+                            default:
+                                mp[df] = map4 + ((((((size_t)(df - 1) * 2 + sc) * 8 + cf) * 8 + cc) * 18) + ct) * 8;
+                                break;
+                        }
+                    }
+                    if (rc) break;
+                    uint8_t *t = &touched[idx * 8];
+                    for (int f = 0; f < 8; f++) {
+                        t[f] = 0;
+                        if (!mp[f]) continue;
+                        for (int c = 0; c < 8; c++) {
+                            uint16_t v = mp[f][c];
+                            int skip = 0;
+                            for (int df = 0; df < f && !skip; df++) {
+                                if (!mp[df]) continue;
+                                for (int k = 0; k < 8; k++)
+                                    if (mp[df][k] == v) { skip = 1; break; }
+                            }
+                            if (!skip) t[f] |= (uint8_t)(1u << c);
+                        }
+                    }
+                }
+                if (rc) break;
+            }
+        }
+    }
+    free(dummy);
+    free(touched);
+    *nnz = cnt;
+    return rc;
+}

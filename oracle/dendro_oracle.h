@@ -59,6 +59,11 @@ void or_da_element_view(const or_da *da, uint32_t *conn8, uint8_t *cnum, uint8_t
 size_t or_da_compress(const or_da *da, uint8_t *rem, uint8_t *mask, uint8_t *order, uint16_t *quot);
 void or_da_decode(const or_da *da, const uint8_t *rem, const uint8_t *mask, const uint8_t *order,
                   const uint16_t *quot, uint32_t *nlist);
+int or_coarsen(const or_tn *in, size_t n, or_tn **out, size_t *nout);
+int or_restrict_entries(const or_da *daf, const or_da *dac, const double *rmat1, const double *rmat2,
+                        const uint16_t *map1, const uint16_t *map2, const uint16_t *map3, const uint16_t *map4,
+                        const uint8_t *supf, const uint8_t *supc, uint32_t *rows, uint32_t *cols, double *vals,
+                        size_t *nnz);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,8 @@ def lib():
         L.or_da_compress.argtypes = [C.POINTER(_DA), vp, vp, vp, vp]
         L.or_da_compress.restype = C.c_size_t
         L.or_da_decode.argtypes = [C.POINTER(_DA), vp, vp, vp, vp, vp]
+        L.or_coarsen.argtypes = [vp, C.c_size_t, C.POINTER(vp), C.POINTER(C.c_size_t)]
+        L.or_restrict_entries.argtypes = [C.POINTER(_DA), C.POINTER(_DA)] + [vp] * 11 + [C.POINTER(C.c_size_t)]
         _lib = L
     return _lib
 
@@ -138,6 +140,15 @@ def balance(tn, inc_corner=True, emulate_limit=False):
     lim = 0 if emulate_limit is False else (1 if emulate_limit is True else int(emulate_limit))
     lib().or_balance(_p(tn), len(tn), 1 if inc_corner else 0, lim, C.byref(ptr), C.byref(n))
     return _take(ptr, n.value)
+
+
+def coarsen(tn):
+    """coarsenOctree on one rank (src/coarsen.cpp:64-104)."""
+    tn = np.ascontiguousarray(tn, np.uint32)
+    out, n = C.c_void_p(), C.c_size_t()
+    if lib().or_coarsen(_p(tn), len(tn), C.byref(out), C.byref(n)) < 0:
+        raise ValueError("coarsen: the octree does not start with a first child")
+    return _take(out, n.value)
 
 
 def is_sorted_unique(tn):
@@ -233,6 +244,36 @@ class DA:
         if rc:
             raise RuntimeError("unknown element type")
         return y
+
+    def restriction(self, coarse, tables, sup_fine=None, sup_coarse=None):
+        """The entries of restrictMatVecType1 from this (fine) DA to `coarse`
+        (src/omg/restrictMatVec.cpp:71-619): rows (coarse compact nodes), cols
+        (fine compact nodes) and values, in the reference's accumulation order.
+        Duplicates add.  tables: the arrays of tests/golden/rmat_stencils.npz.
+        sup_fine / sup_coarse: suppressed nodes (uint8 per compact node)."""
+        t = {k: np.ascontiguousarray(tables[k]) for k in ("rmat1", "rmat2", "map1", "map2", "map3", "map4")}
+        cap = 64 * self.n_elem
+        rows = np.zeros(cap, np.uint32)
+        cols = np.zeros(cap, np.uint32)
+        vals = np.zeros(cap, np.float64)
+        sf = None if sup_fine is None else np.ascontiguousarray(sup_fine, np.uint8)
+        sc = None if sup_coarse is None else np.ascontiguousarray(sup_coarse, np.uint8)
+        nnz = C.c_size_t(cap)
+        rc = lib().or_restrict_entries(C.byref(self._da), C.byref(coarse._da), _p(t["rmat1"]), _p(t["rmat2"]),
+                                       _p(t["map1"]), _p(t["map2"]), _p(t["map3"]), _p(t["map4"]),
+                                       None if sf is None else _p(sf), None if sc is None else _p(sc),
+                                       _p(rows), _p(cols), _p(vals), C.byref(nnz))
+        if rc:
+            raise RuntimeError(f"restriction entries failed ({rc})")
+        n = nnz.value
+        return rows[:n].copy(), cols[:n].copy(), vals[:n].copy()
+
+    def node_coords(self):
+        """Anchor of each compact node in the DA's embedded coordinates, (n_nodes, 3)."""
+        sel = self.local_to_node != 0xFFFFFFFF
+        out = np.zeros((self.n_nodes, 3), np.int64)
+        out[self.local_to_node[sel]] = self.octs[sel, :3]
+        return out
 
     def __del__(self):
         try:
