@@ -284,7 +284,8 @@ def matvec_only(dgx, ev, stream, W):
         assert hip.hipMemcpyAsync(dst.ptr, src.ptr, nb, 3, stream) == 0  # hipMemcpyDeviceToDevice
     dgx.synchronize()
     print(json.dumps({"workload": W, "matvec_avg_ms": mv["avg_ms"], "algorithmic_bytes": mv["algorithmic_bytes"],
-                      "GB_per_s": mv["GB_per_s"], "copy_bytes": nb}))
+                      "GB_per_s": mv["GB_per_s"], "copy_bytes": nb, "elements": da.element_size,
+                      "nodes": da.n_nodes, **da.block_info}))
 
 
 def main():

@@ -39,7 +39,7 @@ EXPORTED = [
     "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
     "dgx_write_nodes_binary", "dgx_write_vtk", "dgx_server_get_stats", "dgx_server_inject_fault",
     "dgx_mg_coarse_info", "dgx_direct_plan_host", "dgx_direct_plan_export", "dgx_direct_plan_free",
-    "dgx_da_matvec_noalloc", "dgx_memory_info", "dgx_mg_level_da", "dgx_da_boundary_flags",
+    "dgx_da_matvec_noalloc", "dgx_memory_info", "dgx_mg_level_da", "dgx_da_boundary_flags", "dgx_mg_restrict", "dgx_mg_prolong",
     "dgx_da_compress_maps", "dgx_da_decode_maps",
 ]
 
@@ -136,6 +136,8 @@ def load_library():
         "dgx_mg_coarse_info": ([vp, vp], i32),
         "dgx_memory_info": ([vp, i32], i32),
         "dgx_mg_level_da": ([vp, i32, C.POINTER(vp)], i32),
+        "dgx_mg_restrict": ([vp, i32, vp, vp], i32),
+        "dgx_mg_prolong": ([vp, i32, vp, vp], i32),
         "dgx_da_boundary_flags": ([vp, vp], i32),
         "dgx_da_compress_maps": ([vp, vp, vp, vp, vp, sz, C.POINTER(sz)], i32),
         "dgx_da_decode_maps": ([vp, vp, vp, vp, vp, sz, vp], i32),
@@ -480,6 +482,20 @@ class MG:
 
     def rhs(self, f, b):
         _check(_lib.dgx_mg_rhs(self._h, f.ptr, b.ptr))
+
+    def level_da(self, level):
+        """The DA of level `level` (0 = finest), owned by this MG."""
+        h = C.c_void_p()
+        _check(_lib.dgx_mg_level_da(self._h, level, C.byref(h)))
+        return DA._borrow(h, self)
+
+    def restrict(self, level, fine, coarse):
+        """coarse = R fine from level `level` to level + 1 (device arrays, FP64)."""
+        _check(_lib.dgx_mg_restrict(self._h, level, fine.ptr, coarse.ptr))
+
+    def prolong(self, level, coarse, fine):
+        """fine = P coarse from level + 1 to level `level` (device arrays, FP64)."""
+        _check(_lib.dgx_mg_prolong(self._h, level, coarse.ptr, fine.ptr))
 
     def solve(self, b, u, fp32=True, pre=2, post=2, omega=0.8, rtol=1e-8, maxit=200, graph=True):
         o = SolveOpts(1 if fp32 else 0, pre, post, omega, rtol, maxit, 1 if graph else 0)

@@ -548,6 +548,24 @@ int dgx_mg_rhs(dgx_mg m, const double *f, double *b) {
     mg_rhs_from_nodal(*m->mg, f, b, S());
     API_END
 }
+int dgx_mg_restrict(dgx_mg m, int level, const double *fine, double *coarse) {
+    API_BEGIN
+    ensure_ctx();
+    if (!m || !fine || !coarse) { dgx_set_error("restrict: null argument"); return -1; }
+    const int nl = (int)m->mg->lev.size();
+    if (level < 0 || level + 1 >= nl) { dgx_set_error("restrict: level %d has no coarser level (%d levels)", level, nl); return -1; }
+    mg_restrict(*m->mg, (size_t)level, fine, coarse, S());
+    API_END
+}
+int dgx_mg_prolong(dgx_mg m, int level, const double *coarse, double *fine) {
+    API_BEGIN
+    ensure_ctx();
+    if (!m || !fine || !coarse) { dgx_set_error("prolong: null argument"); return -1; }
+    const int nl = (int)m->mg->lev.size();
+    if (level < 0 || level + 1 >= nl) { dgx_set_error("prolong: level %d has no coarser level (%d levels)", level, nl); return -1; }
+    mg_prolong(*m->mg, (size_t)level, coarse, fine, S());
+    API_END
+}
 int dgx_mg_solve(dgx_mg m, const double *b, double *u, const dgx_solve_opts *opts, dgx_solve_stats *stats) {
     API_BEGIN
     ensure_ctx();

@@ -34,6 +34,11 @@ struct DaDevice {
     uint64_t n_pairs = 0, partial_slots = 0;
     DevBuf<uint32_t> bnode_off, bnode, bdst, soff, snode;
     DevBuf<uint16_t> lconn, cmeta;
+    // The same records packed into 12 bytes for blocks of at most kPipeMaxNodes
+    // nodes (the pipelined matvec): three words per element slot, bits 10j..10j+9
+    // the local index of vertex j, bits 80..95 cmeta.  Empty when a block
+    // touches more nodes.
+    DevBuf<uint32_t> lrec;
     DevBuf<double> partial;  // partial sums of shared nodes (float or double view), one per matvec in flight
     // Per node: local-buffer index of its octant, and 1 if the node lies on the
     // domain boundary (homogeneous Dirichlet rows, omgJac.C:958-1057).
@@ -42,6 +47,9 @@ struct DaDevice {
     uint32_t n_bdy_nodes = 0;
     ~DaDevice();
 };
+
+// Largest block node list the pipelined matvec (and its 10-bit packed records) takes.
+constexpr uint32_t kPipeMaxNodes = 1024;
 
 // colour: renumber each block's nodes against LDS bank conflicts (k_blk_color).
 // It makes every later matvec faster and costs about 40 matvecs once, so a

@@ -765,10 +765,41 @@ __global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint
             }
         __syncthreads();
     }
+    // Inside each colour the slots follow the node order (ascending ids), so
+    // that neighbouring slots hold nearby nodes and a wave's gathers and
+    // stores touch fewer lines.  Thread (part p, colour c) counts its colour
+    // in the p-th eighth of the list, then renumbers it.
+    {
+        static_assert(kColB == 8 * 16, "one thread per (eighth, colour)");
+        __shared__ uint32_t ccount[8][16];
+        const uint32_t c = tid & 15u, p = tid >> 4;
+        const uint32_t ch = (n + 7) / 8, i0 = min(n, p * ch), i1 = min(n, i0 + ch);
+        uint32_t k = 0;
+        for (uint32_t i = i0; i < i1; ++i) k += (newidx[i] & 15u) == c ? 1u : 0u;
+        ccount[p][c] = k;
+        __syncthreads();
+        uint32_t r = 0;
+        for (uint32_t q = 0; q < p; ++q) r += ccount[q][c];
+        // the colour bits of newidx[] do not change, so other threads' tests stay valid
+        for (uint32_t i = i0; i < i1; ++i)
+            if ((newidx[i] & 15u) == c) newidx[i] = (uint16_t)((r++) * 16u + c);
+        __syncthreads();
+    }
     for (uint32_t t = tid; t < nuse; t += kColB) rec[t] = (uint16_t)(newidx[rec[t] >> 3] << 3);
     for (uint32_t i = tid; i < n; i += kColB) tmp[i] = bnode[s0 + i];
     __syncthreads();
     for (uint32_t i = tid; i < n; i += kColB) bnode[s0 + newidx[i]] = tmp[i];
+}
+// 12-byte element records: the 8 local indices (10 bits each, from lconn's
+// byte offsets of 8-byte slots) and cmeta in the top 16 bits.
+__global__ void k_blk_pack(const uint16_t *lconn, const uint16_t *cmeta, size_t n, uint32_t *rec) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    uint32_t v[8];
+    for (int j = 0; j < 8; ++j) v[j] = (lconn[8 * e + j] >> 3) & 1023u;
+    rec[3 * e] = v[0] | v[1] << 10 | v[2] << 20 | v[3] << 30;
+    rec[3 * e + 1] = v[3] >> 2 | v[4] << 8 | v[5] << 18 | v[6] << 28;
+    rec[3 * e + 2] = v[6] >> 4 | v[7] << 6 | (uint32_t)cmeta[e] << 16;
 }
 __global__ void k_blk_maxnodes(const uint32_t *boff, uint32_t nblk, uint32_t *mx) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -865,6 +896,12 @@ void da_build_blocks(DaDevice &da, hipStream_t s, bool colour) {
     if ((uint64_t)hmx * 8 > 65535) {  // lconn holds 16-bit byte offsets of 8-byte slots
         dgx_set_error("DA: an element block touches %u nodes, more than the 8191 its records can address", hmx);
         throw DgxError();
+    }
+    if (hmx <= kPipeMaxNodes) {
+        const size_t slots = (size_t)nblk * eb;
+        da.lrec.alloc(3 * slots, s);
+        k_blk_pack<<<grid_for(slots, B), B, 0, s>>>(da.lconn.p, da.cmeta.p, slots, da.lrec.p);
+        HIP_OK(hipGetLastError());
     }
     da.n_blocks = nblk;
     da.n_pairs = np;

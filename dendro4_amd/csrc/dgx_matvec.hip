@@ -263,6 +263,19 @@ __device__ __forceinline__ void unpack_record(const uint4 v, uint32_t *off) {
     off[6] = (v.w & 0xffffu) >> SH;
     off[7] = (v.w >> 16) >> SH;
 }
+// 12-byte records (DaDevice::lrec): 10-bit local indices of the 8 vertices in
+// bits 10j..10j+9 of the 96-bit record, cmeta in bits 80..95.  Offsets come
+// out as byte offsets of 8-byte slots, as unpack_record<0> gives them.
+__device__ __forceinline__ void unpack_rec12(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t *off) {
+    off[0] = (w0 & 1023u) << 3;
+    off[1] = ((w0 >> 10) & 1023u) << 3;
+    off[2] = ((w0 >> 20) & 1023u) << 3;
+    off[3] = (__builtin_amdgcn_alignbit(w1, w0, 30) & 1023u) << 3;
+    off[4] = ((w1 >> 8) & 1023u) << 3;
+    off[5] = ((w1 >> 18) & 1023u) << 3;
+    off[6] = (__builtin_amdgcn_alignbit(w2, w1, 28) & 1023u) << 3;
+    off[7] = ((w2 >> 6) & 1023u) << 3;
+}
 template <int SH>
 __device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, uint32_t *off) {
     unpack_record<SH>(*reinterpret_cast<const uint4 *>(lconn + e * 8), off);
@@ -337,8 +350,7 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
 // x); EPI_RESID: y_i = b_i - v.  In place is safe: a block writes only nodes
 // no other block reads, and shared nodes are finished by the second pass.
 template <typename T, bool DIR, int NPT, int EPI>
-__global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ lconn,
-                                                   const uint16_t *__restrict__ cmeta, size_t ne, uint32_t nblk,
+__global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict__ rec, size_t ne, uint32_t nblk,
                                                    const uint32_t *__restrict__ boff,
                                                    const uint32_t *__restrict__ bnode,
                                                    const uint32_t *__restrict__ bdst, const T *x, T *y,
@@ -357,12 +369,14 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
     if (b >= hi) return;  // uniform over the workgroup
     init_lambda<T>(lam, lap_fac, mass_fac);
 
-    auto load_elems = [&](uint32_t bb, uint4 *ev, uint32_t *mt) {
+    auto load_elems = [&](uint32_t bb, uint3 *ev) {
         const size_t e = (size_t)bb * EB + threadIdx.x;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            ev[k] = *reinterpret_cast<const uint4 *>(lconn + (e + k * B) * 8);
-            mt[k] = cmeta[e + k * B];
+            const uint32_t *r = rec + (e + k * B) * 3;
+            ev[k].x = r[0];
+            ev[k].y = r[1];
+            ev[k].z = r[2];
         }
     };
     auto load_entries = [&](uint32_t s0, uint32_t n, uint32_t *g) {
@@ -381,10 +395,10 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
 
     // block b: element records, node entries, x values
     uint32_t s0 = boff[b], nb = boff[b + 1] - s0;
-    uint4 ev[PER];
-    uint32_t mt[PER], g[NPT], sd[NPT];
+    uint3 ev[PER];
+    uint32_t g[NPT], sd[NPT];
     T xv[NPT];
-    load_elems(b, ev, mt);
+    load_elems(b, ev);
     load_entries(s0, nb, g);
     load_slots(s0, nb, sd);
     gather(g, xv);
@@ -427,11 +441,11 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
         }
         __syncthreads();
         const bool moreA = bA < hi;
-        uint4 evA[PER];
-        uint32_t mtA[PER], gB[NPT], sdA[NPT];
+        uint3 evA[PER];
+        uint32_t gB[NPT], sdA[NPT];
         if (moreA) {
             gather(gA, xv);
-            load_elems(bA, evA, mtA);
+            load_elems(bA, evA);
             load_slots(s0A, nA, sdA);
         }
         if (bB < hi) load_entries(s0B, nB, gB);
@@ -445,8 +459,8 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             uint32_t off[8];
-            unpack_record<0>(ev[k], off);
-            accumulate_element<T>(off, mt[k], e0 + k * B < ne, xb, yb, lam);
+            unpack_rec12(ev[k].x, ev[k].y, ev[k].z, off);
+            accumulate_element<T>(off, ev[k].z >> 16, e0 + k * B < ne, xb, yb, lam);
         }
         __syncthreads();
 #pragma unroll
@@ -472,10 +486,7 @@ __global__ void __launch_bounds__(B) k_matvec_pipe(const uint16_t *__restrict__ 
         s0 = s0A;
         nb = nA;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            ev[k] = evA[k];
-            mt[k] = mtA[k];
-        }
+        for (int k = 0; k < PER; ++k) ev[k] = evA[k];
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             g[i] = gA[i];
@@ -578,10 +589,11 @@ void matvec_blocks(const DaDevice &da, const T *x, T *y, T lap_fac, T mass_fac, 
     // DGX_MATVEC_BLOCKED=1 takes the path of DAs with large blocks for every
     // DA (a test hook: real meshes rarely reach it)
     const char *forced = getenv("DGX_MATVEC_BLOCKED");
-    if (mn <= 4 * B && !(forced && forced[0] == '1')) {
+    static_assert(4 * B == kPipeMaxNodes, "pipelined matvec block size");
+    if (mn <= kPipeMaxNodes && da.lrec.p && !(forced && forced[0] == '1')) {
         const unsigned grid =
             pipe_grid(reinterpret_cast<const void *>(&k_matvec_pipe<T, DIR, 4, EPI>), da.n_blocks);
-        k_matvec_pipe<T, DIR, 4, EPI><<<grid, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.n_blocks,
+        k_matvec_pipe<T, DIR, 4, EPI><<<grid, B, 0, s>>>(da.lrec.p, da.n_elem, da.n_blocks,
                                                          da.bnode_off.p, da.bnode.p, da.bdst.p, x, y, partial,
                                                          (double)lap_fac, (double)mass_fac, bvec, dinv, omega);
         if (da.n_shared)

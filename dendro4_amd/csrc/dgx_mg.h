@@ -83,3 +83,6 @@ void mg_solve(MgHierarchy &mg, const double *b, double *u, const SolveOpts &o, S
 void mg_vcycle_apply(MgHierarchy &mg, const double *r, double *z, const SolveOpts &o, hipStream_t s);
 // FE load vector b = D M f (mass-weighted nodal f, zero on boundary nodes).
 void mg_rhs_from_nodal(MgHierarchy &mg, const double *f, double *b, hipStream_t s);
+// coarse = R fine / fine = P coarse between levels k and k + 1 (FP64 compact node vectors).
+void mg_restrict(MgHierarchy &mg, size_t k, const double *fine, double *coarse, hipStream_t s);
+void mg_prolong(MgHierarchy &mg, size_t k, const double *coarse, double *fine, hipStream_t s);

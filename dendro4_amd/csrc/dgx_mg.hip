@@ -866,6 +866,25 @@ void mg_rhs_from_nodal(MgHierarchy &mg, const double *f, double *b, hipStream_t 
     HIP_OK(hipGetLastError());
 }
 
+// coarse = R fine and fine = P coarse between level k (finer) and k + 1, on
+// compact node vectors in FP64: the transfers the V-cycle applies
+// (restrictMatVecType1 / prolongMatVecType1 with the boundary nodes
+// suppressed, src/omg/restrictMatVec.cpp:486-619, prolongMatVec.cpp:154-292).
+void mg_restrict(MgHierarchy &mg, size_t k, const double *fine, double *coarse, hipStream_t s) {
+    const Transfer &t = mg.lev[k + 1].to_finer;
+    if (!t.nc) return;
+    k_restrict<double><<<grid_for(t.nc * 8, B), B, 0, s>>>(t.roff.p, t.rcol.p, t.rval.p, (uint32_t)t.nc, fine, coarse);
+    HIP_OK(hipGetLastError());
+}
+void mg_prolong(MgHierarchy &mg, size_t k, const double *coarse, double *fine, hipStream_t s) {
+    const Transfer &t = mg.lev[k + 1].to_finer;
+    const uint32_t n = mg.lev[k].da->n_nodes;
+    if (!n) return;
+    k_zero<<<grid_for(n, B), B, 0, s>>>(n, fine);
+    k_prolong_add<double><<<grid_for(n, B), B, 0, s>>>(t.poff.p, t.pcol.p, t.pval.p, n, coarse, fine);
+    HIP_OK(hipGetLastError());
+}
+
 void mg_solve(MgHierarchy &mg, const double *b, double *xout, const SolveOpts &o, SolveStats *st, hipStream_t s) {
     auto t0 = std::chrono::steady_clock::now();
     MgWork &w = work_of(mg, s);

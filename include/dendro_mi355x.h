@@ -188,6 +188,18 @@ int dgx_mg_info(dgx_mg mg, int *nlevels, uint64_t *octants, uint64_t *nodes);
 int dgx_mg_level_da(dgx_mg mg, int level, dgx_da *da);
 /* The finest level's DA (owned by mg; do not free). */
 int dgx_mg_finest_da(dgx_mg mg, dgx_da *da);
+/* coarse = R fine: restrictMatVecType1 (src/omg/restrictMatVec.cpp:486-619)
+ * from level `level` to level + 1 (0 = finest), with the Dirichlet boundary
+ * nodes of both levels suppressed as DAMGCreateSuppressedDOFs marks them
+ * (ITLB_SET_VALUE_SUPPRESSED_DOFS, :71-82).  R holds, for every fine node i
+ * and coarse node j, the value at i of the coarse basis function of j (the
+ * trilinear interpolation with the coarse element's hanging-node constraints,
+ * which is what RmatType1/RmatType2Stencils.inp tabulate), each pair once.
+ * Device pointers to compact node vectors of the two levels. */
+int dgx_mg_restrict(dgx_mg mg, int level, const double *fine, double *coarse);
+/* fine = P coarse = R^T coarse: prolongMatVecType1 (src/omg/prolongMatVec.cpp:
+ * 154-292) from level + 1 to level. */
+int dgx_mg_prolong(dgx_mg mg, int level, const double *coarse, double *fine);
 /* Load vector b = M f of a nodal f with boundary entries set to zero. */
 int dgx_mg_rhs(dgx_mg mg, const double *f, double *b);
 
