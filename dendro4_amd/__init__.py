@@ -471,9 +471,6 @@ class MG:
         self.nlevels = n.value
         self.level_octants = list(octs)[: n.value]
         self.level_nodes = list(nodes)[: n.value]
-        h = C.c_void_p()
-        _check(_lib.dgx_mg_finest_da(self._h, C.byref(h)))
-        self.da = DA._borrow(h, self)
         self.n_nodes = self.level_nodes[0]
         ci = (C.c_uint64 * 6)()
         _check(_lib.dgx_mg_coarse_info(self._h, ci))
@@ -482,6 +479,15 @@ class MG:
 
     def rhs(self, f, b):
         _check(_lib.dgx_mg_rhs(self._h, f.ptr, b.ptr))
+
+    @property
+    def da(self):
+        """The finest level's DA, owned by this MG.  Made on each access, so
+        that the MG holds no reference to it and is freed as soon as the
+        caller drops it."""
+        h = C.c_void_p()
+        _check(_lib.dgx_mg_finest_da(self._h, C.byref(h)))
+        return DA._borrow(h, self)
 
     def level_da(self, level):
         """The DA of level `level` (0 = finest), owned by this MG."""

@@ -5,6 +5,9 @@
 // (bin/optionsFBM:57, src/omg/blockDiag.cpp:780-1047 and 1172-1308), a sparse
 // LU of the coarsest level's matrix through PETSc.
 #include "dgx_direct.h"
+#include <cstdio>
+#include <string>
+#include <chrono>
 
 #include <algorithm>
 #include <atomic>
@@ -156,6 +159,20 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
                const uint32_t *xyz, uint64_t max_entries, uint32_t leaf, uint32_t dense) {
     MfPlan P;
     P.n = n;
+    static const bool timing = [] {
+        const char *e = getenv("DGX_SETUP_TIMING");
+        return e && e[0] == '1';
+    }();
+    auto tp = std::chrono::steady_clock::now();
+    std::string tlog;
+    auto lap = [&](const char *what) {
+        if (!timing) return;
+        const auto now = std::chrono::steady_clock::now();
+        char b[64];
+        snprintf(b, sizeof b, " %s %.2f", what, std::chrono::duration<double, std::milli>(now - tp).count());
+        tlog += b;
+        tp = now;
+    };
     if (E * 64 > 0xffffffffull) {
         dgx_set_error("coarse solver: %zu elements exceed the 32-bit contribution index", E);
         throw DgxError();
@@ -218,6 +235,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
         for (auto &pv : part) rcol.insert(rcol.end(), pv.begin(), pv.end());
     }
     const uint32_t nnz = roff[ni];
+    lap("pattern");
     std::vector<uint32_t> cxyz(3 * (size_t)ni);
     for (uint32_t i = 0; i < ni; ++i)
         for (int a = 0; a < 3; ++a) cxyz[3 * (size_t)i + a] = xyz[3 * (size_t)P.inode[i] + a];
@@ -230,6 +248,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
         D.run(std::move(all));
     }
     const uint32_t nf = (uint32_t)D.S.size();
+    lap("dissection");
     std::vector<uint32_t> front_of(ni), sstart(nf), send(nf), at_pos(ni);
     for (uint32_t f = 0; f < nf; ++f) {
         sstart[f] = D.pos[D.S[f][0]];
@@ -264,6 +283,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
             U[f].swap(cand);
         }
     }
+    lap("update-sets");
     // fronts
     P.fronts.resize(nf);
     uint64_t off = 0, upd = 0;
@@ -341,6 +361,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
             fmo += (uint32_t)uc.size();
         }
     }
+    lap("fronts+maps");
     // The entries each front owns: (i, j) whose first-eliminated node is in S.
     // They lie in the rows of S (columns eliminated at S or later) and in the
     // rows of U (columns in S).  Fronts are independent: each range of fronts
@@ -378,6 +399,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
         dgx_set_error("coarse solver: a matrix entry falls outside its front");
         throw DgxError();
     }
+    lap("entry-places");
     // element-matrix contributions of each entry, in element order: row i's
     // entries come from its incident elements (in element order), each
     // element adding vals[e*64 + k*8 + j] to (ids[j^c], ids[k^c])
@@ -418,6 +440,7 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
         P.a_src.reserve(P.a_off[nnz]);
         for (auto &pv : part) P.a_src.insert(P.a_src.end(), pv.begin(), pv.end());
     }
+    lap("contributions");
     // levels by height, with their work items and scratch
     uint32_t H = 0;
     for (uint32_t f = 0; f < nf; ++f) H = std::max(H, P.height[f] + 1);
@@ -452,6 +475,8 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
         dgx_set_error("coarse solver: %u fronts in one tree level (limit 65535)", P.dinv_max);
         throw DgxError();
     }
+    lap("levels");
+    if (timing) fprintf(stderr, "mf_plan: %u interior nodes, %zu elements, %u fronts, ms:%s\n", ni, E, nf, tlog.c_str());
     return P;
 }
 

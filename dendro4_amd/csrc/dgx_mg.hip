@@ -23,6 +23,7 @@
 
 #include "../../include/dendro_mi355x.h"  // dgx_last_error
 
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <functional>
@@ -417,10 +418,23 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
 
 // Coarsest level: symbolic analysis (nested dissection, host) and numeric
 // factorisation (device) of the sparse FP64 direct solver (dgx_direct.hip).
-static void build_coarse(MgHierarchy &mg, hipStream_t s) {
+static double ms_since(std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+static void build_coarse(MgHierarchy &mg, hipStream_t s, double *ms_analyse = nullptr, double *ms_factor = nullptr) {
     const DaDevice &d = *mg.lev.back().da;
+    auto t0 = std::chrono::steady_clock::now();
     mg.coarse = coarse_analyse(d, s);
+    if (ms_analyse) {
+        HIP_OK(hipStreamSynchronize(s));
+        *ms_analyse = ms_since(t0);
+    }
+    t0 = std::chrono::steady_clock::now();
     coarse_factor(*mg.coarse, d, mg.lap_fac, mg.mass_fac, s);
+    if (ms_factor) {
+        HIP_OK(hipStreamSynchronize(s));
+        *ms_factor = ms_since(t0);
+    }
 }
 
 namespace {
@@ -562,6 +576,15 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     mg->lap_fac = lap_fac;
     mg->mass_fac = mass_fac;
     if (nlevels < 1) nlevels = 1;
+    // DGX_SETUP_TIMING=1: one line per set-up on stderr with the wall time of
+    // each phase (diagnostics; the phases are timed on the thread that runs them)
+    static const bool timing = [] {
+        const char *e = getenv("DGX_SETUP_TIMING");
+        return e && e[0] == '1';
+    }();
+    const auto t_start = std::chrono::steady_clock::now();
+    double t_cb = 0, t_wait = 0, t_tr = 0, t_an = 0, t_fa = 0;
+    std::atomic<double> t_da{0.0};
     mg->lev.reserve(32);  // the helper holds pointers to levels: no reallocation (at most 30 levels)
     mg->lev.emplace_back();
     MgLevel &L0 = mg->lev.back();
@@ -574,9 +597,14 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     // needs a level's node count only once the requested levels exist.
     SetupHelper helper(overlap ? helper_stream() : s, s, overlap);
     auto build_da = [&, md, curve, colour](MgLevel *L) {
-        return helper.submit([L, md, curve, lap_fac, mass_fac, colour](hipStream_t hs) {
+        return helper.submit([L, md, curve, lap_fac, mass_fac, colour, &t_da](hipStream_t hs) {
+            const auto t0 = std::chrono::steady_clock::now();
             L->da = da_build(L->keys.p, L->n_oct, md, curve, hs, colour);
             build_level_diag(*L, lap_fac, mass_fac, hs);
+            if (timing) {
+                HIP_OK(hipStreamSynchronize(hs));
+                t_da = t_da + ms_since(t0);
+            }
         });
     };
     size_t last = build_da(&L0);
@@ -585,12 +613,16 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
     // hierarchy coarsen further.
     for (int k = 1; k < 30; ++k) {
         if (k >= nlevels) {
+            const auto tw = std::chrono::steady_clock::now();
             helper.wait(last);
+            t_wait += ms_since(tw);
             if (mg->lev.back().da->n_nodes <= kMaxDirectNodes) break;
         }
         MgLevel &prev = mg->lev.back();
+        const auto tc = std::chrono::steady_clock::now();
         DevBuf<Key> c = coarsen_build(prev.keys.p, prev.n_oct, s);
         DevBuf<Key> b = balance_build(c.p, c.n, md, curve, s, nullptr);  // synchronises s
+        t_cb += ms_since(tc);
         if (b.n >= prev.n_oct || b.n <= 8) break;  // nothing left to coarsen
         mg->lev.emplace_back();
         MgLevel &L = mg->lev.back();
@@ -599,12 +631,27 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
         last = build_da(&L);
         if (L.n_oct < 500) break;  // blockDiag.cpp:1417-1420
     }
-    helper.wait_all();
+    {
+        const auto tw = std::chrono::steady_clock::now();
+        helper.wait_all();
+        t_wait += ms_since(tw);
+    }
     // the coarse solver on the helper, the transfers here
-    helper.submit([&mg](hipStream_t hs) { build_coarse(*mg, hs); });
+    helper.submit([&mg, &t_an, &t_fa](hipStream_t hs) {
+        build_coarse(*mg, hs, timing ? &t_an : nullptr, timing ? &t_fa : nullptr);
+    });
+    const auto tt = std::chrono::steady_clock::now();
     for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
+    if (timing) HIP_OK(hipStreamSynchronize(s));
+    t_tr = ms_since(tt);
     helper.wait_all();
     HIP_OK(hipStreamSynchronize(s));
+    if (timing)
+        fprintf(stderr,
+                "mg_build: %zu octants, %zu levels, coarsest %u nodes; total %.2f ms: coarsen+balance %.2f, "
+                "DA+diagonal %.2f (helper), waits %.2f, transfers %.2f, coarse analysis %.2f, factorisation %.2f\n",
+                n, mg->lev.size(), mg->lev.back().da->n_nodes, ms_since(t_start), t_cb, t_da.load(), t_wait, t_tr,
+                t_an, t_fa);
     return mg;
 }
 

@@ -67,6 +67,27 @@ def test_dirichlet_matvec_parity(dgx, problem):
     assert np.abs(yn - A @ x).max() / np.abs(A @ x).max() < 1e-12
 
 
+def test_dropped_hierarchy_returns_its_memory(dgx, problem):
+    """An MG the caller drops is freed at once (no reference cycle keeps it):
+    the library's live bytes go back to where they were, and a hierarchy
+    built again takes the same bytes."""
+    ob = problem[0]
+    dgx.synchronize()
+    base = dgx.memory_info()["library_live"]
+    mg = dgx.MG(ob, 5)
+    _ = mg.da.n_nodes  # the borrowed finest DA must not keep the MG alive
+    held = dgx.memory_info()["library_live"] - base
+    assert held > 0
+    mg = None
+    dgx.synchronize()
+    assert dgx.memory_info()["library_live"] == base
+    mg = dgx.MG(ob, 5)
+    assert dgx.memory_info()["library_live"] - base == held
+    mg = None
+    dgx.synchronize()
+    assert dgx.memory_info()["library_live"] == base
+
+
 @pytest.mark.parametrize("nlevels", [1, 3, 5])
 def test_mg_solve_matches_direct_solve(dgx, problem, nlevels):
     ob, rda, A, bd = problem
