@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -26,10 +27,6 @@
 
 namespace {
 
-struct Cached {
-    void *p;
-    hipStream_t s;
-};
 struct Live {
     size_t bytes;
     hipStream_t s;
@@ -42,7 +39,9 @@ struct TagUse {
 std::mutex g_mu;
 std::unordered_map<void *, Live> g_live;
 std::unordered_map<uint64_t, TagUse> g_tags;  // tag 0 (untagged) is not tracked
-std::multimap<size_t, Cached> g_cache;        // class bytes -> free block
+// free blocks per stream: class bytes -> block.  Per stream, so that an
+// allocation and a make-room pass look only at the blocks they can use
+std::unordered_map<hipStream_t, std::multimap<size_t, void *>> g_cache;
 size_t g_cached_bytes = 0;
 size_t g_live_bytes = 0, g_live_peak = 0;     // bytes handed out and not freed, and their peak
 constexpr size_t kCacheCap = size_t(8) << 30;  // bytes held back from the pool at most
@@ -71,15 +70,13 @@ void note_alloc_locked(void *p, size_t cls, hipStream_t s) {
 
 // Return the blocks cached for stream s to the pool (queued on s).
 void trim_stream_locked(hipStream_t s) {
-    for (auto it = g_cache.begin(); it != g_cache.end();) {
-        if (it->second.s == s) {
-            (void)hipFreeAsync(it->second.p, s);
-            g_cached_bytes -= it->first;
-            it = g_cache.erase(it);
-        } else {
-            ++it;
-        }
+    auto m = g_cache.find(s);
+    if (m == g_cache.end()) return;
+    for (auto &kv : m->second) {
+        (void)hipFreeAsync(kv.second, s);
+        g_cached_bytes -= kv.first;
     }
+    g_cache.erase(m);
 }
 
 // The cap is full: return blocks cached for other streams to the pool (each
@@ -89,15 +86,23 @@ void trim_stream_locked(hipStream_t s) {
 // the pool for every block.
 // Streams inside a graph capture are skipped: a free queued there would join
 // the capture.
+// Largest blocks first, so that few frees make the room.
 void make_room_locked(size_t need, hipStream_t keep) {
-    for (auto it = g_cache.begin(); it != g_cache.end() && g_cached_bytes + need > kCacheCap;) {
-        if (it->second.s != keep && !g_capturing.count(it->second.s)) {
-            (void)hipFreeAsync(it->second.p, it->second.s);
-            g_cached_bytes -= it->first;
-            it = g_cache.erase(it);
-        } else {
-            ++it;
+    for (auto m = g_cache.begin(); m != g_cache.end() && g_cached_bytes + need > kCacheCap;) {
+        const hipStream_t st = m->first;
+        if (st == keep || g_capturing.count(st)) {
+            ++m;
+            continue;
         }
+        auto &blocks = m->second;
+        while (!blocks.empty() && g_cached_bytes + need > kCacheCap) {
+            auto it = std::prev(blocks.end());
+            (void)hipFreeAsync(it->second, st);
+            g_cached_bytes -= it->first;
+            blocks.erase(it);
+        }
+        if (blocks.empty()) m = g_cache.erase(m);
+        else ++m;
     }
 }
 
@@ -108,13 +113,16 @@ void *dgx_alloc(size_t bytes, hipStream_t s) {
     {
         std::lock_guard<std::mutex> lk(g_mu);
         // smallest cached block of this class or up to 25% larger, same stream
-        for (auto it = g_cache.lower_bound(cls); it != g_cache.end() && it->first <= cls + cls / 4; ++it) {
-            if (it->second.s != s) continue;
-            void *p = it->second.p;
-            note_alloc_locked(p, it->first, s);
-            g_cached_bytes -= it->first;
-            g_cache.erase(it);
-            return p;
+        auto m = g_cache.find(s);
+        if (m != g_cache.end()) {
+            auto it = m->second.lower_bound(cls);
+            if (it != m->second.end() && it->first <= cls + cls / 4) {
+                void *p = it->second;
+                note_alloc_locked(p, it->first, s);
+                g_cached_bytes -= it->first;
+                m->second.erase(it);
+                return p;
+            }
         }
     }
     void *p = nullptr;
@@ -162,13 +170,14 @@ void dgx_free(void *p, hipStream_t s) {
         (void)hipFreeAsync(p, as);
         return;
     }
-    g_cache.emplace(cls, Cached{p, as});
+    g_cache[as].emplace(cls, p);
     g_cached_bytes += cls;
 }
 
 void dgx_alloc_trim() {
     std::lock_guard<std::mutex> lk(g_mu);
-    for (auto &kv : g_cache) (void)hipFreeAsync(kv.second.p, kv.second.s);
+    for (auto &m : g_cache)
+        for (auto &kv : m.second) (void)hipFreeAsync(kv.second, m.first);
     g_cache.clear();
     g_cached_bytes = 0;
 }

@@ -102,6 +102,7 @@ struct dgx_server_s {
     std::unordered_map<uint64_t, Done> done;
     std::unordered_set<uint64_t> consumed;  // ids whose result dgx_server_wait has handed out
     std::list<Resident> cache;  // front = most recently used
+    std::list<Resident> evicted;  // taken out of the cache under mu, freed by the evicting worker after it
     uint64_t reserved = 0;      // estimates of running requests + bytes of resident problems
     uint64_t next_id = 1;
     uint64_t running = 0, evictions = 0;
@@ -132,7 +133,8 @@ struct dgx_server_s {
                 if (!it->busy) victim = it;  // last idle entry = least recently used
             if (victim == cache.end()) return false;
             reserved -= victim->bytes;
-            cache.erase(victim);
+            // freed outside the lock by the worker that evicts (its stream)
+            evicted.splice(evicted.end(), cache, victim);
             ++evictions;
         }
         return true;
@@ -147,6 +149,7 @@ struct dgx_server_s {
         // streams, which may be capturing a graph right now: their (finished)
         // memory is returned on this thread's stream instead.
         dgx_alloc_free_stream(s);
+        std::list<Resident> dead;
         for (;;) {
             Request rq;
             {
@@ -162,7 +165,11 @@ struct dgx_server_s {
                 queue.pop_front();
                 reserved += rq.est;
                 ++running;
+                dead.splice(dead.end(), evicted);
             }
+            // the problems this admission evicted: their memory goes back here,
+            // without the lock, before this request allocates
+            dead.clear();
             Done d;
             d.r.id = rq.id;
             d.r.t_submit = rq.t_submit;
@@ -509,6 +516,7 @@ int dgx_server_free(dgx_server sv) {
     for (auto &t : sv->workers) t.join();
     (void)hipSetDevice(sv->device);
     sv->cache.clear();  // frees the resident problems into the workers' streams
+    sv->evicted.clear();
     for (hipStream_t st : sv->streams) {
         dgx_alloc_release_stream(st);
         (void)hipStreamDestroy(st);
