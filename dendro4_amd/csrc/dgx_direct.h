@@ -70,7 +70,13 @@ struct MfPlan {
 // Limits of the device kernels: a front's size and separator must fit the
 // LDS vectors of the solve sweeps (8192 doubles = 64 KiB).
 constexpr uint32_t kMfMaxFront = 8192;
-constexpr uint32_t kDenseNodes = 2048;  // at most this many interior nodes: one dense front
+// At most this many interior nodes make one dense front.  Its inverse costs
+// more to factorise (about 5 ms at 2800 nodes) and less to apply than the
+// multifrontal tree (one GEMV against two launches per tree level and V-cycle:
+// lognormal-4M solves 331 -> 366/s, set-up 15.8 -> 21.4 ms), so hierarchies
+// solved once (the request server) keep the smaller kDenseNodesOneSolve.
+constexpr uint32_t kDenseNodes = 4096;
+constexpr uint32_t kDenseNodesOneSolve = 2048;
 
 // Throws DgxError when a front exceeds kMfMaxFront or the fronts exceed
 // max_entries doubles.  conn is [8][E] (conn[j*E + e]), meta as DaDevice::meta,
@@ -95,7 +101,7 @@ struct CoarseSolver {
 };
 
 // Symbolic analysis of the coarsest DA (host) and upload of the plan.
-std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s);
+std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, uint32_t dense = kDenseNodes);
 // Numeric factorisation for the operator lap_fac K + mass_fac M with Dirichlet
 // rows; throws DgxError when a pivot is not positive.
 void coarse_factor(CoarseSolver &cs, const DaDevice &d, double lap_fac, double mass_fac, hipStream_t s);

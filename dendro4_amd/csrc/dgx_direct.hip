@@ -21,6 +21,23 @@
 // ------------------------------------------------------------ host plan ----
 namespace {
 
+// Runs body(lo, hi, t) over [0, n) in up to 8 contiguous ranges on host
+// threads; each range writes only its own outputs, so the result does not
+// depend on the split.
+template <typename F>
+void parallel_ranges(size_t n, size_t grain, F body) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned t = (unsigned)std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n / grain));
+    if (t <= 1) {
+        body(size_t(0), n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < t; ++k) th.emplace_back(body, n * k / t, n * (k + 1) / t, k);
+    body(size_t(0), n / t, 0u);
+    for (auto &x : th) x.join();
+}
+
 // Geometric nested dissection of the interior-node graph.  A part is split by
 // a plane c_a = t: L = nodes with c_a < t, the separator S = nodes with
 // c_a >= t that have a neighbour in L, R = the rest; no edge joins L and R.
@@ -33,34 +50,48 @@ namespace {
 // the smallest coordinate over a node and its neighbours in the part.
 struct Dissection {
     static constexpr uint32_t kBins = 256;
+    // parts at least this large split their two halves over two host threads,
+    // down to kParDepth levels of the tree (at most 2^kParDepth threads)
+    static constexpr size_t kParMin = 8192;
+    static constexpr int kParDepth = 3;
     uint32_t ni = 0, leaf = 128;
     const std::vector<uint32_t> &roff, &rcol;  // interior graph (diagonal included)
     const std::vector<uint32_t> &xyz;          // 3 per interior node
-    std::vector<uint32_t> stamp;
-    uint32_t cur = 0;
-    std::vector<std::vector<uint32_t>> S, kids;
-    std::vector<uint32_t> pos;
-    uint32_t next_pos = 0;
+    // stamp[v] = id of the part v was last placed in; parts split concurrently
+    // are disjoint, and a neighbour in another part never carries this part's
+    // id, so relaxed accesses suffice
+    std::vector<std::atomic<uint32_t>> stamp;
+    std::atomic<uint32_t> next_id{0};
+
+    // A subtree: its fronts in creation order (children before parents), each
+    // front's nodes and children (indices into this list), and its roots.
+    struct Forest {
+        std::vector<std::vector<uint32_t>> S, kids;
+        std::vector<uint32_t> roots;
+    };
 
     Dissection(const std::vector<uint32_t> &ro, const std::vector<uint32_t> &rc, const std::vector<uint32_t> &c,
                uint32_t n_int, uint32_t lf)
-        : ni(n_int), leaf(lf), roff(ro), rcol(rc), xyz(c), stamp(n_int, 0), pos(n_int, 0) {}
+        : ni(n_int), leaf(lf), roff(ro), rcol(rc), xyz(c), stamp(n_int) {
+        for (auto &a : stamp) a.store(0, std::memory_order_relaxed);
+    }
 
-    uint32_t new_front(std::vector<uint32_t> &&sep, std::vector<uint32_t> &&ch) {
-        for (uint32_t v : sep) pos[v] = next_pos++;
-        S.push_back(std::move(sep));
-        kids.push_back(std::move(ch));
-        return (uint32_t)(S.size() - 1);
+    static Forest single(std::vector<uint32_t> &&P) {
+        Forest f;
+        f.S.push_back(std::move(P));
+        f.kids.emplace_back();
+        f.roots.push_back(0);
+        return f;
     }
 
     // smallest coordinate along each axis over v and its neighbours in the part
-    void low3(uint32_t v, uint32_t *m) const {
+    void low3(uint32_t v, uint32_t cur, uint32_t *m) const {
         m[0] = xyz[3 * (size_t)v];
         m[1] = xyz[3 * (size_t)v + 1];
         m[2] = xyz[3 * (size_t)v + 2];
         for (uint32_t k = roff[v]; k < roff[v + 1]; ++k) {
             const uint32_t w = rcol[k];
-            if (stamp[w] != cur) continue;
+            if (stamp[w].load(std::memory_order_relaxed) != cur) continue;
             const uint32_t *c = &xyz[3 * (size_t)w];
             m[0] = std::min(m[0], c[0]);
             m[1] = std::min(m[1], c[1]);
@@ -68,17 +99,16 @@ struct Dissection {
         }
     }
 
-    // returns the fronts that root the part's subtrees
-    std::vector<uint32_t> run(std::vector<uint32_t> P) {
+    Forest run(std::vector<uint32_t> P, int depth = 0) {
         const size_t np = P.size();
         if (np == 0) return {};
-        if (np <= leaf) return {new_front(std::move(P), {})};
-        ++cur;
-        for (uint32_t v : P) stamp[v] = cur;
+        if (np <= leaf) return single(std::move(P));
+        const uint32_t cur = next_id.fetch_add(1, std::memory_order_relaxed) + 1;
+        for (uint32_t v : P) stamp[v].store(cur, std::memory_order_relaxed);
         std::vector<uint32_t> mlow(3 * np);
         uint32_t bmin[3] = {~0u, ~0u, ~0u}, bmax[3] = {0, 0, 0};
         for (size_t i = 0; i < np; ++i) {
-            low3(P[i], &mlow[3 * i]);
+            low3(P[i], cur, &mlow[3 * i]);
             for (int a = 0; a < 3; ++a) {
                 const uint32_t c = xyz[3 * (size_t)P[i] + a];
                 bmin[a] = std::min(bmin[a], c);
@@ -118,7 +148,7 @@ struct Dissection {
                 }
             }
         }
-        if (best_a < 0) return {new_front(std::move(P), {})};  // no admissible plane: one dense front
+        if (best_a < 0) return single(std::move(P));  // no admissible plane: one dense front
         std::vector<uint32_t> L, R, Sv;
         for (size_t i = 0; i < np; ++i) {
             const uint32_t v = P[i];
@@ -128,30 +158,34 @@ struct Dissection {
         }
         std::vector<uint32_t>().swap(P);
         std::vector<uint32_t>().swap(mlow);
-        std::vector<uint32_t> roots = run(std::move(L));
-        std::vector<uint32_t> rr = run(std::move(R));
-        roots.insert(roots.end(), rr.begin(), rr.end());
-        if (Sv.empty()) return roots;  // the part was disconnected along the plane
-        return {new_front(std::move(Sv), std::move(roots))};
+        Forest fl, fr;
+        if (depth < kParDepth && L.size() >= kParMin && R.size() >= kParMin) {
+            std::thread th([&] { fl = run(std::move(L), depth + 1); });
+            fr = run(std::move(R), depth + 1);
+            th.join();
+        } else {
+            fl = run(std::move(L), depth + 1);
+            fr = run(std::move(R), depth + 1);
+        }
+        // the fronts of L, then those of R (renumbered), then the separator
+        Forest out;
+        const uint32_t shift = (uint32_t)fl.S.size();
+        out.S = std::move(fl.S);
+        out.kids = std::move(fl.kids);
+        out.roots = std::move(fl.roots);
+        for (size_t f = 0; f < fr.S.size(); ++f) {
+            for (uint32_t &c : fr.kids[f]) c += shift;
+            out.S.push_back(std::move(fr.S[f]));
+            out.kids.push_back(std::move(fr.kids[f]));
+        }
+        for (uint32_t r : fr.roots) out.roots.push_back(r + shift);
+        if (Sv.empty()) return out;  // the part was disconnected along the plane
+        out.S.push_back(std::move(Sv));
+        out.kids.push_back(std::move(out.roots));
+        out.roots.assign(1, (uint32_t)(out.S.size() - 1));
+        return out;
     }
 };
-
-// Runs body(lo, hi, t) over [0, n) in up to 8 contiguous ranges on host
-// threads; each range writes only its own outputs, so the result does not
-// depend on the split.
-template <typename F>
-void parallel_ranges(size_t n, size_t grain, F body) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned t = (unsigned)std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n / grain));
-    if (t <= 1) {
-        body(size_t(0), n, 0u);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (unsigned k = 1; k < t; ++k) th.emplace_back(body, n * k / t, n * (k + 1) / t, k);
-    body(size_t(0), n / t, 0u);
-    for (auto &x : th) x.join();
-}
 
 }  // namespace
 
@@ -240,12 +274,22 @@ MfPlan mf_plan(uint32_t n, size_t E, const uint32_t *conn, const uint16_t *meta,
     for (uint32_t i = 0; i < ni; ++i)
         for (int a = 0; a < 3; ++a) cxyz[3 * (size_t)i + a] = xyz[3 * (size_t)P.inode[i] + a];
 
-    // nested dissection
-    Dissection D(roff, rcol, cxyz, ni, ni <= dense ? std::max(ni, 1u) : leaf);
+    // nested dissection; positions in front order (postorder), nodes in list order
+    struct {
+        std::vector<std::vector<uint32_t>> S, kids;
+        std::vector<uint32_t> pos;
+    } D;
     {
+        Dissection nd(roff, rcol, cxyz, ni, ni <= dense ? std::max(ni, 1u) : leaf);
         std::vector<uint32_t> all(ni);
         std::iota(all.begin(), all.end(), 0u);
-        D.run(std::move(all));
+        Dissection::Forest fo = nd.run(std::move(all));
+        D.S = std::move(fo.S);
+        D.kids = std::move(fo.kids);
+        D.pos.assign(ni, 0);
+        uint32_t next_pos = 0;
+        for (const auto &sep : D.S)
+            for (uint32_t v : sep) D.pos[v] = next_pos++;
     }
     const uint32_t nf = (uint32_t)D.S.size();
     lap("dissection");
@@ -769,7 +813,7 @@ void download(std::vector<T> &h, const T *d, size_t n, hipStream_t s) {
 
 }  // namespace
 
-std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s) {
+std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, uint32_t dense) {
     const uint32_t n = d.n_nodes;
     const size_t E = d.n_elem;
     std::vector<uint32_t> conn, n2l, x, y, z;
@@ -791,7 +835,7 @@ std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s) {
     }
     std::unique_ptr<CoarseSolver> cs(new CoarseSolver());
     // the fronts may take at most 4 GiB of doubles
-    cs->plan = mf_plan(n, E, conn.data(), meta.data(), bdy.data(), xyz.data(), (4ull << 30) / 8);
+    cs->plan = mf_plan(n, E, conn.data(), meta.data(), bdy.data(), xyz.data(), (4ull << 30) / 8, 128, dense);
     const MfPlan &P = cs->plan;
     cs->n = n;
     cs->ni = P.ni;

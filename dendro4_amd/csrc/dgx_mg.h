@@ -60,9 +60,11 @@ constexpr uint32_t kMaxDirectNodes = 40000;
 // on a second host thread and stream while the next levels are coarsened
 // (faster when nothing else drives the device; a caller that already runs
 // several set-ups at once passes false).  colour as for da_build.
+// dense_nodes: the coarsest level's dense-front limit (kDenseNodes, or
+// kDenseNodesOneSolve for a hierarchy solved once).
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
                                       double lap_fac, double mass_fac, hipStream_t s, bool overlap = true,
-                                      bool colour = true);
+                                      bool colour = true, uint32_t dense_nodes = kDenseNodes);
 
 struct SolveOpts {
     int fp32_vcycle = 1;

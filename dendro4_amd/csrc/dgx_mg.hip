@@ -421,10 +421,11 @@ static void build_level_diag(MgLevel &L, double lap, double mass, hipStream_t s)
 static double ms_since(std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
-static void build_coarse(MgHierarchy &mg, hipStream_t s, double *ms_analyse = nullptr, double *ms_factor = nullptr) {
+static void build_coarse(MgHierarchy &mg, hipStream_t s, uint32_t dense, double *ms_analyse = nullptr,
+                         double *ms_factor = nullptr) {
     const DaDevice &d = *mg.lev.back().da;
     auto t0 = std::chrono::steady_clock::now();
-    mg.coarse = coarse_analyse(d, s);
+    mg.coarse = coarse_analyse(d, s, dense);
     if (ms_analyse) {
         HIP_OK(hipStreamSynchronize(s));
         *ms_analyse = ms_since(t0);
@@ -569,7 +570,8 @@ class SetupHelper {
 }  // namespace
 
 std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, int curve, int nlevels,
-                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap, bool colour) {
+                                      double lap_fac, double mass_fac, hipStream_t s, bool overlap, bool colour,
+                                      uint32_t dense_nodes) {
     std::unique_ptr<MgHierarchy> mg(new MgHierarchy());
     mg->md = md;
     mg->curve = curve;
@@ -637,8 +639,8 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
         t_wait += ms_since(tw);
     }
     // the coarse solver on the helper, the transfers here
-    helper.submit([&mg, &t_an, &t_fa](hipStream_t hs) {
-        build_coarse(*mg, hs, timing ? &t_an : nullptr, timing ? &t_fa : nullptr);
+    helper.submit([&mg, &t_an, &t_fa, dense_nodes](hipStream_t hs) {
+        build_coarse(*mg, hs, dense_nodes, timing ? &t_an : nullptr, timing ? &t_fa : nullptr);
     });
     const auto tt = std::chrono::steady_clock::now();
     for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);

@@ -293,7 +293,8 @@ struct dgx_server_s {
             // the levels of its own set-up instead.  A request solves once, so
             // its DAs are not coloured (the colouring costs more than the
             // matvecs of one solve save).
-            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s, o.slots == 1, false);
+            own = mg_build(b.p, b.n, rq.md, DGX_CURVE_HILBERT, o.nlevels, 1.0, 1.0, s, o.slots == 1, false,
+                           kDenseNodesOneSolve);
             mg = own.get();
             b.release();
             HIP_OK(hipStreamSynchronize(s));
