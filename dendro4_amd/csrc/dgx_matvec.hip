@@ -7,13 +7,11 @@
 // the element's hanging vertices from its parent's corners: a vertex hanging on
 // a parent edge takes the mean of the two edge ends, one hanging on a parent
 // face the mean of the four face corners (checked against the reference tables
-// to 8e-16 by the test suite).  The kernel applies C, K and C^T directly from
-// the 8-bit hanging mask and the child number, so no stencil table is read:
-// per element it moves 32 B of node ids + 2 B of meta + 8 gathers + 8 scatters.
-//
-// K is applied by sum factorisation with the 1-D operators
-//   m = [[2/3,1/3],[1/3,2/3]] (mass), k = [[1/2,-1/2],[-1/2,1/2]] (stiffness):
-//   K u = (k(x) m(y) m(z) + m(x) k(y) m(z) + m(x) m(y) k(z)) u,  M u = m(x) m(y) m(z) u.
+// to 8e-16 by the test suite).  The kernels apply C, K and C^T directly from
+// the hanging mask in the element's child frame, with K diagonalised by a 3-D
+// Haar butterfly, so no stencil table is read.  Node sums are accumulated per
+// element block in LDS as fixed-point integers, which makes every result
+// independent of the order of the accumulations.
 #include "dgx_da.h"
 
 #include <algorithm>
@@ -25,54 +23,6 @@
 namespace {
 
 constexpr int B = 256;
-
-template <typename T>
-struct Elem8 {
-    T v[8];
-};
-
-// 1-D operators along axis a (stride s = 1, 2, 4) on an 8-vector.
-template <typename T, int S>
-__device__ __forceinline__ void op_m(const T *u, T *o) {
-    const T third = T(1.0 / 3.0);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        if (i & S) continue;
-        T a = u[i], b = u[i + S];
-        o[i] = (a + a + b) * third;
-        o[i + S] = (a + b + b) * third;
-    }
-}
-template <typename T, int S>
-__device__ __forceinline__ void op_k(const T *u, T *o) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        if (i & S) continue;
-        T d = T(0.5) * (u[i] - u[i + S]);
-        o[i] = d;
-        o[i + S] = -d;
-    }
-}
-
-// v = fac1 * K u + fac2 * M u
-template <typename T>
-__device__ __forceinline__ void apply_q1(const T *u, T fac1, T fac2, T *v) {
-    T a[8], b[8], c[8], d[8], e[8], w[8], kb[8];
-    op_m<T, 4>(u, a);   // Mz u
-    op_m<T, 2>(a, b);   // My Mz u
-    op_k<T, 2>(a, c);   // Ky Mz u
-    op_k<T, 4>(u, d);   // Kz u
-    op_m<T, 2>(d, e);   // My Kz u
-#pragma unroll
-    for (int i = 0; i < 8; ++i) w[i] = fac1 * (c[i] + e[i]) + fac2 * b[i];
-    op_m<T, 1>(w, v);   // Mx (...)
-    op_k<T, 1>(b, kb);  // Kx My Mz u
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] += fac1 * kb[i];
-}
-
-__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
 // Unscaled 1-D operators along the axis with vertex stride S:
 //   mhat = [[2,1],[1,2]] (3 x mass of [-1,1]),  khat = [[1,-1],[-1,1]] (2 x stiffness)
@@ -190,23 +140,117 @@ __device__ __forceinline__ void init_lambda(T (*lam)[4], double lap_fac, double 
     }
 }
 
-// One element: gather its 8 vertex values from the LDS x slots (byte offsets
-// off[j], child frame), apply r = C^T (g1 K + g2 M) C u and add r into the LDS
-// FP64 accumulators.  Every lane of the wave calls it (the hanging test is a
-// wave vote); lanes with valid == false do nothing.
-template <typename T>
-__device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t cm, bool valid,
-                                                   const unsigned char *xb, unsigned char *yb, const T (*lam)[4]) {
-    const uint32_t mask = cm & 255u, lev = cm >> 8;
-    const bool wave_hanging = __ballot(valid && mask != 0u) != 0ull;
-    if (!valid) return;
-    T u[8];
-    // x slots are sizeof(T) bytes, accumulators 8 bytes: off[] addresses the latter
-    constexpr int XSH = sizeof(T) == 4 ? 1 : 0;
+// ---------------------------------------------------------------------------
+// Fixed-point node sums.  Each block adds its element contributions into the
+// LDS accumulators as 64-bit integers q = round(v 2^E), with one exponent E
+// per block.  Integer addition is exact and associative, so a node's sum does
+// not depend on the order in which the atomics land: the matvec, and
+// everything built on it, is deterministic.  E comes from an a-priori bound.
+// With u = C x (|u| <= X = max |x| over the block's nodes, C averages) and
+// A_e = g1 K + g2 M, every contribution satisfies
+//   |r_j| <= 3.25 X ||A_e||_inf <= 3.25 X (24 g1 + 27 g2),
+// since K's rows sum to 0 around the diagonal 12 g1, M >= 0 has row sums
+// 27 g2, and 3.25 = 1 + 3/2 + 3/4 bounds the hanging transpose C^T.  g1, g2
+// are largest at the block's coarsest element.  With X < 2^xe and that factor
+// < 2^se, E = 50 - xe - se keeps every |q| < 2^50 and a node's sum of at most
+// 512 x 8 terms below 2^62.  The quantisation step is 2^-E, at most 2^-48 of
+// the bound.  A non-finite x value makes every row its block touches NaN
+// (with atomics in floating point, only its neighbours' rows).
+// ---------------------------------------------------------------------------
+constexpr int kExpZero = -2000, kExpBad = 4000;
+
+// |v| < 2^mag_exp(v); kExpZero for 0, kExpBad for inf / NaN.
+__device__ __forceinline__ int mag_exp(double v) {
+    const double a = fabs(v);
+    if (!(a <= 1.7976931348623157e308)) return kExpBad;
+    if (a == 0.0) return kExpZero;
+    return __builtin_amdgcn_frexp_exp(a);
+}
+
+// Maximum over the 64 lanes of a wave (every lane active), by DPP row shifts
+// and row broadcasts; the result is wave-uniform.
+__device__ __forceinline__ int wave_max_i32(int v) {
+    constexpr int lo = -2147483647 - 1;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+struct FixScale {
+    double s;  // 2^E
+    int e;     // E
+    bool bad;  // non-finite input: rows become NaN
+};
+
+// se[lv]: the bound factor 3.25 (24 g1 + 27 g2) of an element of level lv is
+// below 2^se[lv] (threads 0..31 fill the table; a barrier publishes it).
+__device__ __forceinline__ void init_level_exps(int *se, double lap_fac, double mass_fac) {
+    if (threadIdx.x < 32) {
+        const double h = ldexp(1.0, -(int)threadIdx.x);
+        se[threadIdx.x] =
+            mag_exp(3.25 * (24.0 * fabs(lap_fac) * h * (1.0 / 36.0) + 27.0 * fabs(mass_fac) * h * h * h * (1.0 / 216.0)));
+    }
+}
+
+// E for a block whose node values satisfy |x| < 2^xe and whose elements'
+// bound factors are below 2^se (wave-uniform inputs, scalar results).
+__device__ __forceinline__ FixScale fix_scale(int xe, int se) {
+    FixScale f;
+    int e = 50 - xe - se;
+    // 2^E must be a normal double; a block past the lower end would need a
+    // bound above 2^1070, beyond any finite x and operator factors in use
+    f.bad = xe >= kExpBad || se >= kExpBad || e < -1022;
+    e = f.bad ? 0 : min(e, 1023);
+    e = __builtin_amdgcn_readfirstlane(e);
+    f.e = e;
+    f.s = __longlong_as_double((long long)(e + 1023) << 52);
+    f.bad = __builtin_amdgcn_readfirstlane((int)f.bad) != 0;
+    return f;
+}
+
+// round(v 2^E) as a two's-complement integer: for |v 2^E| < 2^51 the sum
+// v 2^E + 1.5 2^52 is rounded to an integer once and holds it in its low
+// mantissa bits.
+__device__ __forceinline__ unsigned long long to_fixed(double v, double s) {
+    const double t = fma(v, s, 6755399441055744.0);
+    return (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;
+}
+__device__ __forceinline__ double from_fixed(unsigned long long q, const FixScale &f) {
+    return f.bad ? __builtin_nan("") : ldexp((double)(long long)q, -f.e);
+}
+
+// Block maxima of xe and se through one LDS pair per wave; the caller's next
+// __syncthreads publishes them and block_scale() reads them.
+__device__ __forceinline__ void publish_scale_inputs(int xe, int se, int (*wred)[2]) {
+    xe = wave_max_i32(xe);
+    se = wave_max_i32(se);
+    if ((threadIdx.x & 63u) == 0) {
+        wred[threadIdx.x >> 6][0] = xe;
+        wred[threadIdx.x >> 6][1] = se;
+    }
+}
+__device__ __forceinline__ FixScale block_scale(const int (*wred)[2]) {
+    int xe = wred[0][0], se = wred[0][1];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) u[j] = *reinterpret_cast<const T *>(xb + (off[j] >> XSH));
+    for (int w = 1; w < B / 64; ++w) {
+        xe = max(xe, wred[w][0]);
+        se = max(se, wred[w][1]);
+    }
+    return fix_scale(xe, se);
+}
+
+// The element operator in its child frame, in place on the 8 vertex values:
+// r = C^T (g1 K + g2 M) C u with C fixed by the hanging bits of cm.  With
+// hanging == false the interpolation is skipped (the mask must then be 0).
+template <typename T>
+__device__ __forceinline__ void element_apply(T *u, uint32_t cm, bool hanging, const T (*lam)[4]) {
+    const uint32_t mask = cm & 255u, lev = cm >> 8;
     bool h1 = false, h2 = false, h4 = false, h3 = false, h5 = false, h6 = false;
-    if (wave_hanging) {
+    if (hanging) {
         // hanging vertices in the child frame: edge midpoints 1,2,4 (mean of
         // vertex 0 and the edge end), face centres 3,5,6 (mean of 0, the face
         // vertex and its two edge vertices)
@@ -235,7 +279,7 @@ __device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t
     u[6] *= l2;
     u[7] *= l3;
     haar3<T>(u);
-    if (wave_hanging) {
+    if (hanging) {
         const T q3 = h3 ? T(0.25) * u[3] : T(0), q5 = h5 ? T(0.25) * u[5] : T(0), q6 = h6 ? T(0.25) * u[6] : T(0);
         const T e1 = h1 ? T(0.5) * u[1] : T(0), e2 = h2 ? T(0.5) * u[2] : T(0), e4 = h4 ? T(0.5) * u[4] : T(0);
         u[0] = u[0] + (e1 + e2 + e4) + (q3 + q5 + q6);
@@ -246,8 +290,28 @@ __device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t
         if (h5) u[5] = q5;
         if (h6) u[6] = q6;
     }
+}
+
+// One element: gather its 8 vertex values from the LDS x slots (byte offsets
+// off[j], child frame), apply the element operator and add r into the LDS
+// fixed-point accumulators.  Every lane of the wave calls it (the hanging
+// test is a wave vote); lanes with valid == false do nothing.
+template <typename T>
+__device__ __forceinline__ void accumulate_element(const uint32_t *off, uint32_t cm, bool valid,
+                                                   const unsigned char *xb, unsigned char *yb, const T (*lam)[4],
+                                                   double qs) {
+    const uint32_t mask = cm & 255u;
+    const bool wave_hanging = __ballot(valid && mask != 0u) != 0ull;
+    if (!valid) return;
+    T u[8];
+    // x slots are sizeof(T) bytes, accumulators 8 bytes: off[] addresses the latter
+    constexpr int XSH = sizeof(T) == 4 ? 1 : 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) atomicAdd(reinterpret_cast<double *>(yb + off[j]), (double)u[j]);
+    for (int j = 0; j < 8; ++j) u[j] = *reinterpret_cast<const T *>(xb + (off[j] >> XSH));
+    element_apply<T>(u, cm, wave_hanging, lam);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        atomicAdd(reinterpret_cast<unsigned long long *>(yb + off[j]), to_fixed((double)u[j], qs));
 }
 
 // The element record holds 8-byte slot offsets; SH = 1 turns them into
@@ -283,15 +347,15 @@ __device__ __forceinline__ void unpack_offsets(const uint16_t *lconn, size_t e, 
 
 // Pass 1, one workgroup per element block (any block size up to MAXN nodes).
 // Element data are loaded first, the block's nodes are gathered into LDS,
-// element contributions are accumulated there in FP64 (ds_add_f64 for both
-// precisions: on gfx950 it runs well ahead of ds_add_f32), and each node sum
-// goes to y (node touched by this block only)
-// or to its partial slot (shared node).  Node entries: bits 0-29 node id,
-// bit 30 Dirichlet boundary, bit 31 shared.  With DIR, boundary columns are
-// dropped and boundary rows are the identity.  LDS: x slots (8 bytes each)
-// then the accumulators, both static, so the per-vertex byte offsets read from
-// lconn address them directly.  Elements without hanging vertices come first
-// in each block, so most waves skip the hanging interpolation.
+// element contributions are accumulated there as fixed-point integers
+// (ds_add_u64, exact and order-independent), and each node sum goes to y
+// (node touched by this block only) or to its partial slot (shared node).
+// Node entries: bits 0-29 node id, bit 30 Dirichlet boundary, bit 31 shared.
+// With DIR, boundary columns are dropped and boundary rows are the identity.
+// LDS: x slots (8 bytes each) then the accumulators, both static, so the
+// per-vertex byte offsets read from lconn address them directly.  Elements
+// without hanging vertices come first in each block, so most waves skip the
+// hanging interpolation.
 template <typename T, bool DIR, int MAXN>
 __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ lconn, const uint16_t *__restrict__ cmeta,
                                                   size_t ne, const uint32_t *__restrict__ boff,
@@ -299,14 +363,17 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
                                                   const T *__restrict__ x, T *__restrict__ y, T *__restrict__ partial,
                                                   double lap_fac, double mass_fac) {
     constexpr int EB = PER * B;
-    __shared__ __align__(16) T xs[MAXN];       // x of the block's nodes
-    __shared__ __align__(16) double ys[MAXN];  // FP64 accumulators (ds_add_f64 outruns ds_add_f32 here)
+    __shared__ __align__(16) T xs[MAXN];                   // x of the block's nodes
+    __shared__ __align__(16) unsigned long long ys[MAXN];  // fixed-point node sums
     __shared__ T lam[32][4];
+    __shared__ int se_tab[32];
+    __shared__ int wred[B / 64][2];
     unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
     unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
     const uint32_t b = blockIdx.x, n0 = boff[b], nb = boff[b + 1] - n0;
     const size_t e0 = (size_t)b * EB;
     uint32_t off[PER][8], mt[PER];
+    int lv = 31;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const size_t e = e0 + threadIdx.x + k * B;
@@ -314,23 +381,32 @@ __global__ void __launch_bounds__(B) k_matvec_blk(const uint16_t *__restrict__ l
         if (e < ne) {
             unpack_offsets<0>(lconn, e, off[k]);
             mt[k] = cmeta[e];
+            lv = min(lv, (int)(mt[k] >> 8));
         }
     }
     init_lambda<T>(lam, lap_fac, mass_fac);
+    init_level_exps(se_tab, lap_fac, mass_fac);
+    int xe = kExpZero;
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
-        xs[l] = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
-        ys[l] = 0.0;
+        const T v = (DIR && (g & BDY_BIT)) ? T(0) : x[g & ID_MASK];
+        xs[l] = v;
+        ys[l] = 0ull;
+        xe = max(xe, mag_exp((double)v));
     }
+    __syncthreads();  // se_tab
+    publish_scale_inputs(xe, se_tab[lv], wred);
     __syncthreads();
+    const FixScale fs = block_scale(wred);
 #pragma unroll
     for (int k = 0; k < PER; ++k)
-        accumulate_element<T>(off[k], mt[k], e0 + threadIdx.x + k * B < ne, xb, yb, lam);
+        accumulate_element<T>(off[k], mt[k], e0 + threadIdx.x + k * B < ne, xb, yb, lam, fs.s);
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < nb; l += B) {
         const uint32_t g = bnode[n0 + l];
-        if (g & SHARED_BIT) partial[bdst[n0 + l]] = (T)ys[l];
-        else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : (T)ys[l];
+        const T v = (T)from_fixed(ys[l], fs);
+        if (g & SHARED_BIT) partial[bdst[n0 + l]] = v;
+        else y[g & ID_MASK] = (DIR && (g & BDY_BIT)) ? x[g & ID_MASK] : v;
     }
 }
 
@@ -359,7 +435,9 @@ __global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict
                                                    T omega) {
     constexpr int EB = PER * B, MAXN = NPT * B;
     __shared__ __align__(16) T xs[MAXN];
-    __shared__ __align__(16) double ys[MAXN];
+    __shared__ __align__(16) unsigned long long ys[MAXN];  // fixed-point node sums
+    __shared__ int se_tab[32];
+    __shared__ int wred[B / 64][2];
     __shared__ T lam[32][4];
     unsigned char *xb = reinterpret_cast<unsigned char *>(xs);
     unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
@@ -368,6 +446,7 @@ __global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict
     uint32_t b = lo + (blockIdx.x >> 3);
     if (b >= hi) return;  // uniform over the workgroup
     init_lambda<T>(lam, lap_fac, mass_fac);
+    init_level_exps(se_tab, lap_fac, mass_fac);
 
     auto load_elems = [&](uint32_t bb, uint3 *ev) {
         const size_t e = (size_t)bb * EB + threadIdx.x;
@@ -418,20 +497,28 @@ __global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict
     // see the loads issued inside the loop, and the element phase does not
     // wait for the loads issued ahead of it in the same iteration.
     __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // lam, se_tab
     for (;;) {
         // x of the block's nodes for the identity rows of boundary nodes (the
         // Jacobi writeout reads x back from the LDS slots instead and reloads
         // it for boundary nodes, whose slots hold 0)
         T xr[NPT];
+        int xe = kExpZero, lv = 31;
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
-                xs[l] = (DIR && (g[i] & BDY_BIT)) ? T(0) : xv[i];
-                ys[l] = 0.0;
+                const T v = (DIR && (g[i] & BDY_BIT)) ? T(0) : xv[i];
+                xs[l] = v;
+                ys[l] = 0ull;
+                xe = max(xe, mag_exp((double)v));
             }
             if (DIR && EPI != EPI_JACOBI) xr[i] = xv[i];
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if ((size_t)b * EB + threadIdx.x + k * B < ne) lv = min(lv, (int)(ev[k].z >> 24));
+        publish_scale_inputs(xe, se_tab[lv], wred);
         // right-hand side and inverse diagonal of the block's nodes, for the writeout
         T bv[NPT], dv[NPT];
 #pragma unroll
@@ -440,6 +527,7 @@ __global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict
             if (EPI == EPI_JACOBI) dv[i] = dinv[g[i] & ID_MASK];
         }
         __syncthreads();
+        const FixScale fs = block_scale(wred);
         const bool moreA = bA < hi;
         uint3 evA[PER];
         uint32_t gB[NPT], sdA[NPT];
@@ -460,21 +548,22 @@ __global__ void __launch_bounds__(B, 4) k_matvec_pipe(const uint32_t *__restrict
         for (int k = 0; k < PER; ++k) {
             uint32_t off[8];
             unpack_rec12(ev[k].x, ev[k].y, ev[k].z, off);
-            accumulate_element<T>(off, ev[k].z >> 16, e0 + k * B < ne, xb, yb, lam);
+            accumulate_element<T>(off, ev[k].z >> 16, e0 + k * B < ne, xb, yb, lam, fs.s);
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
             const uint32_t l = threadIdx.x + i * B;
             if (l < nb) {
+                const T av = (T)from_fixed(ys[l], fs);
                 if (g[i] & SHARED_BIT) {
-                    partial[sd[i]] = (T)ys[l];
+                    partial[sd[i]] = av;
                 } else if (EPI == EPI_JACOBI) {
                     const T xi = (DIR && (g[i] & BDY_BIT)) ? x[g[i] & ID_MASK] : xs[l];
-                    const T v = (DIR && (g[i] & BDY_BIT)) ? xi : (T)ys[l];
+                    const T v = (DIR && (g[i] & BDY_BIT)) ? xi : av;
                     y[g[i] & ID_MASK] = xi + omega * dv[i] * (bv[i] - v);
                 } else {
-                    const T v = (DIR && (g[i] & BDY_BIT)) ? xr[i] : (T)ys[l];
+                    const T v = (DIR && (g[i] & BDY_BIT)) ? xr[i] : av;
                     if (EPI == EPI_Y) y[g[i] & ID_MASK] = v;
                     else y[g[i] & ID_MASK] = bv[i] - v;
                 }
@@ -710,47 +799,94 @@ template void da_dirichlet_residual<float>(const DaDevice &, const float *, cons
 
 namespace {
 
-// diag[idx_k] += (C^T K C)[k][k]: column k of C is the interpolation weight of
-// DOF k at each vertex; the diagonal entry is that column's K-norm.
-__global__ void __launch_bounds__(B) k_diagonal(const uint32_t *__restrict__ conn, const uint16_t *__restrict__ meta,
-                                                size_t ne, double *__restrict__ diag, double lap_fac,
-                                                double mass_fac) {
-    size_t e = (size_t)blockIdx.x * B + threadIdx.x;
-    if (e >= ne) return;
-    uint32_t mt = meta[e];
-    uint32_t mask = mt & 255u, c = (mt >> 8) & 7u, lev = mt >> 11;
-    double h = 1.0 / (double)(1u << lev);
-    double fac1 = lap_fac * h * 0.5, fac2 = mass_fac * h * h * h * 0.125;
-    for (int k = 0; k < 8; ++k) {
-        double u[8], v[8];
-        for (int a = 0; a < 8; ++a) {
-            double w = 0.0;
-            if (!((mask >> a) & 1u)) {
-                w = (a == k) ? 1.0 : 0.0;
-            } else {
-                uint32_t dff = a ^ c, lo = dff & (0u - dff);
-                if (__popc(dff) == 1) {
-                    w = ((uint32_t)k == c || k == a) ? 0.5 : 0.0;
-                } else {
-                    w = ((uint32_t)k == c || k == a || (uint32_t)k == (c ^ lo) || (uint32_t)k == (c ^ (dff ^ lo))) ? 0.25
-                                                                                                                  : 0.0;
-                }
-            }
-            u[a] = w;
+// Diagonal (C^T A_e C)[m][m] of each element in its child frame: the element
+// operator applied to the unit vector of vertex m, entry m.  Summed per block
+// with the matvec's fixed-point accumulators (|x| = 1 in the bound), shared
+// nodes through the partial slots, so the diagonal is deterministic too.
+template <int MAXN>
+__global__ void __launch_bounds__(B) k_diag_blk(const uint16_t *__restrict__ lconn, const uint16_t *__restrict__ cmeta,
+                                                size_t ne, const uint32_t *__restrict__ boff,
+                                                const uint32_t *__restrict__ bnode, const uint32_t *__restrict__ bdst,
+                                                double *__restrict__ diag, double *__restrict__ partial,
+                                                double lap_fac, double mass_fac) {
+    constexpr int EB = PER * B;
+    __shared__ __align__(16) unsigned long long ys[MAXN];
+    __shared__ double lam[32][4];
+    __shared__ int se_tab[32];
+    __shared__ int wred[B / 64][2];
+    unsigned char *yb = reinterpret_cast<unsigned char *>(ys);
+    const uint32_t b = blockIdx.x, n0 = boff[b], nb = boff[b + 1] - n0;
+    const size_t e0 = (size_t)b * EB;
+    uint32_t off[PER][8], mt[PER];
+    int lv = 31;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const size_t e = e0 + threadIdx.x + k * B;
+        mt[k] = 0;
+        if (e < ne) {
+            unpack_offsets<0>(lconn, e, off[k]);
+            mt[k] = cmeta[e];
+            lv = min(lv, (int)(mt[k] >> 8));
         }
-        apply_q1<double>(u, fac1, fac2, v);
-        double dk = 0.0;
-        for (int a = 0; a < 8; ++a) dk += u[a] * v[a];
-        atomic_add(&diag[conn[(size_t)k * ne + e]], dk);
+    }
+    init_lambda<double>(lam, lap_fac, mass_fac);
+    init_level_exps(se_tab, lap_fac, mass_fac);
+    for (uint32_t l = threadIdx.x; l < nb; l += B) ys[l] = 0ull;
+    __syncthreads();  // se_tab
+    publish_scale_inputs(mag_exp(1.0), se_tab[lv], wred);
+    __syncthreads();
+    const FixScale fs = block_scale(wred);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (e0 + threadIdx.x + k * B >= ne) continue;
+        for (int m = 0; m < 8; ++m) {
+            double u[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u[j] = j == m ? 1.0 : 0.0;
+            element_apply<double>(u, mt[k], (mt[k] & 255u) != 0u, lam);
+            double dm = u[0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j)
+                if (j == m) dm = u[j];
+            atomicAdd(reinterpret_cast<unsigned long long *>(yb + off[k][m]), to_fixed(dm, fs.s));
+        }
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < nb; l += B) {
+        const uint32_t g = bnode[n0 + l];
+        const double v = from_fixed(ys[l], fs);
+        if (g & SHARED_BIT) partial[bdst[n0 + l]] = v;
+        else diag[g & ID_MASK] = v;
     }
 }
 
 }  // namespace
 
 void da_diagonal_f64(const DaDevice &da, double *diag, double lap_fac, double mass_fac, hipStream_t s) {
-    HIP_OK(hipMemsetAsync(diag, 0, (size_t)da.n_nodes * sizeof(double), s));
-    if (!da.n_elem) return;
-    k_diagonal<<<grid_for(da.n_elem, B), B, 0, s>>>(da.conn.p, da.meta.p, da.n_elem, diag, lap_fac, mass_fac);
+    if (!da.n_elem) {
+        HIP_OK(hipMemsetAsync(diag, 0, (size_t)da.n_nodes * sizeof(double), s));
+        return;
+    }
+    if (da.elem_block != PER * B) {
+        dgx_set_error("diagonal: DA built with element blocks of %u, kernel expects %d", da.elem_block, PER * B);
+        throw DgxError();
+    }
+    double *partial = reinterpret_cast<double *>(da.partial.p);
+    const uint32_t mn = da.max_block_nodes;
+#define DGX_DIAG_LAUNCH(MAXN)                                                                                    \
+    k_diag_blk<MAXN><<<da.n_blocks, B, 0, s>>>(da.lconn.p, da.cmeta.p, da.n_elem, da.bnode_off.p, da.bnode.p,  \
+                                               da.bdst.p, diag, partial, lap_fac, mass_fac)
+    if (mn <= 1024) DGX_DIAG_LAUNCH(1024);
+    else if (mn <= 2048) DGX_DIAG_LAUNCH(2048);
+    else if (mn <= 4096) DGX_DIAG_LAUNCH(4096);
+    else {
+        dgx_set_error("diagonal: block with %u nodes exceeds 4096", mn);
+        throw DgxError();
+    }
+#undef DGX_DIAG_LAUNCH
+    if (da.n_shared)
+        k_matvec_shared<double, false, EPI_Y><<<grid_for(da.n_shared, B), B, 0, s>>>(
+            da.soff.p, da.snode.p, da.n_shared, partial, nullptr, diag, nullptr, nullptr, 0.0);
     HIP_OK(hipGetLastError());
 }
 

@@ -1,7 +1,8 @@
 """GPU tests of the request server (dgx_server_*): each request's result is
-the same alone and in a mix of concurrent requests (within the solver
-tolerance), resident problems are reused, the HBM budget gates admission,
-and concurrent requests overlap on the device."""
+the same alone and in a mix of concurrent requests (bit for bit: every sum
+in the build and the solve runs in a fixed order), resident problems are
+reused, the HBM budget gates admission, and concurrent requests overlap on
+the device."""
 import numpy as np
 import pytest
 
@@ -37,10 +38,8 @@ def test_result_is_the_same_alone_and_in_the_mix(dgx):
         assert a["status"] == 0 and m["status"] == 0
         assert a["nodes"] == m["nodes"] and a["octants"] == m["octants"] and a["levels"] == m["levels"]
         assert a["rel_residual"] <= 1e-8 and m["rel_residual"] <= 1e-8
-        assert abs(a["iterations"] - m["iterations"]) <= 1
-        # LDS accumulation order is not fixed: solutions agree to the solver tolerance
-        d = np.linalg.norm(a["x"] - m["x"]) / np.linalg.norm(a["x"])
-        assert d <= 1e-8, d
+        assert a["iterations"] == m["iterations"]
+        assert np.array_equal(a["x"], m["x"])
 
 
 def test_resident_problem_is_reused(dgx):
@@ -51,7 +50,7 @@ def test_resident_problem_is_reused(dgx):
     c = sv.wait(sv.submit(pts, 30, req["max_num_pts"], 8), solution=True)
     sv.close()
     assert a["reused"] == 0 and b["reused"] == 1 and c["reused"] == 1
-    assert np.linalg.norm(a["x"] - b["x"]) <= 1e-10 * np.linalg.norm(a["x"])
+    assert np.array_equal(a["x"], b["x"])  # eager first solve, graph replay after
     assert np.linalg.norm(a["x"] - c["x"]) > 1e-3 * np.linalg.norm(a["x"])  # another right-hand side
 
 
@@ -154,7 +153,7 @@ def test_resident_resolves_on_two_slots_with_graphs(dgx):
     sv.close()
     for a, b in zip(first, again):
         assert b["status"] == 0 and b["reused"] == 1
-        assert np.linalg.norm(a["x"] - b["x"]) <= 1e-10 * np.linalg.norm(a["x"])
+        assert np.array_equal(a["x"], b["x"])
 
 
 @pytest.mark.timeout(300)
@@ -173,6 +172,5 @@ def test_mix_with_a_2_22_point_request(dgx):
     for rc, m in zip(reqs, mixed):
         a, = run_all(dgx, [rc], slots=1)
         assert a["nodes"] == m["nodes"] and a["levels"] == m["levels"]
-        assert abs(a["iterations"] - m["iterations"]) <= 1
-        d = np.linalg.norm(a["x"] - m["x"]) / np.linalg.norm(a["x"])
-        assert d <= 1e-8, d
+        assert a["iterations"] == m["iterations"]
+        assert np.array_equal(a["x"], m["x"])
