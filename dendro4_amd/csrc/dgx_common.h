@@ -148,28 +148,74 @@ __host__ __device__ inline void sfc_decode(const SfcTables &T, const Key &k, uin
         rot = T.crot[rot * 8 + m];
     }
 }
-// Morton keys without tables (digit = z<<2|y<<1|x of the anchor bits).
+// Morton keys without tables (digit = z<<2|y<<1|x of the anchor bits).  Bit
+// j of a 21-bit coordinate slice goes to bit 3j of the key word (x; y one
+// above, z two above): the hi word takes anchor bits md-1 .. md-21 (levels
+// 1..21), the lo word anchor bits md-22 .. 0 (levels 22..md) at bit
+// 126 - 3 md + 3 b.  Anchor bits finer than the level are ignored, as are
+// digits finer than the level when decoding.
+__host__ __device__ inline uint64_t morton_spread21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+__host__ __device__ inline uint32_t morton_compact21(uint64_t v) {
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
 __host__ __device__ inline Key morton_encode(uint32_t x, uint32_t y, uint32_t z, uint32_t lev, uint32_t md) {
-    Key k{0ull, 0ull};
-    for (uint32_t t = 1; t <= lev; ++t) {
-        uint32_t b = md - t;
-        uint64_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
-        if (t <= 21) k.hi |= m << (63 - 3 * t);
-        else k.lo |= m << (63 - 3 * (t - 21));
+    // anchor bits md-1 .. md-lev
+    const uint64_t keep = ((1ull << md) - 1ull) & ~((1ull << (md - lev)) - 1ull);
+    const uint64_t xm = x & keep, ym = y & keep, zm = z & keep;
+    Key k;
+    const uint64_t xh = md >= 21 ? xm >> (md - 21) : xm << (21 - md);
+    const uint64_t yh = md >= 21 ? ym >> (md - 21) : ym << (21 - md);
+    const uint64_t zh = md >= 21 ? zm >> (md - 21) : zm << (21 - md);
+    k.hi = morton_spread21(xh) | morton_spread21(yh) << 1 | morton_spread21(zh) << 2;
+    k.lo = lev;
+    if (lev > 21) {
+        const uint64_t low = (1ull << (md - 21)) - 1ull;
+        const uint32_t sh = 126 - 3 * md;
+        k.lo |= (morton_spread21(xm & low) | morton_spread21(ym & low) << 1 | morton_spread21(zm & low) << 2) << sh;
     }
-    k.lo |= lev;
     return k;
 }
 __host__ __device__ inline void morton_decode(const Key &k, uint32_t md, uint32_t &x, uint32_t &y, uint32_t &z,
                                               uint32_t &lev) {
     lev = key_level(k);
-    x = y = z = 0;
-    for (uint32_t t = 1; t <= lev; ++t) {
-        uint32_t m = key_digit(k, t), b = md - t;
-        x |= (m & 1u) << b;
-        y |= ((m >> 1) & 1u) << b;
-        z |= ((m >> 2) & 1u) << b;
+    const uint32_t lh = lev < 21 ? lev : 21;
+    // digits 1..lh of the hi word: bits 63 - 3 lh .. 62
+    const uint64_t hi = lh ? k.hi & ~((1ull << (63 - 3 * lh)) - 1ull) : 0ull;
+    uint64_t xs = morton_compact21(hi), ys = morton_compact21(hi >> 1), zs = morton_compact21(hi >> 2);
+    if (md >= 21) {
+        xs <<= md - 21;
+        ys <<= md - 21;
+        zs <<= md - 21;
+    } else {
+        xs >>= 21 - md;
+        ys >>= 21 - md;
+        zs >>= 21 - md;
     }
+    if (lev > 21) {
+        // digits 22..lev of the lo word: bits 126 - 3 lev .. 62
+        const uint64_t lo = k.lo & ~((1ull << (126 - 3 * lev)) - 1ull);
+        const uint32_t sh = 126 - 3 * md;
+        xs |= morton_compact21(lo >> sh);
+        ys |= morton_compact21(lo >> (sh + 1));
+        zs |= morton_compact21(lo >> (sh + 2));
+    }
+    x = (uint32_t)xs;
+    y = (uint32_t)ys;
+    z = (uint32_t)zs;
 }
 
 // ---------------------------------------------------------------------------

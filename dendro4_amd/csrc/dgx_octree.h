@@ -42,6 +42,47 @@ __device__ inline Key dev_encode_rot(const uint8_t *rc, uint32_t x, uint32_t y, 
     k.lo |= lev;
     return k;
 }
+// Two curve levels per table read: rc2[r*64 + (m1 << 3 | m2)] = rank1 << 3 |
+// rank2 | rot2 << 6 for Morton children m1 (level t) and m2 (level t+1) below
+// rotation state r.  Filled in LDS from rc by the workgroup.
+__host__ __device__ inline uint16_t rc2_entry(const uint8_t *rc, uint32_t i) {
+    const uint32_t r = i >> 6, m1 = (i >> 3) & 7u, m2 = i & 7u;
+    const uint32_t v1 = rc[r * 8 + m1], v2 = rc[(v1 >> 3) * 8 + m2];
+    return (uint16_t)(((v1 & 7u) << 3) | (v2 & 7u) | ((v2 >> 3) << 6));
+}
+// The output-curve key of the octant with Morton key mk, and the curve's
+// rotation state below it (as dev_encode_rot of its anchor).  A Morton
+// digit pair of levels t, t+1 is one 6-bit field, and the curve's digit pair
+// goes to the same field.
+__host__ __device__ inline Key morton_to_curve_rot(const uint16_t *rc2, const uint8_t *rc, const Key &mk,
+                                                   uint32_t &rot) {
+    const uint32_t lev = key_level(mk);
+    Key k{0ull, 0ull};
+    uint32_t r = 0, t = 1;
+    for (; t + 1 <= lev && t + 1 <= 20; t += 2) {  // pairs (1,2) .. (19,20) in the hi word
+        const uint32_t sh = 60 - 3 * t, v = rc2[r * 64 + ((uint32_t)(mk.hi >> sh) & 63u)];
+        r = v >> 6;
+        k.hi |= (uint64_t)(v & 63u) << sh;
+    }
+    for (; t <= lev && t <= 21; ++t) {  // an odd last level, or level 21
+        const uint32_t v = rc[r * 8 + key_digit(mk, t)];
+        r = v >> 3;
+        k.hi |= (uint64_t)(v & 7u) << (63 - 3 * t);
+    }
+    for (; t + 1 <= lev; t += 2) {  // pairs (22,23) .. in the lo word
+        const uint32_t sh = 60 - 3 * (t - 21), v = rc2[r * 64 + ((uint32_t)(mk.lo >> sh) & 63u)];
+        r = v >> 6;
+        k.lo |= (uint64_t)(v & 63u) << sh;
+    }
+    if (t <= lev) {
+        const uint32_t v = rc[r * 8 + key_digit(mk, t)];
+        r = v >> 3;
+        k.lo |= (uint64_t)(v & 7u) << (63 - 3 * (t - 21));
+    }
+    k.lo |= lev;
+    rot = r;
+    return k;
+}
 // Key of Morton child m (bit 0: +x, 1: +y, 2: +z) of the octant with key p
 // and rotation state rot (from dev_encode_rot); equals dev_encode of the child.
 __device__ inline Key child_key(const uint8_t *rc, Key p, uint32_t rot, uint32_t m) {

@@ -26,18 +26,6 @@
 #include <algorithm>
 #include <vector>
 
-SfcPacked make_packed(int curve) {
-    const SfcTables &T = host_sfc(curve);
-    SfcPacked p;
-    for (int r = 0; r < 24; ++r)
-        for (int m = 0; m < 8; ++m) {
-            p.rc[r * 8 + m] = (uint8_t)(T.rank[r * 8 + m] | (T.crot[r * 8 + m] << 3));
-            uint8_t mm = T.morton[r * 8 + m];  // m plays the role of the rank here
-            p.mc[r * 8 + m] = (uint8_t)(mm | (T.crot[r * 8 + mm] << 3));
-        }
-    return p;
-}
-
 namespace {
 
 #define LOAD_TABLES(tab)                                                 \
@@ -46,6 +34,17 @@ namespace {
         s_rc[i_] = (tab).rc[i_];                                          \
         s_mc[i_] = (tab).mc[i_];                                          \
     }                                                                    \
+    __syncthreads();
+// LOAD_TABLES plus the two-level encode table s_rc2 (morton_to_curve_rot).
+#define LOAD_TABLES2(tab)                                                \
+    __shared__ uint8_t s_rc[192], s_mc[192];                             \
+    __shared__ uint16_t s_rc2[24 * 64];                                  \
+    for (int i_ = threadIdx.x; i_ < 192; i_ += blockDim.x) {             \
+        s_rc[i_] = (tab).rc[i_];                                          \
+        s_mc[i_] = (tab).mc[i_];                                          \
+    }                                                                    \
+    for (int i_ = threadIdx.x; i_ < 24 * 64; i_ += blockDim.x)           \
+        s_rc2[i_] = rc2_entry((tab).rc, (uint32_t)i_);                    \
     __syncthreads();
 
 constexpr int B = 256;
@@ -555,7 +554,7 @@ __global__ void __launch_bounds__(LE_B) k_leaf_emit_scan(const Key *Q, const uin
                                                          uint32_t md, SfcPacked tab, Key *out, uint32_t *d_nlv,
                                                          uint64_t *status, uint32_t *ctr, uint32_t epoch,
                                                          uint32_t *fault) {
-    LOAD_TABLES(tab)
+    LOAD_TABLES2(tab)
     __shared__ uint32_t s_tile, s_excl, tmp[LE_B / 64 + 1];
     if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[epoch], 1u);
     __syncthreads();
@@ -576,9 +575,8 @@ __global__ void __launch_bounds__(LE_B) k_leaf_emit_scan(const Key *Q, const uin
     for (uint32_t e = 0; e < LE_ITEMS; ++e) {
         const size_t i = i0 + e;
         if (i >= nq) break;
-        uint32_t x, y, z, l, rot;
-        morton_decode(Q[i], md, x, y, z, l);
-        const Key pk = dev_encode_rot(s_rc, x, y, z, l, md, rot);
+        uint32_t rot;
+        const Key pk = morton_to_curve_rot(s_rc2, s_rc, Q[i], rot);
         for (uint32_t c = 0; c < 8; ++c)
             if (!((msk[e] >> c) & 1u)) out[pos++] = child_key(s_rc, pk, rot, c);
     }
@@ -703,7 +701,9 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
                                                      uint32_t nprev0, const Key *inpar, const uint32_t *loff,
                                                      uint32_t md, SfcPacked tab, Key *leaves, uint32_t used0,
                                                      Key *wbuf0, Key *wbuf1, Key *qbuf0, Key *qbuf1, FuseState *st) {
-    LOAD_TABLES(tab)
+    __shared__ uint32_t s_loff[33];  // input parents of level l: inpar[s_loff[l - 1] .. s_loff[l])
+    for (uint32_t i = threadIdx.x; i < 33; i += kFuseB) s_loff[i] = loff[i];
+    LOAD_TABLES2(tab)
     __shared__ Key sk[kFuseCap];
     __shared__ uint64_t sh[kFuseCap];  // the high words, when the low word is the level alone
     __shared__ uint32_t cm[kFuseCap];
@@ -718,25 +718,47 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
     const Key *W = win, *PQ = prevq;
     bool from_inpar = false, w_lds = false, pq_lds = false;
     for (;;) {
+        // The next level's input parents are loaded now (one per thread; the
+        // rest when they are stored), so the load overlaps this level's work.
+        const uint32_t nin_next = l >= 2 ? s_loff[l] - s_loff[l - 1] : 0u;
+        Key pf{0ull, 0ull};
+        if (t < nin_next) pf = inpar[s_loff[l - 1] + t];
         // seeds of level l (1 <= nw <= kFuseCap): sort, then unique in place -> Q = sk[0..nq)
         uint32_t P = 1;
         while (P < nw) P <<= 1;
-        // seeds are parents at level l-1: at levels <= 21 every digit is in the
-        // high word and the low word is the level, so 8-byte values are sorted
-        const uint64_t lo_lv = l - 1;
-        bool narrow = true;
-        for (uint32_t i = t; i < P; i += kFuseB) {
-            const Key w = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, lo_lv};
-            sh[i] = w.hi;
-            narrow = narrow && w.lo == lo_lv;
-        }
-        narrow = __syncthreads_and(narrow) != 0;
-        if (narrow) {
-            fuse_bitonic(sh, P);
-            for (uint32_t i = t; i < nw; i += kFuseB) sk[i] = Key{sh[i], lo_lv};
+        if (P <= 64) {
+            // a few seeds (the staircase below a cluster of identical points):
+            // one wave sorts them in registers by a bitonic network of shuffles
+            if (t < 64) {
+                Key w = t < nw ? (w_lds ? wl[t] : W[t]) : Key{~0ull, ~0ull};
+                for (uint32_t kk = 2; kk <= P; kk <<= 1)
+                    for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                        Key o;
+                        o.hi = __shfl_xor(w.hi, (int)j, 64);
+                        o.lo = __shfl_xor(w.lo, (int)j, 64);
+                        const bool up = (t & kk) == 0, lower = (t & j) == 0;
+                        if (lower == up ? key_lt(o, w) : key_lt(w, o)) w = o;
+                    }
+                if (t < nw) sk[t] = w;
+            }
         } else {
-            for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, ~0ull};
-            fuse_bitonic(sk, P);
+            // seeds are parents at level l-1: at levels <= 21 every digit is in the
+            // high word and the low word is the level, so 8-byte values are sorted
+            const uint64_t lo_lv = l - 1;
+            bool narrow = true;
+            for (uint32_t i = t; i < P; i += kFuseB) {
+                const Key w = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, lo_lv};
+                sh[i] = w.hi;
+                narrow = narrow && w.lo == lo_lv;
+            }
+            narrow = __syncthreads_and(narrow) != 0;
+            if (narrow) {
+                fuse_bitonic(sh, P);
+                for (uint32_t i = t; i < nw; i += kFuseB) sk[i] = Key{sh[i], lo_lv};
+            } else {
+                for (uint32_t i = t; i < P; i += kFuseB) sk[i] = i < nw ? (w_lds ? wl[i] : W[i]) : Key{~0ull, ~0ull};
+                fuse_bitonic(sk, P);
+            }
         }
         __syncthreads();
         // thread t owns entries kFuseF t .. kFuseF t + kFuseF - 1
@@ -775,9 +797,8 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         for (uint32_t e = 0; e < kFuseF; ++e) {
             const uint32_t i = ib + e;
             if (i >= nq) continue;
-            uint32_t x, y, z, lq, rot;
-            morton_decode(sk[i], md, x, y, z, lq);
-            const Key pk = dev_encode_rot(s_rc, x, y, z, lq, md, rot);
+            uint32_t rot;
+            const Key pk = morton_to_curve_rot(s_rc2, s_rc, sk[i], rot);
             const uint32_t msk = cm[i];
             for (uint32_t c = 0; c < 8; ++c) {
                 if (msk & (1u << c)) continue;
@@ -792,8 +813,8 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
             break;
         }
         // seeds of level l-1: input parents of that level, then neighbour parents of Q
-        const uint32_t nin = loff[l] - loff[l - 1];
-        const Key *src = inpar + loff[l - 1];
+        const uint32_t nin = nin_next;
+        const Key *src = inpar + s_loff[l - 1];
         Key *Wn = (k & 1) ? wbuf1 : wbuf0;
         const uint32_t lvl = l - 1;
         uint32_t pc0[kFuseF][3], pc1[kFuseF][3];
@@ -810,11 +831,12 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         // level l-1 runs here too when it has seeds and they fit (a level
         // without seeds starts from input parents, below, and needs no Q)
         const bool stay = nwn > 0 && nwn <= kFuseCap;
+        Key *wdst = stay ? wl : Wn;
+        if (t < nin) wdst[t] = pf;
+        for (uint32_t i = t + kFuseB; i < nin; i += kFuseB) wdst[i] = src[i];
         if (stay) {
-            for (uint32_t i = t; i < nin; i += kFuseB) wl[i] = src[i];
             for (uint32_t i = t; i < nq; i += kFuseB) pq[i] = sk[i];
         } else {
-            for (uint32_t i = t; i < nin; i += kFuseB) Wn[i] = src[i];
             for (uint32_t i = t; i < nq; i += kFuseB) Qg[i] = sk[i];
         }
         for (uint32_t e = 0; e < kFuseF; ++e)
@@ -839,8 +861,8 @@ __global__ void __launch_bounds__(kFuseB) k_bal_fused(uint32_t l0, const Key *wi
         while (nw == 0 && l >= 1) {
             nprev = 0;
             if (l >= 2) {
-                nw = loff[l] - loff[l - 1];
-                W = inpar + loff[l - 1];
+                nw = s_loff[l] - s_loff[l - 1];
+                W = inpar + s_loff[l - 1];
                 w_lds = false;
                 from_inpar = true;
             }

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "dgx_common.h"
+#include "dgx_octree.h"
 
 namespace {
 
@@ -64,4 +65,16 @@ extern "C" int dgx_sfc_tables(int curve, uint8_t *rank192, uint8_t *morton192, u
     if (morton192) memcpy(morton192, t.morton, 192);
     if (crot192) memcpy(crot192, t.crot, 192);
     return 0;
+}
+
+SfcPacked make_packed(int curve) {
+    const SfcTables &T = host_sfc(curve);
+    SfcPacked p;
+    for (int r = 0; r < 24; ++r)
+        for (int m = 0; m < 8; ++m) {
+            p.rc[r * 8 + m] = (uint8_t)(T.rank[r * 8 + m] | (T.crot[r * 8 + m] << 3));
+            uint8_t mm = T.morton[r * 8 + m];  // m plays the role of the rank here
+            p.mc[r * 8 + m] = (uint8_t)(mm | (T.crot[r * 8 + mm] << 3));
+        }
+    return p;
 }
