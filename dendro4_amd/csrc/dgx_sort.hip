@@ -277,9 +277,11 @@ __global__ void __launch_bounds__(1024) k_sort_plan(const uint32_t *ghist, uint3
 // One pass of the one-sweep LSD sort over the 8-bit digit at `bit`, on tiles
 // of NT x SITEMS keys.  Tiles are taken in the order of a global counter, so a
 // tile only ever waits for tiles that are already running.  A tile: loads its
-// keys; ranks them stably (per round of NT keys: 8 ballots give each lane its
-// peers of equal digit within the wave, one lane per group records the group
-// size, LDS counters order the waves and the rounds), which also yields the
+// keys, wave w the contiguous 64 x SITEMS keys w of the tile; ranks them
+// stably within the wave without workgroup barriers (per round of 64 keys: 8
+// ballots give each lane its peers of equal digit, the wave's own LDS counter
+// row gives the count from its earlier rounds, and the group's first lane
+// advances it); turns the waves' digit counts into per-wave offsets and the
 // tile's digit counts; publishes the counts (AGG); stages the keys in digit
 // order in LDS; sums the counts of the tiles before it by walking back until
 // a tile that already published its inclusive count (INCL); publishes its
@@ -293,8 +295,7 @@ __global__ void __launch_bounds__(NT) k_onesweep(Key *ka, Key *kb, uint32_t *va,
     constexpr int NW = NT / 64, TILE = NT * SITEMS;
     __shared__ uint32_t run_off[256];   // output position of staged index 0 of digit d's run, minus its start
     __shared__ uint32_t tile_off[256];  // start of digit d in the staged tile
-    __shared__ uint32_t seen[256];      // keys of digit d ranked so far
-    __shared__ uint32_t wcnt[NW][256];  // keys of digit d in wave w of the current round
+    __shared__ uint32_t wcnt[NW][256];  // keys of digit d in wave w; then the waves' exclusive prefix
     __shared__ uint32_t scan_tmp[NW + 1];
     __shared__ uint32_t s_tile;
     __shared__ Key sk[TILE];
@@ -308,27 +309,27 @@ __global__ void __launch_bounds__(NT) k_onesweep(Key *ka, Key *kb, uint32_t *va,
     uint32_t *vout = from_b ? va : vb;
     const uint64_t epoch = (uint64_t)(j + 1) << 32;
     if (tid == 0) s_tile = atomicAdd(&tile_ctr[j], 1u);
-    if (tid < 256) seen[tid] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t base = tile * (uint32_t)TILE;
+    const uint32_t base = tile * (uint32_t)TILE, wbase = base + (uint32_t)wid * (64u * SITEMS);
     Key k[SITEMS];
     uint32_t v[SITEMS], d[SITEMS], prank[SITEMS];
 #pragma unroll
     for (int r = 0; r < SITEMS; ++r) {
-        const uint32_t idx = base + (uint32_t)(r * NT + tid);
+        const uint32_t idx = wbase + (uint32_t)(r * 64 + lane);
         const bool valid = idx < n;
         k[r] = valid ? kin[idx] : Key{0ull, 0ull};
         v[r] = (HAS_VALS && valid) ? vin[idx] : 0u;
         d[r] = digit_of(k[r], bit);
     }
+    // this wave's counter row: LDS operations of one wave complete in order,
+    // so the row needs no barrier between its rounds
+    uint32_t *wc = wcnt[wid];
+    for (int i = lane; i < 256; i += 64) wc[i] = 0;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < SITEMS; ++r) {
-        const bool valid = base + (uint32_t)(r * NT + tid) < n;
-        if (tid < 256)
-            for (int w = 0; w < NW; ++w) wcnt[w][tid] = 0;
-        __syncthreads();
+        const bool valid = wbase + (uint32_t)(r * 64 + lane) < n;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -336,20 +337,22 @@ __global__ void __launch_bounds__(NT) k_onesweep(Key *ka, Key *kb, uint32_t *va,
             peers &= ((d[r] >> b) & 1u) ? bal : ~bal;
         }
         const uint32_t rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[wid][d[r]] = __popcll(peers);
-        __syncthreads();
-        uint32_t pos = seen[d[r]] + rank;
-        for (int w = 0; w < wid; ++w) pos += wcnt[w][d[r]];
-        prank[r] = pos;
-        __syncthreads();
-        if (tid < 256) {
-            uint32_t a = 0;
-            for (int w = 0; w < NW; ++w) a += wcnt[w][tid];
-            seen[tid] += a;
-        }
+        const uint32_t before = wc[d[r]];
+        prank[r] = before + rank;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) wc[d[r]] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    const uint32_t cnt = tid < 256 ? seen[tid] : 0u;
+    // digit tid: the waves' counts become exclusive offsets, their sum the tile's count
+    uint32_t cnt = 0;
+    if (tid < 256) {
+        for (int w = 0; w < NW; ++w) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = cnt;
+            cnt += c;
+        }
+    }
     uint64_t *mine = status + (size_t)tile * 256 + (tid & 255);
     if (tid < 256) status_store(mine, (tile == 0 ? ST_INCL : ST_AGG) | epoch | cnt);
     uint32_t total;
@@ -358,8 +361,8 @@ __global__ void __launch_bounds__(NT) k_onesweep(Key *ka, Key *kb, uint32_t *va,
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SITEMS; ++r)
-        if (base + (uint32_t)(r * NT + tid) < n) {
-            const uint32_t pos = tile_off[d[r]] + prank[r];
+        if (wbase + (uint32_t)(r * 64 + lane) < n) {
+            const uint32_t pos = tile_off[d[r]] + wc[d[r]] + prank[r];
             sk[pos] = k[r];
             if (HAS_VALS) sv[pos] = v[r];
         }
