@@ -182,8 +182,10 @@ __global__ void __launch_bounds__(1024) k_sort_small(Key *keys, uint32_t n) {
 }
 
 // Global histograms of the digit positions in pmask (bit p: the byte at bit
-// 8p), into 16 x 256 counters.  Four keys per thread are loaded ahead.
-constexpr int GH_U = 4;
+// 8p), into 16 x 256 counters.  Four keys per thread are loaded ahead.  The
+// counters have one copy per XCD (workgroup b adds into copy b % 8, the XCD
+// it runs on), so the flushes of the workgroups spread over 8 places.
+constexpr int GH_U = 4, GH_COPIES = 8;
 __global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, uint32_t *ghist, uint32_t pmask) {
     __shared__ uint32_t h[16 * 256];
     for (int i = threadIdx.x; i < 16 * 256; i += SB) h[i] = 0;
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__(SB) k_global_hist(const Key *keys, size_t n, u
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 16 * 256; i += SB)
-        if (h[i]) atomicAdd(&ghist[i], h[i]);
+        if (h[i]) atomicAdd(&ghist[(blockIdx.x % GH_COPIES) * (16 * 256) + i], h[i]);
 }
 
 // Look-back status of one (tile, digit) of one pass: bits 0-31 the count,
@@ -249,7 +251,9 @@ __global__ void __launch_bounds__(1024) k_sort_plan(const uint32_t *ghist, uint3
         uint32_t c[4], sum = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            c[q] = ghist[p * 256 + lane * 4 + q];
+            c[q] = 0;
+#pragma unroll
+            for (int x = 0; x < GH_COPIES; ++x) c[q] += ghist[x * (16 * 256) + p * 256 + lane * 4 + q];
             sum += c[q];
         }
         const bool trivial = __ballot(c[0] == n || c[1] == n || c[2] == n || c[3] == n) != 0;
@@ -525,18 +529,18 @@ int sort_threads(size_t n, bool has_vals) {
     return 256;
 }
 
-// Workspace of one sort, zeroed by one fill: 16 x 256 global counters, 16
-// tile counters, then the plan and the look-back status words.
+// Workspace of one sort, zeroed by one fill: GH_COPIES x 16 x 256 global
+// counters, 16 tile counters, then the plan and the look-back status words.
 void run_passes(Key *keys, uint32_t *vals, size_t n, const std::vector<int> &passes, hipStream_t s) {
     if (passes.empty()) return;
     const int np = (int)passes.size();
     const int nt = sort_threads(n, vals != nullptr);
     const size_t ntiles = (n + (size_t)nt * SITEMS - 1) / ((size_t)nt * SITEMS);
-    const size_t off_plan = 16 * 256 * 4 + 16 * 4;
+    const size_t off_plan = GH_COPIES * 16 * 256 * 4 + 16 * 4;
     const size_t off_status = (off_plan + sizeof(SortPlan) + 255) / 256 * 256;
     const size_t ws_bytes = off_status + ntiles * 256 * sizeof(uint64_t);
     DevBuf<uint8_t> ws(ws_bytes, s);
-    uint32_t *ghist = (uint32_t *)ws.p, *ctr = ghist + 16 * 256;
+    uint32_t *ghist = (uint32_t *)ws.p, *ctr = ghist + GH_COPIES * 16 * 256;
     SortPlan *plan = (SortPlan *)(ws.p + off_plan);
     uint64_t *status = (uint64_t *)(ws.p + off_status);
     HIP_OK(hipMemsetAsync(ws.p, 0, ws_bytes, s));
