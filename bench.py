@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--matvec-only", metavar="WORKLOAD", default=None,
                     help="build the workload's DA, run matvecs and a 256 MiB device copy (counter runs)")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the timed points2Octree + balanceOctree steps of lognormal-4M (kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -335,6 +337,10 @@ def main():
     ou, ob, t_build = build_section(dgx, dpts, MD, MP, args.steps, 0)
     U, Bn = len(ou), len(ob)
     octants_per_s = Bn / t_build
+    if args.build_only:
+        print(json.dumps({"workload": W, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_build * 1e3,
+                          "p2o_ms": t_p2o * 1e3, "balanced_octants": Bn, "octants_per_s": octants_per_s}))
+        return
     build_bytes = 24 * N + 2 * 28 * U + 28 * Bn
     log(f"build+balance {t_build * 1e3:.2f} ms/step ({Bn} octants)")
 
