@@ -102,6 +102,19 @@ struct CoarseSolver {
 
 // Symbolic analysis of the coarsest DA (host) and upload of the plan.
 std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, uint32_t dense = kDenseNodes);
+// The same in three steps, so that the host analysis can run beside device
+// work: host copies of what the analysis reads (synchronises s), the analysis
+// itself (host only: no HIP calls), and the upload of the plan.
+struct CoarseInputs {
+    uint32_t n = 0;
+    size_t E = 0;
+    std::vector<uint32_t> conn, xyz;  // conn [8][E]; xyz 3 per node
+    std::vector<uint16_t> meta;
+    std::vector<uint8_t> bdy;
+};
+CoarseInputs coarse_download(const DaDevice &d, hipStream_t s);
+MfPlan coarse_plan(const CoarseInputs &in, uint32_t dense);
+std::unique_ptr<CoarseSolver> coarse_upload(MfPlan &&plan, uint32_t n, size_t E, hipStream_t s);
 // Numeric factorisation for the operator lap_fac K + mass_fac M with Dirichlet
 // rows; throws DgxError when a pivot is not positive.
 void coarse_factor(CoarseSolver &cs, const DaDevice &d, double lap_fac, double mass_fac, hipStream_t s);

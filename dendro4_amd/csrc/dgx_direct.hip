@@ -813,29 +813,37 @@ void download(std::vector<T> &h, const T *d, size_t n, hipStream_t s) {
 
 }  // namespace
 
-std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, uint32_t dense) {
-    const uint32_t n = d.n_nodes;
-    const size_t E = d.n_elem;
-    std::vector<uint32_t> conn, n2l, x, y, z;
-    std::vector<uint16_t> meta;
-    std::vector<uint8_t> bdy;
-    download(conn, d.conn.p, 8 * E, s);
-    download(meta, d.meta.p, E, s);
-    download(bdy, d.bdy_node.p, n, s);
-    download(n2l, d.n2l.p, n, s);
+CoarseInputs coarse_download(const DaDevice &d, hipStream_t s) {
+    CoarseInputs in;
+    in.n = d.n_nodes;
+    in.E = d.n_elem;
+    std::vector<uint32_t> n2l, x, y, z;
+    download(in.conn, d.conn.p, 8 * in.E, s);
+    download(in.meta, d.meta.p, in.E, s);
+    download(in.bdy, d.bdy_node.p, in.n, s);
+    download(n2l, d.n2l.p, in.n, s);
     download(x, d.x.p, d.n_local, s);
     download(y, d.y.p, d.n_local, s);
     download(z, d.z.p, d.n_local, s);
     HIP_OK(hipStreamSynchronize(s));
-    std::vector<uint32_t> xyz(3 * (size_t)n);
-    for (uint32_t i = 0; i < n; ++i) {
-        xyz[3 * (size_t)i] = x[n2l[i]];
-        xyz[3 * (size_t)i + 1] = y[n2l[i]];
-        xyz[3 * (size_t)i + 2] = z[n2l[i]];
+    in.xyz.resize(3 * (size_t)in.n);
+    for (uint32_t i = 0; i < in.n; ++i) {
+        in.xyz[3 * (size_t)i] = x[n2l[i]];
+        in.xyz[3 * (size_t)i + 1] = y[n2l[i]];
+        in.xyz[3 * (size_t)i + 2] = z[n2l[i]];
     }
-    std::unique_ptr<CoarseSolver> cs(new CoarseSolver());
+    return in;
+}
+
+MfPlan coarse_plan(const CoarseInputs &in, uint32_t dense) {
     // the fronts may take at most 4 GiB of doubles
-    cs->plan = mf_plan(n, E, conn.data(), meta.data(), bdy.data(), xyz.data(), (4ull << 30) / 8, 128, dense);
+    return mf_plan(in.n, in.E, in.conn.data(), in.meta.data(), in.bdy.data(), in.xyz.data(), (4ull << 30) / 8, 128,
+                   dense);
+}
+
+std::unique_ptr<CoarseSolver> coarse_upload(MfPlan &&plan, uint32_t n, size_t E, hipStream_t s) {
+    std::unique_ptr<CoarseSolver> cs(new CoarseSolver());
+    cs->plan = std::move(plan);
     const MfPlan &P = cs->plan;
     cs->n = n;
     cs->ni = P.ni;
@@ -864,8 +872,13 @@ std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, u
     cs->upd.alloc(P.upd_len, s);
     cs->ws.alloc(P.ni, s);
     cs->x.alloc(n, s);
-    HIP_OK(hipStreamSynchronize(s));  // the host vectors of the uploads go out of scope
+    HIP_OK(hipStreamSynchronize(s));  // the uploads read host memory of the plan
     return cs;
+}
+
+std::unique_ptr<CoarseSolver> coarse_analyse(const DaDevice &d, hipStream_t s, uint32_t dense) {
+    const CoarseInputs in = coarse_download(d, s);
+    return coarse_upload(coarse_plan(in, dense), in.n, in.E, s);
 }
 
 void coarse_factor(CoarseSolver &cs, const DaDevice &d, double lap_fac, double mass_fac, hipStream_t s) {

@@ -638,15 +638,61 @@ std::unique_ptr<MgHierarchy> mg_build(const Key *finest, size_t n, uint32_t md, 
         helper.wait_all();
         t_wait += ms_since(tw);
     }
-    // the coarse solver on the helper, the transfers here
-    helper.submit([&mg, &t_an, &t_fa, dense_nodes](hipStream_t hs) {
-        build_coarse(*mg, hs, dense_nodes, timing ? &t_an : nullptr, timing ? &t_fa : nullptr);
-    });
-    const auto tt = std::chrono::steady_clock::now();
-    for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
-    if (timing) HIP_OK(hipStreamSynchronize(s));
-    t_tr = ms_since(tt);
-    helper.wait_all();
+    if (overlap) {
+        // the coarse solver on the helper, the transfers here
+        helper.submit([&mg, &t_an, &t_fa, dense_nodes](hipStream_t hs) {
+            build_coarse(*mg, hs, dense_nodes, timing ? &t_an : nullptr, timing ? &t_fa : nullptr);
+        });
+        const auto tt = std::chrono::steady_clock::now();
+        for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
+        if (timing) HIP_OK(hipStreamSynchronize(s));
+        t_tr = ms_since(tt);
+        helper.wait_all();
+    } else {
+        // One stream: the coarsest level's host analysis runs on a second host
+        // thread (it makes no HIP calls) while this thread builds the
+        // transfers on s; then the plan is uploaded and factorised on s.
+        const DaDevice &dc = *mg->lev.back().da;
+        const CoarseInputs in = coarse_download(dc, s);
+        MfPlan plan;
+        std::string perr;
+        bool pfail = false;
+        std::thread an([&] {
+            const auto ta = std::chrono::steady_clock::now();
+            try {
+                plan = coarse_plan(in, dense_nodes);
+            } catch (DgxError &) {
+                pfail = true;
+                perr = dgx_last_error();
+            } catch (std::exception &e) {
+                pfail = true;
+                perr = e.what();
+            }
+            t_an = ms_since(ta);
+        });
+        struct Join {
+            std::thread &t;
+            ~Join() {
+                if (t.joinable()) t.join();
+            }
+        } join{an};
+        const auto tt = std::chrono::steady_clock::now();
+        for (size_t k = 1; k < mg->lev.size(); ++k) build_transfer(mg->lev[k - 1], mg->lev[k], md, curve, s);
+        if (timing) HIP_OK(hipStreamSynchronize(s));
+        t_tr = ms_since(tt);
+        an.join();
+        if (pfail) {
+            dgx_set_error("%s", perr.c_str());
+            throw DgxError();
+        }
+        mg->coarse = coarse_upload(std::move(plan), in.n, in.E, s);
+        const auto tf = std::chrono::steady_clock::now();
+        coarse_factor(*mg->coarse, dc, mg->lap_fac, mg->mass_fac, s);
+        if (timing) {
+            HIP_OK(hipStreamSynchronize(s));
+            t_fa = ms_since(tf);
+        }
+    }
     HIP_OK(hipStreamSynchronize(s));
     if (timing)
         fprintf(stderr,
