@@ -123,11 +123,18 @@ int dgx_init(int device) {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     HIP_OK(hipDeviceGetDefaultMemPool(&g_ctx.pool, device));
-    // The pool keeps up to 4 GiB of freed memory mapped; beyond that it returns
-    // memory to the device at synchronisation points, so the runtime's own
-    // allocations (kernel scratch, library workspaces) still find room.  The
-    // library's caching allocator absorbs the churn of temporaries.
-    uint64_t thr = uint64_t(4) << 30;
+    // The pool keeps up to an eighth of the device of freed memory mapped (as
+    // the caching allocator keeps up to an eighth in blocks); beyond that it
+    // returns memory to the device at synchronisation points, so the runtime's
+    // own allocations (kernel scratch, library workspaces) still find room
+    // beside a request server's budget of three quarters.  Memory returned to
+    // the device has to be mapped again by the next allocation that needs it
+    // (several milliseconds per GB): with 4 GiB here, the request server with
+    // resident problems, whose evictions free whole hierarchies, spent 16 s of
+    // its 2,000-request stream in hipMallocAsync.
+    size_t fr = 0, tot = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    uint64_t thr = (uint64_t)tot / 8;
     HIP_OK(hipMemPoolSetAttribute(g_ctx.pool, hipMemPoolAttrReleaseThreshold, &thr));
     // device address of the static scratch array of dgx_da_matvec_noalloc
     da_noalloc_init();
