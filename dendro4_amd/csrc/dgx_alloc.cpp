@@ -125,11 +125,13 @@ void *dgx_alloc(size_t bytes, hipStream_t s) {
     const size_t cls = size_class(bytes);
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        // smallest cached block of this class or up to 25% larger, same stream
+        // smallest cached block of this class or up to 25% larger (blocks of
+        // up to 1 MiB) or twice as large (larger blocks), same stream
         auto m = g_cache.find(s);
         if (m != g_cache.end()) {
             auto it = m->second.lower_bound(cls);
-            if (it != m->second.end() && it->first <= cls + cls / 4) {
+            const size_t slack = cls <= (size_t(1) << 20) ? cls / 4 : cls;
+            if (it != m->second.end() && it->first <= cls + slack) {
                 void *p = it->second;
                 note_alloc_locked(p, it->first, s);
                 g_cached_bytes -= it->first;
