@@ -136,6 +136,21 @@ def test_saturated_cell_replay_hand_traced(dgx, oracle):
     assert st[1] >= 1 and st[2] >= 1  # a trigger was found and replayed
 
 
+def test_saturated_cell_replay_hand_traced_morton(dgx, oracle):
+    """The Morton-curve hand trace of the oracle tests: O1 stays unrefined."""
+    pts = (np.array([[1, 1, 1]] * 3 + [[2, 0, 0], [3, 0, 0]]) + 0.5) / 4
+    o = dgx.points2octree(pts, 2, 1, 1)
+    got = o.treenodes()[:, :4].tolist()
+    assert got == [[0, 0, 0, 2], [1, 0, 0, 2], [0, 1, 0, 2], [1, 1, 0, 2],
+                   [0, 0, 1, 2], [1, 0, 1, 2], [0, 1, 1, 2], [1, 1, 1, 2],
+                   [2, 0, 0, 1], [0, 2, 0, 1], [2, 2, 0, 1], [0, 0, 2, 1], [2, 0, 2, 1], [0, 2, 2, 1], [2, 2, 2, 1]]
+    oracle.set_ordering(False)
+    try:
+        assert np.array_equal(o.treenodes(), oracle.points2octree(pts, 2, 1))
+    finally:
+        oracle.set_ordering(True)
+
+
 def test_saturated_cells_random(dgx, oracle):
     """Clusters of identical points at maxDepth: the replay of the reference's
     cursor must reproduce the oracle exactly (or both refuse the sentinel case)."""

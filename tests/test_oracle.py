@@ -148,6 +148,35 @@ def test_p2o_saturated_cell_hand_traced(oracle):
     assert len(ref) == len(got) + 7
 
 
+def test_p2o_saturated_cell_hand_traced_morton(oracle):
+    """p2oLocal with a stale cursor on the Morton curve, traced by hand.
+    maxDepth 2, maxNumPts 1; A = cell (1,1,1) x3 (the last child of octant
+    O0), B1 = (2,0,0) and B2 = (3,0,0) (both in octant O1).  O0 is split; its
+    children c0..c5 find no point before their successor; the window at c6
+    takes the first copy of A (A equals its successor c7, and the search uses
+    less_equal, construct.cpp:1037); the window at c7 takes the second.  The
+    cursor then points at the third copy when O1 is tested (:1011): A is not
+    in O1, so O1 stays unrefined.  At level maxDepth-1 the window widens
+    (:1022-1032: inc 2 -> next 5, not past num_pts 5, so again: inc 4 -> 9,
+    clamped to num_pts-1 = 4) and takes A and B1; the loop ends, and the tail
+    emits O2..O7 unrefined (:1051-1061).  The textbook tree would split O1
+    into 8 level-2 cells."""
+    pts = (np.array([[1, 1, 1]] * 3 + [[2, 0, 0], [3, 0, 0]]) + 0.5) / 4
+    oracle.set_ordering(False)
+    try:
+        got = oracle.points2octree(pts, 2, 1)
+        expect = [[0, 0, 0, 2], [1, 0, 0, 2], [0, 1, 0, 2], [1, 1, 0, 2],
+                  [0, 0, 1, 2], [1, 0, 1, 2], [0, 1, 1, 2], [1, 1, 1, 2],
+                  [2, 0, 0, 1], [0, 2, 0, 1], [2, 2, 0, 1], [0, 0, 2, 1], [2, 0, 2, 1], [0, 2, 2, 1], [2, 2, 2, 1]]
+        assert got[:, :4].tolist() == expect
+        assert oracle.is_complete(got)
+        q = oracle.quantise(pts, 2)[:, :3].astype(np.int64)
+        ref = oracle.sort(np.array([tn(*r, md=2) for r in textbook_leaves(q, 1, 2)], np.uint32))
+        assert len(ref) == len(got) + 7
+    finally:
+        oracle.set_ordering(True)
+
+
 def test_p2o_stale_cursor_at_domain_end_is_reported(oracle):
     """When a stale cursor is still active at the last octant, the reference
     compares points with getNext's past-the-end sentinel (TreeNode.tcc:483-492)
