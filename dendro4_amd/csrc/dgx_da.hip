@@ -17,6 +17,7 @@
 // (DA::vecGetBuffer, oda.cpp:863-882).
 #include "dgx_da.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <vector>
 
@@ -234,18 +235,35 @@ __global__ void k_da_mark_nodes(const uint32_t *nlist, size_t nelem, size_t n, u
     if (r < n) isnode[r] = 1;
 }
 
-__global__ void k_da_apply_nodes(const uint8_t *isnode, size_t n, size_t nelem, uint8_t *flags, uint32_t *nodeflag,
-                                 uint32_t *counts /* [0] elem nodes [1] bnd nodes */) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool on = i < n && isnode[i];
-    uint32_t ce = __popcll(__ballot(on && i < nelem)), cb = __popcll(__ballot(on && i >= nelem));
-    if (__lane_id() == 0) {
-        if (ce) atomicAdd(&counts[0], ce);
-        if (cb) atomicAdd(&counts[1], cb);
+// Grid-stride over a capped grid; the two counts are summed per workgroup and
+// added with one atomic each (one atomic per wave on the same two words cost
+// ~5 ns each, serialised: 2.4 ms on a 30M-octant local buffer).
+__global__ void __launch_bounds__(B) k_da_apply_nodes(const uint8_t *isnode, size_t n, size_t nelem, uint8_t *flags,
+                                                      uint32_t *nodeflag,
+                                                      uint32_t *counts /* [0] elem nodes [1] bnd nodes */) {
+    __shared__ uint32_t s_cnt[2];
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t ce = 0, cb = 0;
+    for (size_t i = (size_t)blockIdx.x * B + threadIdx.x; i < n; i += (size_t)gridDim.x * B) {
+        uint8_t f = flags[i];
+        if (isnode[i]) {
+            if (!(f & F_NODE)) flags[i] = f = (uint8_t)(f | F_NODE);
+            if (i < nelem) ++ce;
+            else ++cb;
+        }
+        nodeflag[i] = (f & F_NODE) ? 1u : 0u;
     }
-    if (i >= n) return;
-    if (on) flags[i] |= F_NODE;
-    nodeflag[i] = (flags[i] & F_NODE) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) {
+        ce += (uint32_t)__shfl_xor(ce, o, 64);
+        cb += (uint32_t)__shfl_xor(cb, o, 64);
+    }
+    if (__lane_id() == 0) {
+        if (ce) atomicAdd(&s_cnt[0], ce);
+        if (cb) atomicAdd(&s_cnt[1], cb);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && s_cnt[threadIdx.x]) atomicAdd(&counts[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 __global__ void k_da_l2n(const uint32_t *nodeflag, const uint32_t *pos, size_t n, uint32_t *l2n, uint32_t *n2l) {
@@ -332,7 +350,8 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     HIP_OK(hipMemsetAsync(isnode.p, 0, N, s));
     k_da_mark_nodes<<<grid_for(8 * n, B), B, 0, s>>>(da->nlist.p, n, N, isnode.p);
     DevBuf<uint32_t> nodeflag(N, s), pos(N, s);
-    k_da_apply_nodes<<<grid_for(N, B), B, 0, s>>>(isnode.p, N, n, da->flags.p, nodeflag.p, ctr.p + 2);
+    k_da_apply_nodes<<<(unsigned)std::min<size_t>(grid_for(N, B), 2048), B, 0, s>>>(isnode.p, N, n, da->flags.p,
+                                                                                  nodeflag.p, ctr.p + 2);
     uint64_t nn = scan_exclusive_u32(nodeflag.p, pos.p, N, s);
     da->l2n.alloc(N, s);
     da->n2l.alloc(nn, s);
@@ -496,8 +515,7 @@ __global__ void __launch_bounds__(kBlkNodesB) k_blk_nodes(const uint32_t *conn, 
                                                           uint32_t eb, const uint32_t *epos, uint32_t *ulist,
                                                           uint32_t *ucount, uint32_t nblk, uint16_t *lconn) {
     constexpr uint32_t CAP = 8 * kBlkNodesE, PER_T = CAP / kBlkNodesB;  // 16 entries per thread
-    __shared__ uint32_t v[CAP];
-    __shared__ uint32_t u[CAP];
+    __shared__ uint32_t v[CAP];  // the sorted vertex ids, then the unique ones in place (16 KB: eight blocks per CU)
     __shared__ uint32_t scan_tmp[kBlkNodesB / 64 + 1];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
     const size_t e0 = (size_t)b * eb;
@@ -530,22 +548,28 @@ __global__ void __launch_bounds__(kBlkNodesB) k_blk_nodes(const uint32_t *conn, 
         }
     }
     __syncthreads();
-    // unique: each thread flags 16 consecutive entries
+    // unique, in place: each thread takes its 16 consecutive entries and the
+    // one before them into registers; the scan's barriers separate every
+    // thread's reads from the compacted writes
     const uint32_t i0 = tid * PER_T;
-    uint32_t f = 0;
+    uint32_t r[PER_T];
+    uint32_t prv = i0 > 0 && i0 - 1 < m ? v[i0 - 1] : 0u, f = 0, fl = 0;
 #pragma unroll
     for (int q = 0; q < (int)PER_T; ++q) {
         const uint32_t i = i0 + q;
-        f += (i < m && (i == 0 || v[i] != v[i - 1])) ? 1u : 0u;
+        r[q] = i < m ? v[i] : 0u;
+        const bool head = i < m && (i == 0 || r[q] != prv);
+        prv = r[q];
+        fl |= head ? 1u << q : 0u;
+        f += head ? 1u : 0u;
     }
     uint32_t nu;
     uint32_t w = block_excl_scan_u32(f, scan_tmp, nu);
 #pragma unroll
-    for (int q = 0; q < (int)PER_T; ++q) {
-        const uint32_t i = i0 + q;
-        if (i < m && (i == 0 || v[i] != v[i - 1])) u[w++] = v[i];
-    }
+    for (int q = 0; q < (int)PER_T; ++q)
+        if ((fl >> q) & 1u) v[w++] = r[q];
     __syncthreads();
+    const uint32_t *u = v;
     if (tid == 0) ucount[b] = nu;
     uint32_t *ul = ulist + (size_t)b * 8 * eb;
     for (uint32_t i = tid; i < nu; i += kBlkNodesB) ul[i] = u[i];
@@ -661,25 +685,31 @@ __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
 // Two waves per block colour 8 nodes per step.  Measured on the lognormal-4M
 // DA (matvec back to back / multigrid set-up): one wave, 4 nodes per step,
 // 29.5 us / 30.0 ms; two waves 29.7 us / 29.0 ms; four waves 30.0 us / 28.3 ms.
-constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024, kColB = 128;
+// The LDS tables are sized by the template: <kColSmallN, kColSmallE> (24 KB,
+// seven workgroups per CU) takes the blocks that fit it, and <kColMaxN,
+// kColMaxE> (49 KB, three per CU) the rest (big = true skips the small ones).
+// Both number a block identically.
+constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024, kColSmallN = 1024, kColSmallE = 512, kColB = 128;
+template <uint32_t MAXN, uint32_t MAXE, bool BIG>
 __global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint32_t *boff, uint32_t *bnode,
                                                      size_t ne, uint32_t eb) {
-    __shared__ uint32_t deg[kColMaxN];             // degree, then fill position
-    __shared__ uint16_t ptr[kColMaxN + 1];         // adjacency offsets
-    __shared__ uint16_t adj[8 * kColMaxE];         // lane groups of each node's uses
+    __shared__ uint32_t deg[MAXN];             // degree, then fill position
+    __shared__ uint16_t ptr[MAXN + 1];         // adjacency offsets
+    __shared__ uint16_t adj[8 * MAXE];         // lane groups of each node's uses
     // [lane group][colour] uses so far, one byte each (at most 128 uses per
     // lane group), four to a word for the atomic updates
-    __shared__ uint32_t cnt32[(kColMaxE / 16) * 8 * 16 / 4];
+    __shared__ uint32_t cnt32[(MAXE / 16) * 8 * 16 / 4];
     uint8_t *cnt = reinterpret_cast<uint8_t *>(cnt32);
-    __shared__ uint16_t newidx[kColMaxN];
-    __shared__ uint32_t tmp[kColMaxN];
+    __shared__ uint16_t newidx[MAXN];
+    __shared__ uint32_t tmp[MAXN];
     __shared__ uint32_t used[16];
     constexpr uint32_t kColG = kColB / 16;
     __shared__ uint32_t pick[kColG];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
     const uint32_t s0 = boff[b], n = boff[b + 1] - s0;
     const size_t e0 = (size_t)b * eb;
-    if (n > kColMaxN || eb > kColMaxE || e0 >= ne) return;  // uniform over the workgroup
+    if (n > MAXN || eb > MAXE || e0 >= ne) return;  // uniform over the workgroup
+    if (BIG && n <= kColSmallN && eb <= kColSmallE) return;  // coloured by the small variant
     const uint32_t nbe = (uint32_t)min((size_t)eb, ne - e0), nuse = 8 * nbe;
     const uint32_t ngroups = ((nbe + 15) / 16) * 8;
     uint16_t *rec = lconn + e0 * 8;
@@ -854,7 +884,11 @@ void da_build_blocks(DaDevice &da, hipStream_t s, bool colour) {
     }
     da.cmeta.alloc((size_t)nblk * eb, s);  // whole blocks, like lconn
     k_blk_cmeta<<<grid_for(E, B), B, 0, s>>>(da.meta.p, E, eb, epos.p, da.cmeta.p);
-    if (colour) k_blk_color<<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
+    if (colour) {
+        k_blk_color<kColSmallN, kColSmallE, false><<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p,
+                                                                           E, eb);
+        k_blk_color<kColMaxN, kColMaxE, true><<<nblk, kColB, 0, s>>>(da.lconn.p, da.bnode_off.p, da.bnode.p, E, eb);
+    }
     HIP_OK(hipGetLastError());
     epos.release();
     // shared nodes and their node-major partial slots
