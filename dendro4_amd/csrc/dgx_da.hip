@@ -685,23 +685,29 @@ __global__ void k_blk_mark_bdy(uint32_t *ids, uint64_t n, const uint8_t *bdy) {
 // Two waves per block colour 8 nodes per step.  Measured on the lognormal-4M
 // DA (matvec back to back / multigrid set-up): one wave, 4 nodes per step,
 // 29.5 us / 30.0 ms; two waves 29.7 us / 29.0 ms; four waves 30.0 us / 28.3 ms.
-// The LDS tables are sized by the template: <kColSmallN, kColSmallE> (24 KB,
-// seven workgroups per CU) takes the blocks that fit it, and <kColMaxN,
-// kColMaxE> (49 KB, three per CU) the rest (big = true skips the small ones).
-// Both number a block identically.
+// The LDS tables are sized by the template: <kColSmallN, kColSmallE> (17 KB,
+// nine workgroups per CU) takes the blocks that fit it, and <kColMaxN,
+// kColMaxE> (33 KB, four per CU) the rest (big = true skips the small ones).
+// Both number a block identically.  The colour use counts reuse the degree
+// table, and the node-list copy at the end the adjacency table, each once the
+// earlier one is dead.
 constexpr uint32_t kColMaxN = 2048, kColMaxE = 1024, kColSmallN = 1024, kColSmallE = 512, kColB = 128;
 template <uint32_t MAXN, uint32_t MAXE, bool BIG>
 __global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint32_t *boff, uint32_t *bnode,
                                                      size_t ne, uint32_t eb) {
-    __shared__ uint32_t deg[MAXN];             // degree, then fill position
-    __shared__ uint16_t ptr[MAXN + 1];         // adjacency offsets
-    __shared__ uint16_t adj[8 * MAXE];         // lane groups of each node's uses
-    // [lane group][colour] uses so far, one byte each (at most 128 uses per
-    // lane group), four to a word for the atomic updates
-    __shared__ uint32_t cnt32[(MAXE / 16) * 8 * 16 / 4];
-    uint8_t *cnt = reinterpret_cast<uint8_t *>(cnt32);
+    constexpr uint32_t kCntWords = (MAXE / 16) * 8 * 16 / 4;
+    // degree, then fill position; then [lane group][colour] uses so far, one
+    // byte each (at most 128 uses per lane group), four to a word for the
+    // atomic updates
+    __shared__ uint32_t degcnt[MAXN > kCntWords ? MAXN : kCntWords];
+    uint32_t *deg = degcnt, *cnt32 = degcnt;
+    uint8_t *cnt = reinterpret_cast<uint8_t *>(degcnt);
+    __shared__ uint16_t ptr[MAXN + 1];  // adjacency offsets
+    // lane groups of each node's uses; then the copy of the node list
+    __shared__ uint32_t adjw[4 * MAXE > MAXN ? 4 * MAXE : MAXN];
+    uint16_t *adj = reinterpret_cast<uint16_t *>(adjw);
+    uint32_t *tmp = adjw;
     __shared__ uint16_t newidx[MAXN];
-    __shared__ uint32_t tmp[MAXN];
     __shared__ uint32_t used[16];
     constexpr uint32_t kColG = kColB / 16;
     __shared__ uint32_t pick[kColG];
@@ -714,7 +720,6 @@ __global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint
     const uint32_t ngroups = ((nbe + 15) / 16) * 8;
     uint16_t *rec = lconn + e0 * 8;
     for (uint32_t i = tid; i < n; i += kColB) deg[i] = 0;
-    for (uint32_t i = tid; i < ngroups * 4; i += kColB) cnt32[i] = 0;
     if (tid < 16) used[tid] = 0;
     __syncthreads();
     for (uint32_t t = tid; t < nuse; t += kColB) atomicAdd(&deg[rec[t] >> 3], 1u);
@@ -743,6 +748,8 @@ __global__ void __launch_bounds__(kColB) k_blk_color(uint16_t *lconn, const uint
         const uint32_t q = atomicAdd(&deg[rec[t] >> 3], 1u);
         adj[q] = (uint16_t)((t >> 7) * 8 + (t & 7));  // element slot t/8, lane group (t/8)/16, position t%8
     }
+    __syncthreads();
+    for (uint32_t i = tid; i < ngroups * 4; i += kColB) cnt32[i] = 0;  // the degrees are dead
     __syncthreads();
     const uint32_t base = n >> 4, extra = n & 15u;
     // kColG nodes per step, one per 16-lane group, taken from the kColG equal
