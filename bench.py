@@ -27,7 +27,9 @@ Also reported on the same line:
   serve          the 2,000-request served stream (point counts log-uniform in
                  [2^12, 2^22], three distributions, maxNumPts 1/4/8, Poisson
                  arrivals at 200/s, 200 GB HBM budget, problems kept resident):
-                 solves/s and p50/p99 latency, FP64 and FP32-MG;
+                 solves/s and p50/p99 latency, FP64 and FP32-MG, in a process
+                 of its own as a server runs (--serve-in-process: in this one,
+                 after the other sections);
   cpu_baseline   the oracle (CPU restatement, OpenMP sorts as the reference's
                  omp_par::merge_sort) on all host cores, timed on the whole
                  headline workload.
@@ -36,6 +38,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -250,6 +253,21 @@ def serve_section(dgx, count, slots, budget):
     return out
 
 
+def serve_in_child(dgx, count, slots):
+    """The served stream in a process of its own, as a server runs: this
+    process first returns its cached device memory, then waits for the child,
+    whose progress lines go to stderr."""
+    dgx.trim_cache()
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", "--serve-requests", str(count),
+           "--serve-slots", str(slots)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"served stream process failed with status {r.returncode}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["process"] = "own process"
+    return out
+
+
 def cpu_section(pts, md, mnp):
     from oracle import pyoracle as O
     if not os.path.exists(O.LIB_PATH):
@@ -301,6 +319,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served stream, a prefix of the 2,000-request stream (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-in-process", action="store_true",
+                    help="serve in this process, after the other sections, instead of a process of its own")
+    ap.add_argument("--serve-only", action="store_true", help="only the served stream (one JSON line)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--matvec-only", metavar="WORKLOAD", default=None,
                     help="build the workload's DA, run matvecs and a 256 MiB device copy (counter runs)")
@@ -314,6 +335,9 @@ def main():
 
     import dendro4_amd as dgx
     dgx.init(0)
+    if args.serve_only:
+        print(json.dumps(serve_section(dgx, args.serve_requests, args.serve_slots, 200e9)))
+        return
     stream = dgx.stream()
     ev = HipEvents()
     if args.matvec_only:
@@ -379,7 +403,11 @@ def main():
 
     serve = None
     if args.serve_requests > 0:
-        serve = serve_section(dgx, args.serve_requests, args.serve_slots, 200e9)
+        if args.serve_in_process:
+            serve = serve_section(dgx, args.serve_requests, args.serve_slots, 200e9)
+            serve["process"] = "the bench's own, after the other sections"
+        else:
+            serve = serve_in_child(dgx, args.serve_requests, args.serve_slots)
 
     cpu = None if args.no_cpu else cpu_section(pts, MD, MP)
 

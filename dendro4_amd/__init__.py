@@ -39,7 +39,7 @@ EXPORTED = [
     "dgx_read_data_pts", "dgx_write_data_pts", "dgx_read_nodes", "dgx_write_nodes", "dgx_read_nodes_binary",
     "dgx_write_nodes_binary", "dgx_write_vtk", "dgx_server_get_stats", "dgx_server_inject_fault",
     "dgx_mg_coarse_info", "dgx_direct_plan_host", "dgx_direct_plan_export", "dgx_direct_plan_free",
-    "dgx_da_matvec_noalloc", "dgx_memory_info", "dgx_mg_level_da", "dgx_da_boundary_flags", "dgx_mg_restrict", "dgx_mg_prolong",
+    "dgx_da_matvec_noalloc", "dgx_memory_info", "dgx_trim_cache", "dgx_mg_level_da", "dgx_da_boundary_flags", "dgx_mg_restrict", "dgx_mg_prolong",
     "dgx_da_compress_maps", "dgx_da_decode_maps",
 ]
 
@@ -135,6 +135,7 @@ def load_library():
         "dgx_server_get_stats": ([vp, C.POINTER(ServerStats)], i32),
         "dgx_mg_coarse_info": ([vp, vp], i32),
         "dgx_memory_info": ([vp, i32], i32),
+        "dgx_trim_cache": ([], i32),
         "dgx_mg_level_da": ([vp, i32, C.POINTER(vp)], i32),
         "dgx_mg_restrict": ([vp, i32, vp, vp], i32),
         "dgx_mg_prolong": ([vp, i32, vp, vp], i32),
@@ -216,6 +217,12 @@ def memory_info(reset_peak=False):
     out = (C.c_uint64 * 5)()
     _check(_lib_ready().dgx_memory_info(out, 1 if reset_peak else 0))
     return dict(zip(("total", "device_used", "library_live", "library_peak", "cached"), list(out)))
+
+
+def trim_cache():
+    """Return the library's cached blocks and the pool's unused memory to the
+    device (waits for the device)."""
+    _check(_lib_ready().dgx_trim_cache())
 
 
 def synchronize():

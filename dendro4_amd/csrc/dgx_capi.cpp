@@ -169,6 +169,15 @@ int dgx_memory_info(uint64_t out[5], int reset_peak) {
     API_END
 }
 
+int dgx_trim_cache(void) {
+    API_BEGIN
+    ensure_ctx();
+    dgx_alloc_trim();
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemPoolTrimTo(g_ctx.pool, 0));
+    API_END
+}
+
 int dgx_device_malloc(void **p, size_t bytes) {
     API_BEGIN
     ensure_ctx();

@@ -90,6 +90,33 @@ def test_dropped_hierarchy_returns_its_memory(dgx, problem):
     assert dgx.memory_info()["library_live"] == base
 
 
+def test_trim_cache_empties_the_block_cache(dgx, problem):
+    """dgx_trim_cache returns the cached blocks and the pool's unused memory:
+    the cache is empty afterwards, the device holds no more than before, and
+    the library allocates and solves as before."""
+    ob = problem[0]
+    mg = dgx.MG(ob, 3)
+    mg = None
+    dgx.synchronize()
+    before = dgx.memory_info()
+    assert before["cached"] > 0
+    dgx.trim_cache()
+    after = dgx.memory_info()
+    assert after["cached"] == 0
+    assert after["device_used"] <= before["device_used"]
+    assert after["library_live"] == before["library_live"]
+    mg = dgx.MG(ob, 3)
+    n = mg.n_nodes
+    df = dgx.DeviceArray.from_host(np.random.default_rng(5).standard_normal(n))
+    db = dgx.DeviceArray((n,), np.float64)
+    du = dgx.DeviceArray((n,), np.float64)
+    mg.rhs(df, db)
+    st = mg.solve(db, du, fp32=False)
+    assert st["rel_residual"] < 1e-8
+    mg = None
+    dgx.synchronize()
+
+
 @pytest.mark.parametrize("nlevels", [1, 3, 5])
 def test_mg_solve_matches_direct_solve(dgx, problem, nlevels):
     ob, rda, A, bd = problem
