@@ -306,6 +306,33 @@ def matvec_only(dgx, ev, stream, W):
                       "nodes": da.n_nodes, **da.block_info}))
 
 
+def da_only(dgx, W):
+    """Kernel traces of the set-up: workload W's octree, then 4 DA builds and
+    2 MG(5) set-ups one after another, each timed to the end of its work; one
+    line."""
+    dpts = dgx.DeviceArray.from_host(dgx.workload_points(W))
+    ob = dgx.balance_octree(dgx.points2octree(dpts, 30, dgx.workload_max_num_pts(W)))
+    del dpts
+    dgx.synchronize()
+    t_da, t_mg = [], []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        da = dgx.DA(ob)
+        dgx.synchronize()
+        t_da.append(time.perf_counter() - t0)
+        E = da.element_size
+        del da
+    for _ in range(2):
+        t0 = time.perf_counter()
+        mg = dgx.MG(ob, 5)
+        dgx.synchronize()
+        t_mg.append(time.perf_counter() - t0)
+        levels = mg.level_octants
+        del mg
+    print(json.dumps({"workload": W, "elements": E, "da_build_ms": [t * 1e3 for t in t_da],
+                      "mg_setup_ms": [t * 1e3 for t in t_mg], "level_octants": levels}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,6 +352,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--matvec-only", metavar="WORKLOAD", default=None,
                     help="build the workload's DA, run matvecs and a 256 MiB device copy (counter runs)")
+    ap.add_argument("--da-only", metavar="WORKLOAD", default=None,
+                    help="4 DA builds and 2 MG(5) set-ups of the workload's octree (kernel traces)")
     ap.add_argument("--build-only", action="store_true",
                     help="only the timed points2Octree + balanceOctree steps of lognormal-4M (kernel traces)")
     args = ap.parse_args()
@@ -342,6 +371,9 @@ def main():
     ev = HipEvents()
     if args.matvec_only:
         matvec_only(dgx, ev, stream, args.matvec_only)
+        return
+    if args.da_only:
+        da_only(dgx, args.da_only)
         return
 
     W = "lognormal-4M"
