@@ -111,21 +111,28 @@ __device__ inline bool max_lower_bound(const Key *a, size_t n, const Key &k, siz
     return true;
 }
 
+// The probes' keys come from the octant's own curve path (path_walk once,
+// then key_from_path: only the levels below the first one at which the probe
+// leaves the octant's ancestors are walked) instead of a full encode each.
 __global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uint32_t md, SfcPacked tab,
                                                      const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
                                                      uint8_t *flags) {
     LOAD_TABLES(tab)
+    __shared__ uint8_t s_rot[kTailCols * B];  // rotation states of each thread's path, levels 1..32
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     uint32_t lev = flags[i] & 31u;
     bool hanging = false;
+    uint8_t *rots = s_rot + threadIdx.x;
     if (lev != 1) {
         uint32_t X = xs[i], Y = ys[i], Z = zs[i], s = 1u << (md - lev);
         uint32_t cn = (((Z >> (md - lev)) & 1u) << 2) | (((Y >> (md - lev)) & 1u) << 1) | ((X >> (md - lev)) & 1u);
+        const Key kref = (cn != 0 && cn != 7) ? path_walk(s_rc, X, Y, Z, lev, md, rots, B) : Key{0ull, 0ull};
         // parent of the same-level neighbour at (a,b,c), as a key at level lev-1
         auto probe = [&](uint32_t a, uint32_t b, uint32_t c) {
             uint32_t ps = md - lev + 1;
-            Key pk = dev_encode(s_rc, (a >> ps) << ps, (b >> ps) << ps, (c >> ps) << ps, lev - 1, md);
+            Key pk = key_from_path(s_rc, kref, rots, B, X, Y, Z, (a >> ps) << ps, (b >> ps) << ps, (c >> ps) << ps,
+                                   lev - 1, lev - 1, md, nullptr);
             if (bsearch_exact(k, n, pk)) hanging = true;
         };
         switch (cn) {
@@ -142,14 +149,21 @@ __global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uin
 }
 
 // buildNodeList for one element (odaBuildNlist.cpp:531-934 on one rank).
+// The vertex keys come from the element's curve path (path_walk once, then
+// key_from_path with the zero-digit tails of the level-md keys in `tail`,
+// k_zero_tail) instead of a 31-level encode each.
 __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t nelem, uint32_t md, SfcPacked tab,
                                                 const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
-                                                const uint8_t *flags, uint32_t *nlist, uint8_t *mask,
+                                                const uint8_t *flags, const Key *__restrict__ tail, uint32_t *nlist,
+                                                uint8_t *mask,
                                                 uint32_t *counters /* [0]=unresolved [1]=ghost scans */) {
     LOAD_TABLES(tab)
+    __shared__ uint8_t s_rot[kTailCols * B];  // rotation states of each thread's path, levels 1..32
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
     if (i >= nelem) return;
     const uint32_t d = flags[i] & 31u, x = xs[i], y = ys[i], z = zs[i];
+    uint8_t *rots = s_rot + threadIdx.x;
+    const Key kref = path_walk(s_rc, x, y, z, d, md, rots, B);
     const bool inode = flags[i] & F_NODE;
     const uint32_t ps = md - d + 1, parX = (x >> ps) << ps, parY = (y >> ps) << ps, parZ = (z >> ps) << ps;
     const uint32_t sz = 1u << (md - d);
@@ -163,14 +177,14 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
         if (j == 0) {
             uint32_t r = (uint32_t)i;
             if (!inode) {
-                Key pk = dev_encode(s_rc, wx, wy, wz, md, md);
+                Key pk = key_from_path(s_rc, kref, rots, B, x, y, z, wx, wy, wz, d - 1, md, md, tail);
                 found = max_lower_bound(k, n, pk, idx);
                 if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) r = (uint32_t)idx;
             }
             nl[0] = r;
             continue;
         }
-        Key sk = dev_encode(s_rc, vx, vy, vz, md, md);
+        Key sk = key_from_path(s_rc, kref, rots, B, x, y, z, vx, vy, vz, d, md, md, tail);
         uint32_t sx = vx, sy = vy, szz = vz, lastLevel = d;
         found = max_lower_bound(k, n, sk, idx);
         if (found && !(key_is_ancestor(k[idx], sk) || key_eq(k[idx], sk))) found = false;
@@ -181,7 +195,7 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
         }
         if (!secondary) continue;
         sx = wx; sy = wy; szz = wz; lastLevel = d - 1;
-        Key pk = dev_encode(s_rc, wx, wy, wz, md, md);
+        Key pk = key_from_path(s_rc, kref, rots, B, x, y, z, wx, wy, wz, d - 1, md, md, tail);
         found = max_lower_bound(k, n, pk, idx);
         if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) {
             nl[j] = (uint32_t)idx;
@@ -226,6 +240,12 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
         }
     }
     mask[i] = m;
+}
+
+// zero_tail table of the level-md keys: tail[r * kTailCols + s], r < 24.
+__global__ void k_zero_tail(SfcPacked tab, uint32_t md, Key *tail) {
+    for (uint32_t e = threadIdx.x; e < 24 * kTailCols; e += blockDim.x)
+        tail[e] = zero_tail(tab.rc, e / kTailCols, e % kTailCols, md);
 }
 
 __global__ void k_da_mark_nodes(const uint32_t *nlist, size_t nelem, size_t n, uint8_t *isnode) {
@@ -343,8 +363,10 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     da->mask.alloc(n, s);
     DevBuf<uint32_t> ctr(4, s);
     HIP_OK(hipMemsetAsync(ctr.p, 0, 16, s));
+    DevBuf<Key> tail(24 * kTailCols, s);
+    k_zero_tail<<<1, B, 0, s>>>(tab, md, tail.p);
     k_da_nlist<<<grid_for(n, B), B, 0, s>>>(da->key.p, N, n, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p,
-                                           da->nlist.p, da->mask.p, ctr.p);
+                                           tail.p, da->nlist.p, da->mask.p, ctr.p);
     HIP_OK(hipGetLastError());
     DevBuf<uint8_t> isnode(N, s);
     HIP_OK(hipMemsetAsync(isnode.p, 0, N, s));

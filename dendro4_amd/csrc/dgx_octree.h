@@ -83,6 +83,75 @@ __host__ __device__ inline Key morton_to_curve_rot(const uint16_t *rc2, const ui
     rot = r;
     return k;
 }
+// Curve keys of points near a reference octant, from the reference's path.
+// path_walk encodes the octant at level d anchored at (x, y, z) and records
+// the rotation state before every level t = 1..d+1 in rots[t * stride].
+__host__ __device__ inline Key path_walk(const uint8_t *rc, uint32_t x, uint32_t y, uint32_t z, uint32_t d,
+                                         uint32_t md, uint8_t *rots, uint32_t stride) {
+    Key k{0ull, 0ull};
+    uint32_t rot = 0;
+    for (uint32_t t = 1; t <= d; ++t) {
+        rots[t * stride] = (uint8_t)rot;
+        const uint32_t b = md - t;
+        const uint32_t m = (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+        const uint32_t v = rc[rot * 8 + m];
+        rot = v >> 3;
+        key_set_digit(k, t, v & 7u);
+    }
+    rots[(d + 1) * stride] = (uint8_t)rot;
+    k.lo |= d;
+    return k;
+}
+// Digits of levels s..md of the curve path whose Morton digits are all 0 (a
+// point of the level-(s-1) grid below level s-1), starting in rotation state r
+// before level s; the level field is 0.  Levels past md give no digits.
+__host__ __device__ inline Key zero_tail(const uint8_t *rc, uint32_t r, uint32_t s, uint32_t md) {
+    Key k{0ull, 0ull};
+    for (uint32_t t = s; t <= md; ++t) {
+        const uint32_t v = rc[r * 8];
+        r = v >> 3;
+        key_set_digit(k, t, v & 7u);
+    }
+    return k;
+}
+constexpr uint32_t kTailCols = 33;  // start levels 0..32 of the zero_tail table [24][kTailCols]
+// The key at level lt of the point (px, py, pz) of the level-g grid
+// (coordinates multiples of 2^(md-g), taken mod 2^md), g <= d, from the path
+// of the level-d octant anchored at (x, y, z): its key kref and rotation
+// states rots (path_walk).  Levels above the first one at which the point and
+// the anchor differ are the reference's digits; the levels from there down
+// to g are walked; the levels g+1..lt, where the point's Morton digits are
+// 0, come from tail[rot * kTailCols + g + 1] = zero_tail(rc, rot, g + 1, md)
+// (needed only when lt > g, and then lt must be md).  Equals the per-level
+// encode of the point at level lt.
+__host__ __device__ inline Key key_from_path(const uint8_t *rc, const Key &kref, const uint8_t *rots, uint32_t stride,
+                                             uint32_t x, uint32_t y, uint32_t z, uint32_t px, uint32_t py,
+                                             uint32_t pz, uint32_t g, uint32_t lt, uint32_t md, const Key *tail) {
+    const uint32_t dom = md >= 32 ? 0xffffffffu : (1u << md) - 1u;
+    const uint32_t diff = ((x ^ px) | (y ^ py) | (z ^ pz)) & dom;
+    uint32_t p = g;  // levels 1..p are the reference's digits
+    if (diff) {
+        const uint32_t first = md - (31u - (uint32_t)__builtin_clz(diff));  // first level that differs
+        if (first - 1 < p) p = first - 1;
+    }
+    Key k = key_truncate(kref, p);
+    k.lo &= ~31ull;
+    uint32_t rot = rots[(p + 1) * stride];
+    for (uint32_t t = p + 1; t <= g; ++t) {
+        const uint32_t b = md - t;
+        const uint32_t m = (((pz >> b) & 1u) << 2) | (((py >> b) & 1u) << 1) | ((px >> b) & 1u);
+        const uint32_t v = rc[rot * 8 + m];
+        rot = v >> 3;
+        key_set_digit(k, t, v & 7u);
+    }
+    if (lt > g) {
+        const Key &tl = tail[rot * kTailCols + g + 1];
+        k.hi |= tl.hi;
+        k.lo |= tl.lo;
+    }
+    k.lo |= lt;
+    return k;
+}
 // Key of Morton child m (bit 0: +x, 1: +y, 2: +z) of the octant with key p
 // and rotation state rot (from dev_encode_rot); equals dev_encode of the child.
 __device__ inline Key child_key(const uint8_t *rc, Key p, uint32_t rot, uint32_t m) {

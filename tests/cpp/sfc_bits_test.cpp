@@ -3,9 +3,15 @@
 // encode/decode against the per-level definition (digit t of a key = bit
 // md - t of z, y, x), and the two-levels-per-read conversion of Morton keys
 // to curve keys (morton_to_curve_rot) against the per-level table walk
-// sfc_encode, for both curves, every maxDepth 1..31 and every level.
+// sfc_encode, for both curves, every maxDepth 1..31 and every level.  Also
+// the keys the DA build derives from an element's path (path_walk,
+// key_from_path with the zero_tail table): the element's vertices and its
+// parent's vertices at level md, its neighbours' parents at the level above,
+// points of the coarser grids anywhere in the domain, and vertices one cell
+// past the domain (taken mod 2^md), against sfc_encode.
 #include <cstdio>
 #include <random>
+#include <vector>
 
 #include "dgx_octree.h"
 
@@ -63,6 +69,64 @@ int main() {
                         if (bad++ < 8) printf("morton_to_curve_rot curve=%d md=%u lev=%u\n", curve, md, lev);
                     }
                 }
+    }
+    // keys from an element's path
+    for (int curve = 0; curve < 2; ++curve) {
+        const SfcTables &T = host_sfc(curve);
+        const SfcPacked P = make_packed(curve);
+        for (uint32_t md = 2; md <= 31; ++md) {
+            std::vector<Key> tail(24 * kTailCols);
+            for (uint32_t r = 0; r < 24; ++r)
+                for (uint32_t s0 = 0; s0 < kTailCols; ++s0) tail[r * kTailCols + s0] = zero_tail(P.rc, r, s0, md);
+            const uint32_t dom = md == 32 ? 0xffffffffu : (1u << md) - 1u;
+            for (uint32_t d = 1; d <= md; ++d)
+                for (int r = 0; r < 60; ++r) {
+                    const uint32_t sh = md - d, sz = 1u << sh;
+                    const uint32_t x = ((uint32_t)rng() & dom) >> sh << sh, y = ((uint32_t)rng() & dom) >> sh << sh,
+                                   z = ((uint32_t)rng() & dom) >> sh << sh;
+                    uint8_t rots[33];
+                    const Key kref = path_walk(P.rc, x, y, z, d, md, rots, 1);
+                    const Key e0 = sfc_encode(T, x, y, z, d, md);
+                    ++n;
+                    if (kref.hi != e0.hi || kref.lo != e0.lo) {
+                        if (bad++ < 8) printf("path_walk curve=%d md=%u d=%u\n", curve, md, d);
+                        continue;
+                    }
+                    auto check = [&](uint32_t px, uint32_t py, uint32_t pz, uint32_t g, uint32_t lt, const char *what) {
+                        const Key got = key_from_path(P.rc, kref, rots, 1, x, y, z, px, py, pz, g, lt, md, tail.data());
+                        const Key ref = sfc_encode(T, px & dom, py & dom, pz & dom, lt, md);
+                        ++n;
+                        if (got.hi != ref.hi || got.lo != ref.lo) {
+                            if (bad++ < 8) printf("key_from_path %s curve=%d md=%u d=%u g=%u lt=%u\n", what, curve, md, d, g, lt);
+                        }
+                    };
+                    const uint32_t ps = sh + 1;
+                    const uint32_t parX = d >= 1 ? x >> ps << ps : 0, parY = d >= 1 ? y >> ps << ps : 0,
+                                   parZ = d >= 1 ? z >> ps << ps : 0;
+                    for (uint32_t j = 0; j < 8; ++j) {
+                        const uint32_t bx = j & 1, by = (j >> 1) & 1, bz = (j >> 2) & 1;
+                        // vertex of the element (k_da_nlist), level md
+                        check(x + bx * sz, y + by * sz, z + bz * sz, d, md, "vertex");
+                        // vertex of the parent (k_da_nlist), level md
+                        check(parX + bx * (sz << 1), parY + by * (sz << 1), parZ + bz * (sz << 1), d - 1, md, "parent vertex");
+                    }
+                    if (d >= 2) {
+                        // parents of the same-level neighbours (k_da_flag_nodes), level d - 1
+                        const uint32_t nb[3] = {x >= sz ? x - sz : x, y >= sz ? y - sz : y, z >= sz ? z - sz : z};
+                        for (uint32_t j = 1; j < 8; ++j) {
+                            const uint32_t a = j & 1 ? nb[0] : x, b = j & 2 ? nb[1] : y, c = j & 4 ? nb[2] : z;
+                            check(a >> ps << ps, b >> ps << ps, c >> ps << ps, d - 1, d - 1, "neighbour parent");
+                        }
+                    }
+                    // points of a coarser grid anywhere in the domain, both target levels
+                    const uint32_t g = d - (uint32_t)(rng() % (d + 1)), gs = md - g;
+                    const uint32_t qx = gs >= 32 ? 0 : ((uint32_t)rng() & dom) >> gs << gs;
+                    const uint32_t qy = gs >= 32 ? 0 : ((uint32_t)rng() & dom) >> gs << gs;
+                    const uint32_t qz = gs >= 32 ? 0 : ((uint32_t)rng() & dom) >> gs << gs;
+                    check(qx, qy, qz, g, g, "grid point");
+                    check(qx, qy, qz, g, md, "grid point at md");
+                }
+        }
     }
     printf("%ld cases, %ld failures\n", n, bad);
     return bad ? 1 : 0;
