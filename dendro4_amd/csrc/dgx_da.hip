@@ -91,8 +91,32 @@ __global__ void __launch_bounds__(B) k_da_decode(const Key *k, size_t n, size_t 
     flags[i] = (uint8_t)(l | (i >= nelem ? F_BOUNDARY : 0));
 }
 
-__device__ inline bool bsearch_exact(const Key *a, size_t n, const Key &k) {
+// Search ranges of the sorted local keys by their cell at level 6 (the high
+// word's bits 62..45): every key whose first 18 bits are c lies in
+// [cstart[c], cstart[c + 1]), keys before cstart[c] are smaller than any key
+// of cell c and keys from cstart[c + 1] on larger.  A search starts from its
+// key's cell instead of the whole array and finds the same index.
+constexpr uint32_t kCellBits = 18, kCellShift = 63 - kCellBits;
+__device__ inline uint32_t cell_of(const Key &k) { return (uint32_t)(k.hi >> kCellShift); }
+__global__ void k_cell_starts(const Key *k, size_t n, uint32_t *cstart) {
+    const uint32_t c = blockIdx.x * B + threadIdx.x;
+    if (c > (1u << kCellBits)) return;
+    if (c == (1u << kCellBits)) {  // past every key (bit 63 of the high word is 0)
+        cstart[c] = (uint32_t)n;
+        return;
+    }
+    const uint64_t h = (uint64_t)c << kCellShift;  // first key whose high word is at least h
     size_t lo = 0, hi = n;
+    while (lo < hi) {
+        const size_t mid = lo + (hi - lo) / 2;
+        if (k[mid].hi < h) lo = mid + 1;
+        else hi = mid;
+    }
+    cstart[c] = (uint32_t)lo;
+}
+__device__ inline bool bsearch_exact(const Key *a, size_t n, const uint32_t *cstart, const Key &k) {
+    const uint32_t c = cell_of(k);
+    size_t lo = cstart[c], hi = cstart[c + 1];
     while (lo < hi) {
         size_t mid = lo + (hi - lo) / 2;
         if (key_lt(a[mid], k)) lo = mid + 1; else hi = mid;
@@ -100,8 +124,9 @@ __device__ inline bool bsearch_exact(const Key *a, size_t n, const Key &k) {
     return lo < n && key_eq(a[lo], k);
 }
 // seq::maxLowerBound (seqUtils.tcc:59-100): greatest a[i] <= k.
-__device__ inline bool max_lower_bound(const Key *a, size_t n, const Key &k, size_t &idx) {
-    size_t lo = 0, hi = n;
+__device__ inline bool max_lower_bound(const Key *a, size_t n, const uint32_t *cstart, const Key &k, size_t &idx) {
+    const uint32_t c = cell_of(k);
+    size_t lo = cstart[c], hi = cstart[c + 1];
     while (lo < hi) {
         size_t mid = lo + (hi - lo) / 2;
         if (key_le(a[mid], k)) lo = mid + 1; else hi = mid;
@@ -114,9 +139,9 @@ __device__ inline bool max_lower_bound(const Key *a, size_t n, const Key &k, siz
 // The probes' keys come from the octant's own curve path (path_walk once,
 // then key_from_path: only the levels below the first one at which the probe
 // leaves the octant's ancestors are walked) instead of a full encode each.
-__global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uint32_t md, SfcPacked tab,
-                                                     const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
-                                                     uint8_t *flags) {
+__global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, const uint32_t *__restrict__ cstart,
+                                                     uint32_t md, SfcPacked tab, const uint32_t *xs,
+                                                     const uint32_t *ys, const uint32_t *zs, uint8_t *flags) {
     LOAD_TABLES(tab)
     __shared__ uint8_t s_rot[kTailCols * B];  // rotation states of each thread's path, levels 1..32
     size_t i = (size_t)blockIdx.x * B + threadIdx.x;
@@ -133,7 +158,7 @@ __global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uin
             uint32_t ps = md - lev + 1;
             Key pk = key_from_path(s_rc, kref, rots, B, X, Y, Z, (a >> ps) << ps, (b >> ps) << ps, (c >> ps) << ps,
                                    lev - 1, lev - 1, md, nullptr);
-            if (bsearch_exact(k, n, pk)) hanging = true;
+            if (bsearch_exact(k, n, cstart, pk)) hanging = true;
         };
         switch (cn) {
             case 1: if (Y) probe(X, Y - s, Z); if (Z) probe(X, Y, Z - s); if (Y && Z) probe(X, Y - s, Z - s); break;
@@ -152,7 +177,8 @@ __global__ void __launch_bounds__(B) k_da_flag_nodes(const Key *k, size_t n, uin
 // The vertex keys come from the element's curve path (path_walk once, then
 // key_from_path with the zero-digit tails of the level-md keys in `tail`,
 // k_zero_tail) instead of a 31-level encode each.
-__global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t nelem, uint32_t md, SfcPacked tab,
+__global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, const uint32_t *__restrict__ cstart,
+                                                size_t nelem, uint32_t md, SfcPacked tab,
                                                 const uint32_t *xs, const uint32_t *ys, const uint32_t *zs,
                                                 const uint8_t *flags, const Key *__restrict__ tail, uint32_t *nlist,
                                                 uint8_t *mask,
@@ -178,7 +204,7 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
             uint32_t r = (uint32_t)i;
             if (!inode) {
                 Key pk = key_from_path(s_rc, kref, rots, B, x, y, z, wx, wy, wz, d - 1, md, md, tail);
-                found = max_lower_bound(k, n, pk, idx);
+                found = max_lower_bound(k, n, cstart, pk, idx);
                 if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) r = (uint32_t)idx;
             }
             nl[0] = r;
@@ -186,7 +212,7 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
         }
         Key sk = key_from_path(s_rc, kref, rots, B, x, y, z, vx, vy, vz, d, md, md, tail);
         uint32_t sx = vx, sy = vy, szz = vz, lastLevel = d;
-        found = max_lower_bound(k, n, sk, idx);
+        found = max_lower_bound(k, n, cstart, sk, idx);
         if (found && !(key_is_ancestor(k[idx], sk) || key_eq(k[idx], sk))) found = false;
         bool secondary = true;
         if (found && xs[idx] == vx && ys[idx] == vy && zs[idx] == vz && (flags[idx] & F_NODE)) {
@@ -196,7 +222,7 @@ __global__ void __launch_bounds__(B) k_da_nlist(const Key *k, size_t n, size_t n
         if (!secondary) continue;
         sx = wx; sy = wy; szz = wz; lastLevel = d - 1;
         Key pk = key_from_path(s_rc, kref, rots, B, x, y, z, wx, wy, wz, d - 1, md, md, tail);
-        found = max_lower_bound(k, n, pk, idx);
+        found = max_lower_bound(k, n, cstart, pk, idx);
         if (found && xs[idx] == wx && ys[idx] == wy && zs[idx] == wz && (flags[idx] & F_NODE)) {
             nl[j] = (uint32_t)idx;
             continue;
@@ -356,7 +382,10 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     da->flags.alloc(N, s);
     k_da_decode<<<grid_for(N, B), B, 0, s>>>(da->key.p, N, n, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p);
     // Part2
-    k_da_flag_nodes<<<grid_for(N, B), B, 0, s>>>(da->key.p, N, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p);
+    DevBuf<uint32_t> cstart((1u << kCellBits) + 1, s);
+    k_cell_starts<<<grid_for((1u << kCellBits) + 1, B), B, 0, s>>>(da->key.p, N, cstart.p);
+    k_da_flag_nodes<<<grid_for(N, B), B, 0, s>>>(da->key.p, N, cstart.p, md, tab, da->x.p, da->y.p, da->z.p,
+                                                 da->flags.p);
     HIP_OK(hipGetLastError());
     // Part3: node list
     da->nlist.alloc(8 * n, s);
@@ -365,8 +394,8 @@ std::unique_ptr<DaDevice> da_build(const Key *oct, size_t n, uint32_t md0, int c
     HIP_OK(hipMemsetAsync(ctr.p, 0, 16, s));
     DevBuf<Key> tail(24 * kTailCols, s);
     k_zero_tail<<<1, B, 0, s>>>(tab, md, tail.p);
-    k_da_nlist<<<grid_for(n, B), B, 0, s>>>(da->key.p, N, n, md, tab, da->x.p, da->y.p, da->z.p, da->flags.p,
-                                           tail.p, da->nlist.p, da->mask.p, ctr.p);
+    k_da_nlist<<<grid_for(n, B), B, 0, s>>>(da->key.p, N, cstart.p, n, md, tab, da->x.p, da->y.p, da->z.p,
+                                           da->flags.p, tail.p, da->nlist.p, da->mask.p, ctr.p);
     HIP_OK(hipGetLastError());
     DevBuf<uint8_t> isnode(N, s);
     HIP_OK(hipMemsetAsync(isnode.p, 0, N, s));
