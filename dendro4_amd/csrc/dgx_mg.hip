@@ -142,12 +142,12 @@ __global__ void __launch_bounds__(B) k_prolong_rows(uint32_t nf, const uint32_t 
         ++o;
     }
 }
-__global__ void k_prolong_keys(const uint32_t *pcol, uint32_t nf, Key *keys, uint32_t *vals, uint32_t *cnt) {
-    uint32_t n = blockIdx.x * B + threadIdx.x;
-    uint32_t nloc = 0;  // rows are filled from slot 0; NONE ends them
-    if (n < nf)
-        while (nloc < 8 && pcol[(size_t)nloc * nf + n] != NONE) ++nloc;
-    uint32_t base = wave_reserve(cnt, nloc);  // every lane of the wave reaches this call
+// (coarse node, fine node) pairs of P's entries, row n's at P's own row
+// offsets poff[n] (the rows are filled from slot 0 and NONE ends them)
+__global__ void k_prolong_keys(const uint32_t *pcol, const uint32_t *poff, uint32_t nf, Key *keys, uint32_t *vals) {
+    const uint32_t n = blockIdx.x * B + threadIdx.x;
+    if (n >= nf) return;
+    const uint32_t base = poff[n], nloc = poff[n + 1] - base;
     for (uint32_t k = 0; k < nloc; ++k) {
         keys[base + k] = Key{(uint64_t)pcol[(size_t)k * nf + n], (uint64_t)n};
         vals[base + k] = k * nf + n;
@@ -366,11 +366,12 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
                                                 c.x.p, c.y.p, c.z.p, c.flags.p, c.conn.p, c.meta.p, c.bdy_node.p,
                                                 md0 + 1, make_packed(curve), ecol.p, eval.p, err.p);
     HIP_OK(hipGetLastError());
+    uint64_t pn = 0;
     {
         DevBuf<uint32_t> cnt(nf, s);
         k_ell_count<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, cnt.p);
         t.poff.alloc((size_t)nf + 1, s);
-        const uint64_t pn = scan_exclusive_u32(cnt.p, t.poff.p, nf, s);
+        pn = scan_exclusive_u32(cnt.p, t.poff.p, nf, s);
         set_u32_async(t.poff.p + nf, (uint32_t)pn, s);
         t.pcol.alloc(pn, s);
         t.pval.alloc(pn, s);
@@ -379,12 +380,11 @@ static void build_transfer(MgLevel &fine, MgLevel &coarse, uint32_t md0, int cur
         HIP_OK(hipGetLastError());
     }
     // transpose into CSR over coarse nodes
-    DevBuf<Key> keys(8 * (size_t)nf, s);
-    DevBuf<uint32_t> vals(8 * (size_t)nf, s), cnt(1, s);
-    HIP_OK(hipMemsetAsync(cnt.p, 0, 4, s));
-    k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(ecol.p, nf, keys.p, vals.p, cnt.p);
-    uint32_t hcnt = 0, herr = 0;
-    dgx_read_back(&hcnt, cnt.p, 4, s);
+    DevBuf<Key> keys(pn, s);
+    DevBuf<uint32_t> vals(pn, s);
+    k_prolong_keys<<<grid_for(nf, B), B, 0, s>>>(ecol.p, t.poff.p, nf, keys.p, vals.p);
+    const uint32_t hcnt = (uint32_t)pn;
+    uint32_t herr = 0;
     dgx_read_back(&herr, err.p, 4, s);
     if (herr) {
         dgx_set_error("multigrid: interpolation lookup failed (code %u)", herr);
