@@ -261,9 +261,11 @@ def serve_in_child(dgx, count, slots):
     cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serve-only", "--serve-requests", str(count),
            "--serve-slots", str(slots)]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"served stream process failed with status {r.returncode}")
-    out = json.loads(r.stdout.strip().splitlines()[-1])
+    lines = r.stdout.strip().splitlines()
+    if r.returncode != 0 or not lines:
+        log(f"serve: the served-stream process failed with status {r.returncode}")
+        return {"error": f"served-stream process exited with status {r.returncode}", "process": "own process"}
+    out = json.loads(lines[-1])
     out["process"] = "own process"
     return out
 
