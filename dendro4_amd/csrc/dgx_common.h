@@ -292,16 +292,6 @@ __device__ inline uint32_t lookback_prefix(uint64_t *status, uint32_t tile, uint
     __hip_atomic_store(mine, LB_INCL | ep | (excl + value), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return excl;
 }
-// Reserve n slots on *counter for this lane; returns the first slot.  Every
-// lane of the wave must call it (inactive lanes pass 0).
-__device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
-    uint32_t total, ex = wave_excl_sum(n, total);
-    uint32_t base = 0;
-    if (__lane_id() == 63 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 63, 64);
-    return base + ex;
-}
-
 // Reserve n slots on *counter for this thread with one atomic per workgroup;
 // returns the first slot.  Every thread of the workgroup must call it; lds
 // holds blockDim.x / 64 + 1 entries.
