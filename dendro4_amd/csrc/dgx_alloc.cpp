@@ -189,12 +189,21 @@ void dgx_free(void *p, hipStream_t s) {
     g_cached_bytes += cls;
 }
 
+// Blocks cached for a stream inside a graph capture stay: a free queued there
+// would join the capture.
 void dgx_alloc_trim() {
     std::lock_guard<std::mutex> lk(g_mu);
-    for (auto &m : g_cache)
-        for (auto &kv : m.second) (void)hipFreeAsync(kv.second, m.first);
-    g_cache.clear();
-    g_cached_bytes = 0;
+    for (auto m = g_cache.begin(); m != g_cache.end();) {
+        if (g_capturing.count(m->first)) {
+            ++m;
+            continue;
+        }
+        for (auto &kv : m->second) {
+            (void)hipFreeAsync(kv.second, m->first);
+            g_cached_bytes -= kv.first;
+        }
+        m = g_cache.erase(m);
+    }
 }
 
 size_t dgx_alloc_cached_bytes() {

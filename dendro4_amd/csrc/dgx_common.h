@@ -375,7 +375,7 @@ struct DgxError {};
 // front of the library's HIP memory pool.
 void *dgx_alloc(size_t bytes, hipStream_t s);
 void dgx_free(void *p, hipStream_t s);
-void dgx_alloc_trim();            // return every cached block to the pool
+void dgx_alloc_trim();            // return the cached blocks to the pool (not those of capturing streams)
 size_t dgx_alloc_cached_bytes();  // bytes held in the cache
 // Bytes of the library's device allocations in use, and their peak since the
 // process started or the last reset.

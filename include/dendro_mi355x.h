@@ -48,9 +48,10 @@ int dgx_memset_device(void *dst, int value, size_t bytes);
  * use, [3] their peak since start or the last reset (reset_peak != 0 resets it
  * after reading), [4] bytes the library's block cache holds. */
 int dgx_memory_info(uint64_t out[5], int reset_peak);
-/* No reference counterpart: returns every block the library's block cache
- * holds to the device pool, waits for the device, and releases the pool's
- * unused memory to the device. */
+/* No reference counterpart: returns the blocks the library's block cache
+ * holds to the device pool (except those cached for a stream inside a graph
+ * capture), waits for the device, and releases the pool's unused memory to
+ * the device. */
 int dgx_trim_cache(void);
 
 /* ---- octree ------------------------------------------------------------- */
