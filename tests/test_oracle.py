@@ -384,6 +384,39 @@ def test_da_adaptive_geometry(oracle, seed):
         assert lib().or_elem_type(cn, int(da.mask[e])) >= 0
 
 
+@pytest.mark.parametrize("hilbert", [True, False])
+@pytest.mark.parametrize("seed", [0, 5])
+def test_da_buffer_order_and_node_numbering(oracle, hilbert, seed):
+    """The local buffer and the compact node numbering on adaptive meshes.
+    The reference sorts the local buffer (the elements and the boundary
+    pseudo-octants, odaFactory.cpp), so the buffer must be strictly increasing
+    under the comparator pinned above, elements first.  It numbers the compact
+    nodes by walking that buffer (oda.cpp:853-882): the NODE octants among the
+    elements, then the pseudo-octants flagged NODE and BOUNDARY (flags 128 and
+    64 of the level byte, TreeNode.h:80), in buffer order.  The GPU DA is
+    compared with these maps bit for bit."""
+    oracle.set_ordering(hilbert)
+    try:
+        pts = np.random.default_rng(seed).random((400, 3)) ** 2
+        b = oracle.balance(oracle.points2octree(pts, 30, 1))
+        da = oracle.DA(b)
+        o, E = da.octs, da.n_elem
+        key = o.copy()
+        key[:, 3] &= 31
+        assert np.array_equal(oracle.sort(key), key)
+        assert len(np.unique(key[:, :4], axis=0)) == len(key)
+        f = (o[:, 3] & 0xFF).astype(np.int64)
+        node, bnd = (f & 128) != 0, (f & 64) != 0
+        assert not bnd[:E].any() and bnd[E:].all()
+        want = np.full(da.n_local, 0xFFFFFFFF, np.uint32)
+        sel = np.concatenate([np.nonzero(node[:E])[0], E + np.nonzero(node[E:] & bnd[E:])[0]])
+        want[sel] = np.arange(len(sel), dtype=np.uint32)
+        assert np.array_equal(da.local_to_node, want)
+        assert da.node_size == int(node[:E].sum()) and da.n_nodes == len(sel)
+    finally:
+        oracle.set_ordering(True)
+
+
 def test_oracle_matvec_invariants(oracle, stencils):
     L, M = stencils
     pts = np.random.default_rng(3).random((2000, 3)) ** 2
